@@ -1,0 +1,151 @@
+/*
+ * C ABI of the MI355X PETL projector library (libpetl.so).
+ *
+ * Part 1 keeps every export of the reference's src/petl_c_interface.h:14-97 with the same name, C
+ * signature and bool conventions, so the reference's ctypes bindings (src/petl_server.py) load this
+ * library unchanged.  Each declaration cites the reference line it replaces.  Exports that belong to
+ * subsystems outside the projector / MLEM path (list-mode binning, phantoms, ray tracing, FBP/PFDR, TV
+ * denoising, Gaussian filtering) are present so symbol lookup succeeds, but they print an error and
+ * return false / 0.
+ *
+ * Part 2 adds entry points the reference does not have: device-resident projection on caller-owned
+ * device buffers (torch tensors), back projector mode selection, and a device-resident MLEM/OSEM
+ * reconstruction.  Host-pointer calls are synchronous, like the reference's.
+ *
+ * Threading: like the reference, calls act on the model selected by set_model / create_new_model,
+ * which is process-global state; do not call concurrently from several threads.
+ */
+#ifndef PETL_C_INTERFACE_H
+#define PETL_C_INTERFACE_H
+
+#ifdef __cplusplus
+#include <cstdint>
+#define PETL_EXTERN extern "C"
+#else
+#include <stdbool.h>
+#include <stdint.h>
+#define PETL_EXTERN
+#endif
+
+/* ------------------------------------------------------------------------------------------------
+ * Part 1: reference ABI (src/petl_c_interface.h)
+ * ---------------------------------------------------------------------------------------------- */
+PETL_EXTERN bool set_model(int);                 /* src/petl_c_interface.h:14 */
+PETL_EXTERN int create_new_model();              /* :15 */
+PETL_EXTERN void about();                        /* :16 */
+PETL_EXTERN bool copy_parameters(int);           /* :18 */
+PETL_EXTERN void clearAll();                     /* :20 */
+PETL_EXTERN void print_parameters();             /* :22 */
+PETL_EXTERN bool volume_defined();               /* :23 */
+PETL_EXTERN bool geometry_defined();             /* :24 */
+
+PETL_EXTERN int get_numX();                      /* :26 */
+PETL_EXTERN bool set_numX(int);                  /* :27 */
+PETL_EXTERN int get_numY();                      /* :28 */
+PETL_EXTERN bool set_numY(int);                  /* :29 */
+PETL_EXTERN int get_numZ();                      /* :30 */
+PETL_EXTERN bool set_numZ(int);                  /* :31 */
+PETL_EXTERN float get_voxelWidth();              /* :33 */
+PETL_EXTERN bool set_voxelWidth(float);          /* :34 */
+PETL_EXTERN float get_voxelHeight();             /* :35 */
+PETL_EXTERN bool set_voxelHeight(float);         /* :36 */
+PETL_EXTERN float get_offsetX();                 /* :38 */
+PETL_EXTERN bool set_offsetX(float);             /* :39  (rejects values <= 0, as the reference) */
+PETL_EXTERN float get_offsetY();                 /* :40 */
+PETL_EXTERN bool set_offsetY(float);             /* :41 */
+PETL_EXTERN float get_offsetZ();                 /* :42 */
+PETL_EXTERN bool set_offsetZ(float);             /* :43 */
+
+PETL_EXTERN int get_numPlanograms();                                        /* :45 */
+PETL_EXTERN bool get_planogramSize(int which, int* shape);                  /* :46  shape = {N_v1,N_v0,N_u1,N_u0} */
+/* psi in degrees, full panel separation D, full length L, full height H (the reference halves them) */
+PETL_EXTERN bool add_planogram(float psi, float R, float L, float H, float v_m0, float v_m1, float T); /* :48 */
+PETL_EXTERN bool remove_planogram(int);                                     /* :49 */
+PETL_EXTERN bool keep_only_planogram(int);                                  /* :50 */
+PETL_EXTERN bool set_default_volume(float scale);                           /* :52 */
+PETL_EXTERN bool set_volume(int numX, int numY, int numZ, float voxelWidth, float voxelHeight, float offsetX,
+                            float offsetY, float offsetZ);                  /* :53 */
+
+/* Forward projection: g[i] = A_i f for every planogram i, overwriting all bins (masked bins -> 0).
+ * g: array of host pointers to C-contiguous float32 [N_v1][N_v0][N_u1][N_u0]; f: [numZ][numY][numX]. */
+PETL_EXTERN bool project(float** g, float* f);                              /* :59 */
+/* Back projection: f = sum_i A_i^T g[i], overwriting f.  Semantics follow the selected back projector
+ * mode (set_backprojector_mode), PETL (reference) mode by default. */
+PETL_EXTERN bool backproject(float** g, float* f);                          /* :60 */
+
+/* elementwise host operations with the reference's zero conventions (src/data_cube.cpp:97-167) */
+PETL_EXTERN bool multiply3D(float* x, float* y, int N_1, int N_2, int N_3);               /* :75 */
+PETL_EXTERN bool multiply4D(float* x, float* y, int N_1, int N_2, int N_3, int N_4);      /* :76 */
+PETL_EXTERN bool divide3D(float* x, float* y, int N_1, int N_2, int N_3);                 /* :78  y==0 -> 0 */
+PETL_EXTERN bool divide4D(float* x, float* y, int N_1, int N_2, int N_3, int N_4);        /* :79 */
+PETL_EXTERN bool rdivide3D(float* num, float* denom, int N_1, int N_2, int N_3);          /* :81  denom = num/denom */
+PETL_EXTERN bool rdivide4D(float* num, float* denom, int N_1, int N_2, int N_3, int N_4); /* :82 */
+PETL_EXTERN bool reciprocal3D(float* x, int N_1, int N_2, int N_3, float divide_by_zero_value);          /* :84 */
+PETL_EXTERN bool reciprocal4D(float* x, int N_1, int N_2, int N_3, int N_4, float divide_by_zero_value); /* :85 */
+PETL_EXTERN float inner_product3D(float* x, float* y, float* w, int N_1, int N_2, int N_3);             /* :87 */
+PETL_EXTERN float inner_product4D(float* x, float* y, float* w, int N_1, int N_2, int N_3, int N_4);    /* :88 */
+
+/* prior gradients (3D arrays [N_1][N_2][N_3]) */
+PETL_EXTERN bool TV_gradient(float* f, float* Df, int N_1, int N_2, int N_3, float delta, float beta, float p,
+                             bool data_on_cpu);                                                   /* :92 */
+PETL_EXTERN bool relativeDifferences_gradient(float* f, float* Df, int N_1, int N_2, int N_3, float delta,
+                                              float beta);                                        /* :96 */
+
+/* exports outside the projector / MLEM path: present, return false / 0 with a message */
+PETL_EXTERN bool bin(float** g, char* file_name);                                          /* :55 */
+PETL_EXTERN bool simulate_scatter(float** g, float* mu);                                   /* :57 */
+PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
+PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
+PETL_EXTERN bool add_object(int type, float* c, float* r, float val, float* A, float* clip); /* :64 */
+PETL_EXTERN bool add_module(int type, float* c, float* r, float val, float* A, float* clip); /* :65 */
+PETL_EXTERN bool clear_phantom();                                                          /* :66 */
+PETL_EXTERN bool clear_modules();                                                          /* :67 */
+PETL_EXTERN bool scale_phantom(float, float, float);                                       /* :68 */
+PETL_EXTERN bool voxelize(float* f, int oversampling);                                     /* :69 */
+PETL_EXTERN bool ray_trace(float** g, int oversampling);                                   /* :70 */
+PETL_EXTERN bool stopping_power(float** g, int oversampling);                              /* :71 */
+PETL_EXTERN bool set_solid_angle_correction(float** g, bool do_inverse);                   /* :72 */
+PETL_EXTERN bool apply_corrections(float** g, float** r, float threshold);                 /* :73 */
+PETL_EXTERN float TV_cost(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p,
+                          bool data_on_cpu);                                               /* :91 */
+PETL_EXTERN float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, float delta, float beta, float p,
+                              bool data_on_cpu);                                           /* :93 */
+PETL_EXTERN bool diffuse(float* f, int N_1, int N_2, int N_3, float delta, float p, int numIter,
+                         bool data_on_cpu);                                                /* :94 */
+PETL_EXTERN bool TV_denoise(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p, int numIter,
+                            bool doMean, bool data_on_cpu);                                /* :95 */
+PETL_EXTERN bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims); /* :97 */
+
+/* ------------------------------------------------------------------------------------------------
+ * Part 2: additions (no counterpart in the reference)
+ * ---------------------------------------------------------------------------------------------- */
+/* Back projector semantics: 0 = PETL (the reference's backproject_planogram: voxel-centre u1 test, no u0
+ * mask; the default), 1 = matched (exact transpose of project, needed for <Ax,y> = <x,A^T y> and for
+ * gradients). */
+PETL_EXTERN bool set_backprojector_mode(int mode);
+PETL_EXTERN int get_backprojector_mode();
+/* Kernel family: 0 = automatic (fast separable kernels when the geometry allows, else generic),
+ * 1 = generic kernels only. */
+PETL_EXTERN bool set_projector_path(int path);
+PETL_EXTERN int get_projector_path();
+/* 1 when planogram `which` runs on the fast separable kernels under the current settings. */
+PETL_EXTERN int planogram_uses_fast_path(int which);
+
+/* Device-resident forward / back projection on caller-owned device buffers (e.g. torch tensors).
+ * d_g: host array of get_numPlanograms() device pointers; d_f: device pointer; stream: hipStream_t or
+ * NULL for the library's stream.  Work is enqueued on the stream; the call returns after enqueueing
+ * unless sync is true.  backproject_gpu overwrites d_f. */
+PETL_EXTERN bool project_gpu(float** d_g, float* d_f, void* stream, bool sync);
+PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync);
+
+/* Device-resident MLEM / OSEM (the loop of src/petl_server.py:1534-1606 with every step on the GPU).
+ * g: host planograms (data); f: host volume, used as the initial estimate (all zeros -> ones, negatives
+ * clipped) and overwritten with the result.  prior: 0 none, 1 relative differences (OSL, as the
+ * reference's beta > 0 branch), 2 p-Huber TV (OSL).  mask / response: optional host planogram lists
+ * (NULL for none), used as PETLserver.MLEM uses them.  numSubsets > 1 runs OSEM over interleaved v1
+ * subsets.  loglik: optional host array of numIter doubles receiving sum(y log Af - Af) of each
+ * iteration's forward projection (bins with Af > 0). */
+PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
+                          int numSubsets, float** mask, float** response, double* loglik);
+
+#endif /* PETL_C_INTERFACE_H */

@@ -1,0 +1,254 @@
+"""CPU oracle loader -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings for ``libpetl_oracle.so`` (the C++ restatement of the reference hot path, see
+petl_oracle.cpp) and for ``_ref/libpetl_ref_geom.so`` (the reference's own planogram/dataCube classes,
+built from the reference sources when they are present).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg import this module, as the checker.  The product package never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libpetl_oracle.so")
+_REF = os.path.join(_HERE, "_ref", "libpetl_ref_geom.so")
+
+
+class Plan(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("psi", "R", "L", "H", "v_m0", "v_m1")] + \
+               [(n, ctypes.c_int) for n in ("N_u0", "N_u1", "N_v0", "N_v1")] + \
+               [(n, ctypes.c_float) for n in ("T_u0", "T_u1", "T_v0", "T_v1", "u0_0", "u1_0", "v0_0",
+                                               "v1_0", "cos_psi", "sin_psi")]
+
+    @property
+    def shape(self):
+        return (self.N_v1, self.N_v0, self.N_u1, self.N_u0)
+
+
+class Vol(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("numX", "numY", "numZ")] + \
+               [(n, ctypes.c_float) for n in ("voxelWidth", "voxelHeight", "offsetX", "offsetY", "offsetZ",
+                                               "x0", "y0", "z0")]
+
+    @property
+    def shape(self):
+        return (self.numZ, self.numY, self.numX)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        P, V = ctypes.POINTER(Plan), ctypes.POINTER(Vol)
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
+        L.orc_plan_init.argtypes = [c_float] * 7 + [P]
+        L.orc_plan_init.restype = c_int
+        L.orc_vol_init.argtypes = [c_int] * 3 + [c_float] * 5 + [V]
+        L.orc_bin_masked.argtypes = [P, c_int, c_int, c_int, c_int]
+        L.orc_bin_masked.restype = c_int
+        L.orc_project_f32.argtypes = [P, V, fp, fp, c_int]
+        L.orc_project_f64.argtypes = [P, V, dp, dp, c_int]
+        L.orc_backproject_f32.argtypes = [P, V, fp, fp, c_int, c_int]
+        L.orc_backproject_f64.argtypes = [P, V, dp, dp, c_int, c_int]
+        L.orc_bin_taps.argtypes = [P, V, c_int, c_int, c_int, c_int, i64p, dp, c_int,
+                                   ctypes.POINTER(c_double)]
+        L.orc_bin_taps.restype = c_int
+        L.orc_rd_gradient_f32.argtypes = [fp, fp, c_int, c_int, c_int, c_float, c_float]
+        L.orc_rd_gradient_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_double, c_double]
+        L.orc_rd_cost_f64.argtypes = [dp, c_int, c_int, c_int, c_double, c_double]
+        L.orc_rd_cost_f64.restype = c_double
+        L.orc_tv_gradient_f32.argtypes = [fp, fp, c_int, c_int, c_int, c_float, c_float, c_float]
+        L.orc_tv_gradient_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_float, c_float]
+        L.orc_tv_cost_f64.argtypes = [dp, c_int, c_int, c_int, c_float, c_float, c_float]
+        L.orc_tv_cost_f64.restype = c_double
+        L.orc_max_threads.restype = c_int
+        _lib = L
+    return _lib
+
+
+def nthreads():
+    return lib().orc_max_threads()
+
+
+# --------------------------------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------------------------------
+def planogram(psi, D, L, H, T, v_m0=None, v_m1=None):
+    """Planogram sampling as produced by PETLserver.add_planogram (src/petl_server.py:484-507)."""
+    if v_m0 is None:
+        v_m0 = 1.0
+    if v_m1 is None:
+        v_m1 = min(1.0, (H - 1.5 * T) / D)
+    p = Plan()
+    if not lib().orc_plan_init(psi, D, L, H, v_m0, v_m1, T, ctypes.byref(p)):
+        raise ValueError("invalid planogram")
+    return p
+
+
+def volume(numX, numY, numZ, voxelWidth, voxelHeight=0.0, offsetX=0.0, offsetY=0.0, offsetZ=0.0):
+    if voxelHeight == 0.0:
+        voxelHeight = voxelWidth
+    v = Vol()
+    lib().orc_vol_init(numX, numY, numZ, voxelWidth, voxelHeight, offsetX, offsetY, offsetZ, ctypes.byref(v))
+    return v
+
+
+# --------------------------------------------------------------------------------------------------
+# projectors (lists of planograms, like PETLserver)
+# --------------------------------------------------------------------------------------------------
+def _nt(n):
+    return nthreads() if n is None else n
+
+
+def project(plans, vol, f, dtype=np.float32, nthreads=None):
+    f = np.ascontiguousarray(f, dtype=dtype)
+    fn = lib().orc_project_f32 if dtype == np.float32 else lib().orc_project_f64
+    out = []
+    for p in plans:
+        g = np.zeros(p.shape, dtype=dtype)
+        fn(ctypes.byref(p), ctypes.byref(vol), f, g, _nt(nthreads))
+        out.append(g)
+    return out
+
+
+def backproject(plans, vol, g, mode="petl", dtype=np.float32, nthreads=None):
+    """mode 'petl' = reference back projector semantics, 'matched' = exact transpose of project."""
+    fn = lib().orc_backproject_f32 if dtype == np.float32 else lib().orc_backproject_f64
+    f = np.zeros(vol.shape, dtype=dtype)
+    m = 1 if mode == "matched" else 0
+    for p, gi in zip(plans, g):
+        fn(ctypes.byref(p), ctypes.byref(vol), np.ascontiguousarray(gi, dtype=dtype), f, m, _nt(nthreads))
+    return f
+
+
+def bin_taps(plan, vol, iv1, iv0, iu1, iu0, cap=1 << 16):
+    idx = np.zeros(cap, np.int64)
+    w = np.zeros(cap, np.float64)
+    scale = ctypes.c_double()
+    n = lib().orc_bin_taps(ctypes.byref(plan), ctypes.byref(vol), iv1, iv0, iu1, iu0, idx, w, cap,
+                           ctypes.byref(scale))
+    return idx[:n], w[:n], scale.value
+
+
+def bin_masked(plan, iv1, iv0, iu1, iu0):
+    return bool(lib().orc_bin_masked(ctypes.byref(plan), iv1, iv0, iu1, iu0))
+
+
+# --------------------------------------------------------------------------------------------------
+# priors
+# --------------------------------------------------------------------------------------------------
+def rd_gradient(f, delta, beta, dtype=np.float32):
+    f = np.ascontiguousarray(f, dtype=dtype)
+    D = np.zeros_like(f)
+    N1, N2, N3 = f.shape
+    (lib().orc_rd_gradient_f32 if dtype == np.float32 else lib().orc_rd_gradient_f64)(
+        f, D, N1, N2, N3, delta, beta)
+    return D
+
+
+def rd_cost(f, delta, beta):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_rd_cost_f64(f, *f.shape, delta, beta)
+
+
+def tv_gradient(f, delta, beta, p=1.0, dtype=np.float32):
+    f = np.ascontiguousarray(f, dtype=dtype)
+    D = np.zeros_like(f)
+    N1, N2, N3 = f.shape
+    (lib().orc_tv_gradient_f32 if dtype == np.float32 else lib().orc_tv_gradient_f64)(
+        f, D, N1, N2, N3, delta, beta, p)
+    return D
+
+
+def tv_cost(f, delta, beta, p=1.0):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_tv_cost_f64(f, *f.shape, delta, beta, p)
+
+
+# --------------------------------------------------------------------------------------------------
+# elementwise semantics (src/data_cube.cpp:144-155) and MLEM (src/petl_server.py:1534-1606)
+# --------------------------------------------------------------------------------------------------
+def divide(x, y):
+    """x / y with y == 0 -> 0 (dataCube DIV)."""
+    out = np.zeros_like(x)
+    nz = y != 0
+    out[nz] = x[nz] / y[nz]
+    return out
+
+
+def rdivide(denom, num):
+    """num / denom with denom == 0 -> 0 (dataCube RDIV)."""
+    return divide(num, denom)
+
+
+def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0, dtype=np.float32,
+         mode="petl", callback=None, nthreads=None):
+    """PETLserver.MLEM restated on the oracle projectors.  prior 'rd' follows the reference
+    (relative differences, OSL); prior 'tv' uses the p-Huber TV gradient in the same OSL slot."""
+    if f is None or not np.any(f):
+        f = np.ones(vol.shape, dtype=dtype)
+    else:
+        f = np.array(f, dtype=dtype)
+        f[f < 0] = 0
+    ones = [np.ones(pl.shape, dtype=dtype) for pl in plans]
+    S = backproject(plans, vol, ones, mode=mode, dtype=dtype, nthreads=nthreads)
+    S[S == 0] = 1
+    for n in range(num_iter):
+        Pd = project(plans, vol, f, dtype=dtype, nthreads=nthreads)
+        r = [rdivide(a, np.asarray(b, dtype=dtype)) for a, b in zip(Pd, g)]
+        d = backproject(plans, vol, r, mode=mode, dtype=dtype, nthreads=nthreads)
+        if beta <= 0.0:
+            f = f * divide(d, S)
+        else:
+            if prior == "rd":
+                G = rd_gradient(f, delta, beta, dtype=dtype)
+            else:
+                G = tv_gradient(f, delta, beta, p, dtype=dtype)
+            G = G + S
+            G = np.maximum(dtype(0.1) * S, G)
+            f = f * divide(d, G)
+        if callback is not None:
+            callback(n, f, Pd)
+    return f
+
+
+# --------------------------------------------------------------------------------------------------
+# the reference's own geometry / dataCube (only where the reference sources could be compiled)
+# --------------------------------------------------------------------------------------------------
+_ref = None
+
+
+def ref_lib():
+    """Returns the reference-built helper library, or None when it was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(_REF):
+            return None
+        R = ctypes.CDLL(_REF)
+        c_int, c_float = ctypes.c_int, ctypes.c_float
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        R.ref_plan_init.argtypes = [c_float] * 7 + [np.ctypeslib.ndpointer(np.int32), fp]
+        R.ref_plan_init.restype = c_int
+        R.ref_binary3d.argtypes = [fp, fp, c_int, c_int, c_int, c_int]
+        R.ref_binary3d.restype = c_int
+        R.ref_reciprocal3d.argtypes = [fp, c_int, c_int, c_int, c_float]
+        R.ref_inner3d.argtypes = [fp, fp, ctypes.c_void_p, c_int, c_int, c_int]
+        R.ref_inner3d.restype = c_float
+        _ref = R
+    return _ref

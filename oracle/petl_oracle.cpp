@@ -1,0 +1,677 @@
+// =====================================================================================================
+//  PETL hot-path CPU ORACLE  --  TEST INFRASTRUCTURE ONLY
+//
+//  This file is a plain C++17/OpenMP restatement of the reference algorithm for the separable-footprint
+//  (SF) planogram projector path.  It exists to CHECK the HIP product path, and to be timed as the CPU
+//  baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+//  only as the checker.  Nothing in petl_amd/ links, loads or calls it.
+//
+//  Every routine is templated on the arithmetic type R:
+//     R = float  : the fp32 restatement ("fp32 oracle").  Built with -ffp-contract=off so every
+//                  multiply/add rounds exactly as written.
+//     R = double : the FP64 oracle used for the precision study.  Geometry parameters stay the float
+//                  values the reference computes; discrete decisions (masks, slice-branch choice) are
+//                  evaluated in float exactly as in the fp32 path, so fp32 and fp64 differ only in
+//                  continuous arithmetic.
+//
+//  Reference files followed (paths relative to the reference root):
+//     planogram geometry          src/planogram.cpp:143-186, src/petl_c_interface.cpp:216-219
+//     volume grid                 src/parameters.cpp:250-284
+//     forward projector           src/projectors.cu:11-238 (project_planogram), :379-438 (project_SF)
+//     back projector (PETL mode)  src/projectors.cu:240-377 (backproject_planogram), :440-497,
+//                                 summed over planograms as in src/petl.cpp:226-277
+//     relative-difference grad    src/relative_differences.cu:21-28, 107-173, 481-532
+//     p-Huber TV gradient         src/total_variation.cu:21-78, 434-660
+//
+//  Deliberate differences from the reference, each documented in DESIGN.md:
+//     * texture fetches are replaced by exact array reads: the forward samples texel centres (exact in
+//       CUDA as well); the backprojector uses exact bilinear weights instead of the hardware's 8-bit
+//       fractional filter weights (parity for that rounding is unpinned),
+//     * rsqrtf (built with --use_fast_math in the reference) is 1/sqrt,
+//     * "matched" back projection mode = exact transpose of this forward (the reference has none).
+// =====================================================================================================
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+extern "C" {
+
+typedef struct orc_plan {
+    float psi, R, L, H, v_m0, v_m1;
+    int N_u0, N_u1, N_v0, N_v1;
+    float T_u0, T_u1, T_v0, T_v1;
+    float u0_0, u1_0, v0_0, v1_0;
+    float cos_psi, sin_psi;
+} orc_plan;
+
+typedef struct orc_vol {
+    int numX, numY, numZ;
+    float voxelWidth, voxelHeight;
+    float offsetX, offsetY, offsetZ;
+    float x0, y0, z0;
+} orc_vol;
+
+}  // extern "C"
+
+namespace {
+
+// ------------------------------------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------------------------------------
+
+// src/petl_c_interface.cpp:216-219 converts degrees and halves D/L/H in double, then
+// src/planogram.cpp:143-186 derives the sampling.  Float/double promotion follows the C++ source.
+int plan_init(float psi_deg, float D, float L, float H, float v_m0, float v_m1_in, float T, orc_plan* o)
+{
+    const float psi = (float)((double)psi_deg * 3.1415926535897932385 / 180.0);
+    const float R = (float)(0.5 * (double)D);
+    const float Lh = (float)(0.5 * (double)L);
+    const float Hh = (float)(0.5 * (double)H);
+    if (R <= 0.0f || Lh <= 0.0f || Hh <= 0.0f || v_m0 <= 0.0f || v_m1_in < 0.0f || T <= 0.0f)
+        return 0;
+    o->psi = psi;
+    o->R = R;
+    o->L = Lh;
+    o->H = Hh;
+    o->v_m0 = v_m0;
+    const float h_over_r = Hh / R;
+    o->v_m1 = (v_m1_in < h_over_r) ? v_m1_in : h_over_r;  // std::min(H/R, v_m1)
+    o->T_u0 = T;
+    o->T_u1 = T;
+    const float Tv0 = T / R;
+    const float Tv1 = T / R;
+    o->N_u0 = 2 * (int)((Lh + R * v_m0) / T);
+    o->N_u1 = 2 * (int)(Hh / T);
+    o->N_v0 = 2 * (int)(v_m0 / Tv0) + 1;
+    o->N_v1 = 2 * (int)(o->v_m1 / Tv1) + 1;
+    o->T_v0 = (float)((double)v_m0 * 2.0 / (double)(float)o->N_v0);  // T_v0 recomputed, T_v1 is not
+    o->T_v1 = Tv1;
+    o->u0_0 = (float)((double)(-T * (float)(o->N_u0 - 1)) / 2.0);
+    o->u1_0 = (float)((double)(-T * (float)(o->N_u1 - 1)) / 2.0);
+    o->v0_0 = (float)((double)(-o->T_v0 * (float)(o->N_v0 - 1)) / 2.0);
+    o->v1_0 = (float)((double)(-o->T_v1 * (float)(o->N_v1 - 1)) / 2.0);
+    o->cos_psi = (float)std::cos((double)psi);
+    o->sin_psi = (float)std::sin((double)psi);
+    return 1;
+}
+
+// src/parameters.cpp:271-284: centre of voxel 0, computed in double and stored as float.
+void vol_init(int nx, int ny, int nz, float vw, float vh, float ox, float oy, float oz, orc_vol* v)
+{
+    v->numX = nx;
+    v->numY = ny;
+    v->numZ = nz;
+    v->voxelWidth = vw;
+    v->voxelHeight = vh;
+    v->offsetX = ox;
+    v->offsetY = oy;
+    v->offsetZ = oz;
+    v->x0 = (float)((0.0 - 0.5 * (nx - 1)) * (double)vw + (double)ox);
+    v->y0 = (float)((0.0 - 0.5 * (ny - 1)) * (double)vw + (double)oy);
+    v->z0 = (float)((0.0 - 0.5 * (nz - 1)) * (double)vh + (double)oz);
+}
+
+inline float coord_f(int i, float T, float start) { return (float)i * T + start; }
+
+// 1/sqrt(1+v0^2+v1^2) of a view (src/projectors.cu:237)
+template <typename R>
+inline R obliquity(const orc_plan& P, int iv1, int iv0)
+{
+    const R v1 = R(iv1) * R(P.T_v1) + R(P.v1_0);
+    const R v0 = R(iv0) * R(P.T_v0) + R(P.v0_0);
+    return R(1) / std::sqrt(R(1) + v0 * v0 + v1 * v1);
+}
+
+// Forward masks of src/projectors.cu:30-43, always evaluated in float.
+inline bool bin_masked(const orc_plan& P, int iv1, int iv0, int iu1, int iu0)
+{
+    const float v1 = coord_f(iv1, P.T_v1, P.v1_0);
+    const float v0 = coord_f(iv0, P.T_v0, P.v0_0);
+    const float u1 = coord_f(iu1, P.T_u1, P.u1_0);
+    const float u0 = coord_f(iu0, P.T_u0, P.u0_0);
+    if (std::fabs(u0) - 0.5f * P.T_u0 > P.L + P.R * std::fabs(v0))
+        return true;
+    if (std::fabs(u1) - 0.5f * P.T_u1 > P.H - P.R * std::fabs(v1))
+        return true;
+    return false;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Forward projector: tap enumeration of one bin.  src/projectors.cu:46-237.
+// The visitor receives (voxel linear index, weight w0*w1) for every tap with positive weights; the
+// bin value is  l_phi * sum(f*w0*w1) / sqrt(1+v0^2+v1^2).  Returning the per-bin scale separately lets
+// the forward and its exact transpose share one enumeration.  Returns l_phi; the caller applies
+// 1/sqrt(1+v0^2+v1^2) (rsqrtf in the reference) after it, in the reference's order.
+// ------------------------------------------------------------------------------------------------
+template <typename R, typename Visit>
+R enumerate_bin(const orc_plan& P, const orc_vol& V, int iv1, int iv0, int iu1, int iu0, Visit&& visit)
+{
+    const R v1 = R(iv1) * R(P.T_v1) + R(P.v1_0);
+    const R v0 = R(iv0) * R(P.T_v0) + R(P.v0_0);
+    const R u1 = R(iu1) * R(P.T_u1) + R(P.u1_0);
+    const R u0 = R(iu0) * R(P.T_u0) + R(P.u0_0);
+    const R cs = R(P.cos_psi), sn = R(P.sin_psi);
+    const R half = R(0.5);
+
+    const R iu0_min = R(iu0) - half, iu0_max = R(iu0) + half;
+    const R iu1_min = R(iu1) - half, iu1_max = R(iu1) + half;
+
+    // p + t*r (src/projectors.cu:52-53)
+    const R px = u0 * cs, py = u0 * sn, pz = u1;
+    const R rx = -(v0 * cs + sn), ry = -(v0 * sn - cs), rz = -v1;
+
+    const R Tx = R(V.voxelWidth), Ty = R(V.voxelWidth), Tz = R(V.voxelHeight);
+    const R x0 = R(V.x0), y0 = R(V.y0), z0 = R(V.z0);
+    const R Tx_inv = R(1) / Tx, Ty_inv = R(1) / Ty, Tz_inv = R(1) / Tz;
+    const R Tu1_inv = R(1) / R(P.T_u1), Tu0_inv = R(1) / R(P.T_u0);
+    const R u0_0 = R(P.u0_0), u1_0 = R(P.u1_0);
+
+    const R width_z = half * R(P.T_u1);
+    const R Tz_over_Tu1 = Tz * Tu1_inv;
+    const R z_fw = half * Tz * Tu1_inv;
+    const int Nx = V.numX, Ny = V.numY, Nz = V.numZ;
+
+    // branch decision in float (src/projectors.cu:77)
+    const float v0f = coord_f(iv0, P.T_v0, P.v0_0);
+    const float rxf = -(v0f * P.cos_psi + P.sin_psi), ryf = -(v0f * P.sin_psi - P.cos_psi);
+
+    if (std::fabs(rxf) >= std::fabs(ryf)) {
+        // x-slices (src/projectors.cu:79-129)
+        const R width_y = half * R(P.T_u0) * std::fabs(sn);
+        const R sin_inv = R(1) / sn;
+        const R rx_inv = R(1) / rx;
+        const R l_phi = Tx * std::fabs(sin_inv);
+        const R y_fw = half * Ty * std::fabs(sin_inv) * Tu0_inv;
+        for (int ix = 0; ix < Nx; ix++) {
+            const R x = R(ix) * Tx + x0;
+            const R t = (x - px) * rx_inv;
+            const R y_c = py + t * ry;
+            const R z_c = pz + t * rz;
+            const R y_A = y_c - width_y, y_B = y_c + width_y;
+            const int iy_min = std::max(0, (int)std::ceil((y_A - half * Ty - y0) * Ty_inv));
+            const int iy_max = std::min(Ny - 1, (int)std::floor((y_B + half * Ty - y0) * Ty_inv));
+            const R z_A = z_c - width_z, z_B = z_c + width_z;
+            const int iz_min = std::max(0, (int)std::ceil((z_A - half * Tz - z0) * Tz_inv));
+            const int iz_max = std::min(Nz - 1, (int)std::floor((z_B + half * Tz - z0) * Tz_inv));
+            R y = R(iy_min) * Ty + y0 - t * ry;
+            const R z_start = (R(iz_min) * Tz + z0 - t * rz - u1_0) * Tu1_inv;
+            for (int iy = iy_min; iy <= iy_max; iy++) {
+                const R c = (y * sin_inv - u0_0) * Tu0_inv;
+                const R lo = c - y_fw, hi = c + y_fw;
+                const R w0 = std::min(iu0_max, hi) - std::max(iu0_min, lo);
+                if (w0 > R(0)) {
+                    R z = z_start;
+                    for (int iz = iz_min; iz <= iz_max; iz++) {
+                        const R w1 = std::max(R(0), std::min(iu1_max, z + z_fw) - std::max(iu1_min, z - z_fw));
+                        if (w1 > R(0))
+                            visit((int64_t(iz) * Ny + iy) * Nx + ix, w0, w1);
+                        z += Tz_over_Tu1;
+                    }
+                }
+                y += Ty;
+            }
+        }
+        return l_phi;
+    } else {
+        // y-slices (src/projectors.cu:133-185)
+        const R width_x = half * R(P.T_u0) * std::fabs(cs);
+        const R cos_inv = R(1) / cs;
+        const R ry_inv = R(1) / ry;
+        const R l_phi = Tx * std::fabs(cos_inv);
+        const R x_fw = half * Tx * std::fabs(cos_inv) * Tu0_inv;
+        for (int iy = 0; iy < Ny; iy++) {
+            const R y = R(iy) * Ty + y0;
+            const R t = (y - py) * ry_inv;
+            const R x_c = px + t * rx;
+            const R z_c = pz + t * rz;
+            const R x_A = x_c - width_x, x_B = x_c + width_x;
+            const int ix_min = std::max(0, (int)std::ceil((x_A - half * Tx - x0) * Tx_inv));
+            const int ix_max = std::min(Nx - 1, (int)std::floor((x_B + half * Tx - x0) * Tx_inv));
+            const R z_A = z_c - width_z, z_B = z_c + width_z;
+            const int iz_min = std::max(0, (int)std::ceil((z_A - half * Tz - z0) * Tz_inv));
+            const int iz_max = std::min(Nz - 1, (int)std::floor((z_B + half * Tz - z0) * Tz_inv));
+            R x = R(ix_min) * Tx + x0 - t * rx;
+            const R z_start = (R(iz_min) * Tz + z0 - t * rz - u1_0) * Tu1_inv;
+            for (int ix = ix_min; ix <= ix_max; ix++) {
+                const R c = (x * cos_inv - u0_0) * Tu0_inv;
+                const R lo = c - x_fw, hi = c + x_fw;
+                const R w0 = std::min(iu0_max, hi) - std::max(iu0_min, lo);
+                if (w0 > R(0)) {
+                    R z = z_start;
+                    for (int iz = iz_min; iz <= iz_max; iz++) {
+                        const R w1 = std::max(R(0), std::min(iu1_max, z + z_fw) - std::max(iu1_min, z - z_fw));
+                        if (w1 > R(0))
+                            visit((int64_t(iz) * Ny + iy) * Nx + ix, w0, w1);
+                        z += Tz_over_Tu1;
+                    }
+                }
+                x += Tx;
+            }
+        }
+        return l_phi;
+    }
+}
+
+template <typename R>
+void project_plan(const orc_plan& P, const orc_vol& V, const R* f, R* g, int nthreads)
+{
+    const int NV = P.N_v1 * P.N_v0;
+    const int64_t row = int64_t(P.N_u1) * P.N_u0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int v = 0; v < NV; v++) {
+        const int iv1 = v / P.N_v0, iv0 = v % P.N_v0;
+        R* gv = g + int64_t(v) * row;
+        for (int iu1 = 0; iu1 < P.N_u1; iu1++)
+            for (int iu0 = 0; iu0 < P.N_u0; iu0++) {
+                R out = R(0);
+                if (!bin_masked(P, iv1, iv0, iu1, iu0)) {
+                    R val = R(0);
+                    const R l_phi = enumerate_bin<R>(P, V, iv1, iv0, iu1, iu0,
+                                                     [&](int64_t k, R w0, R w1) { val += f[k] * w0 * w1; });
+                    out = (val * l_phi) * obliquity<R>(P, iv1, iv0);
+                }
+                gv[int64_t(iu1) * P.N_u0 + iu0] = out;
+            }
+    }
+}
+
+// Exact transpose of project_plan: scatter every tap with the same weight.  Accumulates into f.
+template <typename R>
+void backproject_matched_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f, int nthreads)
+{
+    const int64_t nvox = int64_t(V.numX) * V.numY * V.numZ;
+    const int NV = P.N_v1 * P.N_v0;
+    const int64_t row = int64_t(P.N_u1) * P.N_u0;
+    int nt = std::max(1, nthreads);
+    std::vector<std::vector<R>> part(nt, std::vector<R>(nvox, R(0)));
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int v = 0; v < NV; v++) {
+        std::vector<R>& acc = part[omp_get_thread_num()];
+        std::vector<int64_t> kv;
+        std::vector<R> wv;
+        const int iv1 = v / P.N_v0, iv0 = v % P.N_v0;
+        const R* gv = g + int64_t(v) * row;
+        for (int iu1 = 0; iu1 < P.N_u1; iu1++)
+            for (int iu0 = 0; iu0 < P.N_u0; iu0++) {
+                const R gb = gv[int64_t(iu1) * P.N_u0 + iu0];
+                if (gb == R(0) || bin_masked(P, iv1, iv0, iu1, iu0))
+                    continue;
+                kv.clear();
+                wv.clear();
+                const R l_phi = enumerate_bin<R>(P, V, iv1, iv0, iu1, iu0, [&](int64_t k, R w0, R w1) {
+                    kv.push_back(k);
+                    wv.push_back(w0 * w1);
+                });
+                const R s = gb * (l_phi * obliquity<R>(P, iv1, iv0));
+                for (size_t i = 0; i < kv.size(); i++)
+                    acc[kv[i]] += s * wv[i];
+            }
+    }
+    for (int t = 0; t < nt; t++)
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < nvox; i++)
+            f[i] += part[t][i];
+}
+
+// PETL-mode back projector: src/projectors.cu:240-377 with exact bilinear weights.  Accumulates into f.
+template <typename R>
+void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f, int nthreads)
+{
+    const int Nx = V.numX, Ny = V.numY, Nz = V.numZ;
+    const R cs = R(P.cos_psi), sn = R(P.sin_psi), half = R(0.5);
+    const R Tx = R(V.voxelWidth), Ty = R(V.voxelWidth), Tz = R(V.voxelHeight);
+    const R Tu1_inv = R(1) / R(P.T_u1), Tu0_inv = R(1) / R(P.T_u0);
+    const R u0_0 = R(P.u0_0), u1_0 = R(P.u1_0);
+    const int64_t row = int64_t(P.N_u1) * P.N_u0;
+    const int64_t slab = int64_t(P.N_v0) * row;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int iz = 0; iz < Nz; iz++) {
+        for (int iy = 0; iy < Ny; iy++)
+            for (int ix = 0; ix < Nx; ix++) {
+                const R x = R(ix) * Tx + R(V.x0);
+                const R y = R(iy) * Ty + R(V.y0);
+                const R z = R(iz) * Tz + R(V.z0);
+                const float xf = coord_f(ix, V.voxelWidth, V.x0), yf = coord_f(iy, V.voxelWidth, V.y0);
+                const float zf = coord_f(iz, V.voxelHeight, V.z0);
+                R fsum = R(0);
+                for (int iv1 = 0; iv1 < P.N_v1; iv1++) {
+                    R val = R(0);
+                    // u1 test at the voxel centre (src/projectors.cu:284-289), evaluated in float
+                    const float v1f = coord_f(iv1, P.T_v1, P.v1_0);
+                    const float tf = yf * P.cos_psi - xf * P.sin_psi;
+                    const float u1f = zf + v1f * tf;
+                    if (std::fabs(u1f) - 0.5f * P.T_u1 <= P.H - P.R * std::fabs(v1f)) {
+                        const R v1 = R(iv1) * R(P.T_v1) + R(P.v1_0);
+                        const R t = y * cs - x * sn;
+                        const R u1 = z + v1 * t;
+                        const R xdt = x * cs + y * sn;
+                        const R iu1_mid = (u1 - u1_0) * Tu1_inv;
+                        const R iu1_A = iu1_mid - half * Tz * Tu1_inv;
+                        const R iu1_B = iu1_mid + half * Tz * Tu1_inv;
+                        const int iu1_lo = std::max(0, (int)std::ceil(iu1_A - half));
+                        const int iu1_hi = std::min(P.N_u1 - 1, (int)std::floor(iu1_B + half));
+                        const R xdt_dy = half * Ty * sn, t_dy = half * Ty * cs;
+                        const R xdt_dx = half * Tx * cs, t_dx = -half * Tx * sn;
+                        const R* gs = g + int64_t(iv1) * slab;
+                        for (int iv0 = 0; iv0 < P.N_v0; iv0++) {
+                            const R v0 = R(iv0) * R(P.T_v0) + R(P.v0_0);
+                            const R wy = v0 * cs + sn;
+                            const R wx = cs - v0 * sn;
+                            const R l_phi = R(1) / std::max(std::fabs(wy), std::fabs(wx)) /
+                                            std::sqrt(R(1) + v0 * v0 + v1 * v1);
+                            R A, B;
+                            if (std::fabs(wy) > std::fabs(wx)) {
+                                const R a = (xdt - xdt_dy + v0 * (t - t_dy) - u0_0) * Tu0_inv;
+                                const R b = (xdt + xdt_dy + v0 * (t + t_dy) - u0_0) * Tu0_inv;
+                                if (wy >= R(0)) { A = a; B = b; } else { A = b; B = a; }
+                            } else {
+                                const R a = (xdt - xdt_dx + v0 * (t - t_dx) - u0_0) * Tu0_inv;
+                                const R b = (xdt + xdt_dx + v0 * (t + t_dx) - u0_0) * Tu0_inv;
+                                if (wx >= R(0)) { A = a; B = b; } else { A = b; B = a; }
+                            }
+                            const int iu0_lo = std::max(0, (int)std::ceil(A - half));
+                            const int iu0_hi = std::min(P.N_u0 - 1, (int)std::floor(B + half));
+                            const R* gv = gs + int64_t(iv0) * row;
+                            for (int iu0 = iu0_lo; iu0 <= iu0_hi; iu0++) {
+                                const R w0 = std::max(R(0), std::min(R(iu0) + half, B) - std::max(R(iu0) - half, A));
+                                if (!(w0 > R(0)))
+                                    continue;
+                                for (int iu1 = iu1_lo; iu1 <= iu1_hi; iu1++) {
+                                    const R w1 = std::max(R(0), std::min(R(iu1) + half, iu1_B) -
+                                                                    std::max(R(iu1) - half, iu1_A));
+                                    if (w1 > R(0))
+                                        val += gv[int64_t(iu1) * P.N_u0 + iu0] * l_phi * w0 * w1;
+                                }
+                            }
+                        }
+                    }
+                    fsum += val * Tx;
+                }
+                f[(int64_t(iz) * Ny + iy) * Nx + ix] += fsum;
+            }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Relative-difference prior gradient: src/relative_differences.cu:21-28, 107-173, 481-532.
+// Arrays are [N1][N2][N3] (numpy (z,y,x)); neighbours are clamped.
+// ------------------------------------------------------------------------------------------------
+template <typename R>
+inline R drd(R x, R y, R delta)
+{
+    if (x == R(0) && y == R(0))
+        return R(0);
+    const R ad = std::fabs(x - y);
+    const R den = x + y + delta * ad;
+    return (x - y) * (delta * ad + x + R(3) * y) / (den * den);
+}
+
+template <typename R>
+void rd_gradient(const R* f_in, R* Df, int N1, int N2, int N3, R delta, R beta)
+{
+    const int64_t n = int64_t(N1) * N2 * N3;
+    std::vector<R> f(f_in, f_in + n);
+    for (int64_t i = 0; i < n; i++)  // clip kernel (src/relative_differences.cu:107-115)
+        if (f[i] < R(1.0e-8f))
+            f[i] = R(0);
+    const R d1 = R(1.0f) * beta, d2 = R(0.7071067811865475f) * beta, d3 = R(0.5773502691896258f) * beta;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N1; i++) {
+        const int im = std::max(0, i - 1), ip = std::min(N1 - 1, i + 1);
+        for (int j = 0; j < N2; j++) {
+            const int jm = std::max(0, j - 1), jp = std::min(N2 - 1, j + 1);
+            for (int k = 0; k < N3; k++) {
+                const int km = std::max(0, k - 1), kp = std::min(N3 - 1, k + 1);
+                auto F = [&](int a, int b, int c) { return f[(int64_t(a) * N2 + b) * N3 + c]; };
+                const R c0 = F(i, j, k);
+                auto D = [&](int a, int b, int c) { return drd<R>(c0, F(a, b, c), delta); };
+                const R s1 = D(ip, j, k) + D(im, j, k) + D(i, jp, k) + D(i, jm, k) + D(i, j, kp) + D(i, j, km);
+                const R s2 = D(ip, jp, k) + D(ip, jm, k) + D(ip, j, kp) + D(ip, j, km) + D(im, jp, k) +
+                             D(im, jm, k) + D(im, j, kp) + D(im, j, km) + D(i, jp, kp) + D(i, jp, km) +
+                             D(i, jm, kp) + D(i, jm, km);
+                const R s3 = D(ip, jp, kp) + D(ip, jp, km) + D(ip, jm, kp) + D(ip, jm, km) + D(im, jp, kp) +
+                             D(im, jp, km) + D(im, jm, kp) + D(im, jm, km);
+                Df[(int64_t(i) * N2 + j) * N3 + k] = s1 * d1 + s2 * d2 + s3 * d3;
+            }
+        }
+    }
+}
+
+// Relative-difference cost with correct indexing (the reference cost kernel at
+// src/relative_differences.cu:77 drops N.y in one index); f is clipped first as in the gradient.
+double rd_cost(const double* f_in, int N1, int N2, int N3, double delta, double beta)
+{
+    const int64_t n = int64_t(N1) * N2 * N3;
+    std::vector<double> f(f_in, f_in + n);
+    for (int64_t i = 0; i < n; i++)
+        if (f[i] < 1.0e-8)
+            f[i] = 0.0;
+    auto phi = [&](double x, double y) {
+        if (x == y)
+            return 0.0;
+        return (x - y) * (x - y) / ((x + y) + delta * std::fabs(x - y));
+    };
+    double acc = 0.0;
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++)
+            for (int k = 0; k < N3; k++) {
+                const double c0 = f[(int64_t(i) * N2 + j) * N3 + k];
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -1; dk <= 1; dk++) {
+                            const int m = std::abs(di) + std::abs(dj) + std::abs(dk);
+                            if (m == 0)
+                                continue;
+                            const int a = std::min(N1 - 1, std::max(0, i + di));
+                            const int b = std::min(N2 - 1, std::max(0, j + dj));
+                            const int c = std::min(N3 - 1, std::max(0, k + dk));
+                            const double w = (m == 1) ? 1.0 : (m == 2 ? (double)0.7071067811865475f
+                                                                      : (double)0.5773502691896258f);
+                            acc += w * phi(c0, f[(int64_t(a) * N2 + b) * N3 + c]);
+                        }
+            }
+    return beta * acc;
+}
+
+// ------------------------------------------------------------------------------------------------
+// p-Huber TV gradient: src/total_variation.cu:21-78 (USE_P_HUBER on), 434-572, 574-660.
+// beta<=0 -> 1, delta floored at 1e-8, 26 neighbours (8 in-plane ones when N1 == 1).
+// ------------------------------------------------------------------------------------------------
+struct huber {
+    float p, delta, slope, shift;
+    huber(float delta_in, float p_in)
+    {
+        p = p_in;
+        delta = delta_in;
+        slope = (float)(std::pow((double)delta_in, 2.0 - (double)p_in) / (double)p_in);
+        shift = (float)((double)delta_in * (double)delta_in * (0.5 - 1.0 / (double)p_in));
+    }
+    template <typename R>
+    R d(R x) const
+    {
+        if (std::fabs(x) <= R(delta))
+            return x;
+        return R(p) * R(slope) * std::pow(std::fabs(x), R(p)) / x;
+    }
+    double h(double x) const
+    {
+        if (std::fabs(x) <= (double)delta)
+            return 0.5 * x * x;
+        return (double)slope * std::pow(std::fabs(x), (double)p) + (double)shift;
+    }
+};
+
+template <typename R>
+void tv_gradient(const R* f, R* Df, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    if (beta <= 0.0f)
+        beta = 1.0f;
+    if (delta < 1.0e-8f)
+        delta = 1.0e-8f;
+    const huber hb(delta, p);
+    const R d1 = R(1.0f * beta), d2 = R(0.7071067811865475f * beta), d3 = R(0.5773502691896258f * beta);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N1; i++) {
+        const int im = std::max(0, i - 1), ip = std::min(N1 - 1, i + 1);
+        for (int j = 0; j < N2; j++) {
+            const int jm = std::max(0, j - 1), jp = std::min(N2 - 1, j + 1);
+            for (int k = 0; k < N3; k++) {
+                const int km = std::max(0, k - 1), kp = std::min(N3 - 1, k + 1);
+                auto F = [&](int a, int b, int c) { return f[(int64_t(a) * N2 + b) * N3 + c]; };
+                const R c0 = F(i, j, k);
+                auto D = [&](int a, int b, int c) { return hb.d<R>(c0 - F(a, b, c)); };
+                R out;
+                if (N1 == 1) {
+                    out = (D(i, jp, k) + D(i, jm, k) + D(i, j, kp) + D(i, j, km)) * d1 +
+                          (D(i, jp, kp) + D(i, jp, km) + D(i, jm, kp) + D(i, jm, km)) * d2;
+                } else {
+                    const R s1 = D(ip, j, k) + D(im, j, k) + D(i, jp, k) + D(i, jm, k) + D(i, j, kp) + D(i, j, km);
+                    const R s2 = D(ip, jp, k) + D(ip, jm, k) + D(ip, j, kp) + D(ip, j, km) + D(im, jp, k) +
+                                 D(im, jm, k) + D(im, j, kp) + D(im, j, km) + D(i, jp, kp) + D(i, jp, km) +
+                                 D(i, jm, kp) + D(i, jm, km);
+                    const R s3 = D(ip, jp, kp) + D(ip, jp, km) + D(ip, jm, kp) + D(ip, jm, km) +
+                                 D(im, jp, kp) + D(im, jp, km) + D(im, jm, kp) + D(im, jm, km);
+                    out = s1 * d1 + s2 * d2 + s3 * d3;
+                }
+                Df[(int64_t(i) * N2 + j) * N3 + k] = out;
+            }
+        }
+    }
+}
+
+double tv_cost(const double* f, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    if (beta <= 0.0f)
+        beta = 1.0f;
+    if (delta < 1.0e-8f)
+        delta = 1.0e-8f;
+    const huber hb(delta, p);
+    double acc = 0.0;
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++)
+            for (int k = 0; k < N3; k++) {
+                const double c0 = f[(int64_t(i) * N2 + j) * N3 + k];
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -1; dk <= 1; dk++) {
+                            if (N1 == 1 && di != 0)
+                                continue;
+                            const int m = std::abs(di) + std::abs(dj) + std::abs(dk);
+                            if (m == 0)
+                                continue;
+                            const int a = std::min(N1 - 1, std::max(0, i + di));
+                            const int b = std::min(N2 - 1, std::max(0, j + dj));
+                            const int c = std::min(N3 - 1, std::max(0, k + dk));
+                            const double w = (m == 1) ? 1.0 : (m == 2 ? (double)0.7071067811865475f
+                                                                      : (double)0.5773502691896258f);
+                            acc += w * hb.h(c0 - f[(int64_t(a) * N2 + b) * N3 + c]);
+                        }
+            }
+    return (double)beta * acc;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// C entry points (loaded with ctypes by the tests)
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+int orc_plan_init(float psi_deg, float D, float L, float H, float v_m0, float v_m1, float T, orc_plan* out)
+{
+    return plan_init(psi_deg, D, L, H, v_m0, v_m1, T, out);
+}
+
+void orc_vol_init(int nx, int ny, int nz, float vw, float vh, float ox, float oy, float oz, orc_vol* v)
+{
+    vol_init(nx, ny, nz, vw, vh, ox, oy, oz, v);
+}
+
+int orc_bin_masked(const orc_plan* P, int iv1, int iv0, int iu1, int iu0)
+{
+    return bin_masked(*P, iv1, iv0, iu1, iu0) ? 1 : 0;
+}
+
+void orc_project_f32(const orc_plan* P, const orc_vol* V, const float* f, float* g, int nthreads)
+{
+    project_plan<float>(*P, *V, f, g, nthreads);
+}
+
+void orc_project_f64(const orc_plan* P, const orc_vol* V, const double* f, double* g, int nthreads)
+{
+    project_plan<double>(*P, *V, f, g, nthreads);
+}
+
+// mode 0 = PETL (reference semantics), 1 = matched (exact transpose).  Accumulates into f.
+void orc_backproject_f32(const orc_plan* P, const orc_vol* V, const float* g, float* f, int mode, int nthreads)
+{
+    if (mode == 1)
+        backproject_matched_plan<float>(*P, *V, g, f, nthreads);
+    else
+        backproject_petl_plan<float>(*P, *V, g, f, nthreads);
+}
+
+void orc_backproject_f64(const orc_plan* P, const orc_vol* V, const double* g, double* f, int mode, int nthreads)
+{
+    if (mode == 1)
+        backproject_matched_plan<double>(*P, *V, g, f, nthreads);
+    else
+        backproject_petl_plan<double>(*P, *V, g, f, nthreads);
+}
+
+// Taps of one bin (for brute-force / point-source checks).  Returns the tap count; writes at most cap.
+int orc_bin_taps(const orc_plan* P, const orc_vol* V, int iv1, int iv0, int iu1, int iu0, int64_t* idx,
+                 double* w, int cap, double* scale)
+{
+    int n = 0;
+    if (bin_masked(*P, iv1, iv0, iu1, iu0)) {
+        *scale = 0.0;
+        return 0;
+    }
+    const double l_phi = enumerate_bin<double>(*P, *V, iv1, iv0, iu1, iu0, [&](int64_t k, double w0, double w1) {
+        if (n < cap) {
+            idx[n] = k;
+            w[n] = w0 * w1;
+        }
+        n++;
+    });
+    *scale = l_phi * obliquity<double>(*P, iv1, iv0);
+    return n;
+}
+
+void orc_rd_gradient_f32(const float* f, float* Df, int N1, int N2, int N3, float delta, float beta)
+{
+    rd_gradient<float>(f, Df, N1, N2, N3, delta, beta);
+}
+
+void orc_rd_gradient_f64(const double* f, double* Df, int N1, int N2, int N3, double delta, double beta)
+{
+    rd_gradient<double>(f, Df, N1, N2, N3, delta, beta);
+}
+
+double orc_rd_cost_f64(const double* f, int N1, int N2, int N3, double delta, double beta)
+{
+    return rd_cost(f, N1, N2, N3, delta, beta);
+}
+
+void orc_tv_gradient_f32(const float* f, float* Df, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    tv_gradient<float>(f, Df, N1, N2, N3, delta, beta, p);
+}
+
+void orc_tv_gradient_f64(const double* f, double* Df, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    tv_gradient<double>(f, Df, N1, N2, N3, delta, beta, p);
+}
+
+double orc_tv_cost_f64(const double* f, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    return tv_cost(f, N1, N2, N3, delta, beta, p);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

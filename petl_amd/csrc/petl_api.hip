@@ -1,0 +1,758 @@
+// extern "C" boundary of libpetl.so (declared in include/petl_c_interface.h).
+//
+// The reference keeps a process-global list of models and a global "current model" index that every
+// Python call sets first (src/petl_c_interface.cpp:12-30, src/list_of_models.cpp:25-46); the same
+// registry semantics are kept here.  Host-pointer entry points copy inputs to the device, run the HIP
+// kernels on the model's stream and copy results back before returning, as the reference's do.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+#include "petl_common.h"
+#include "petl_model.h"
+#include "proj_kernels.h"
+#include "vec_kernels.h"
+#include "../../include/petl_c_interface.h"
+
+#define PETL_VERSION "1.0-mi355x"
+
+namespace petl {
+
+namespace {
+
+std::vector<std::unique_ptr<Model>> g_models;
+int g_which = 0;
+
+int append_model()
+{
+    g_models.push_back(std::make_unique<Model>());
+    return int(g_models.size()) - 1;
+}
+
+}  // namespace
+
+// src/list_of_models.cpp:37-46: list[i % size], creating one model on first use
+Model* current_model()
+{
+    if (g_models.empty())
+        append_model();
+    const int n = int(g_models.size());
+    int i = g_which % n;
+    if (i < 0)
+        i += n;
+    return g_models[i].get();
+}
+
+namespace {
+
+Model* M() { return current_model(); }
+
+template <typename Fn>
+bool guarded(const char* what, Fn&& fn)
+{
+    try {
+        return fn();
+    } catch (const std::exception& e) {
+        fprintf(stderr, "PETL %s failed: %s\n", what, e.what());
+    } catch (...) {
+        fprintf(stderr, "PETL %s failed\n", what);
+    }
+    return false;
+}
+
+bool unsupported(const char* name)
+{
+    fprintf(stderr,
+            "PETL: %s is outside the projector/MLEM path this library implements; it is not available.\n",
+            name);
+    return false;
+}
+
+// ---- host elementwise operations: src/data_cube.cpp:97-167, 320-397 ----
+enum Op { OP_MULT, OP_DIV, OP_RDIV };
+
+void binary_op(float* x, const float* y, int64_t n, Op op)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        switch (op) {
+        case OP_MULT:
+            x[i] *= y[i];
+            break;
+        case OP_DIV:
+            if (y[i] == 0.0f)
+                x[i] = 0.0f;
+            else
+                x[i] /= y[i];
+            break;
+        case OP_RDIV:
+            if (x[i] == 0.0f)
+                x[i] = 0.0f;
+            else
+                x[i] = y[i] / x[i];
+            break;
+        }
+    }
+}
+
+bool dims_ok(std::initializer_list<int> d)
+{
+    for (int v : d)
+        if (v <= 0)
+            return false;
+    return true;
+}
+
+// double accumulation per N_1 slab then summed in order (src/data_cube.cpp:320-397)
+float inner3(const float* x, const float* y, const float* w, int N1, int N2, int N3)
+{
+    std::vector<double> sums(N1);
+    const int64_t slab = int64_t(N2) * N3;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N1; i++) {
+        double s = 0.0;
+        const int64_t o = int64_t(i) * slab;
+        for (int64_t k = 0; k < slab; k++)
+            s += w ? double(x[o + k] * y[o + k] * w[o + k]) : double(x[o + k] * y[o + k]);
+        sums[i] = s;
+    }
+    double acc = 0.0;
+    for (int i = 0; i < N1; i++)
+        acc += sums[i];
+    return float(acc);
+}
+
+// ---- device staging helpers ----
+struct PlanBufs {
+    std::vector<float*> ptr;
+};
+
+void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs)
+{
+    bufs.resize(m.plans.size());
+    for (size_t i = 0; i < m.plans.size(); i++)
+        bufs[i].ensure(size_t(m.plans[i].elems()) * sizeof(float));
+}
+
+std::vector<float*> ptrs(std::vector<DevBuf>& bufs)
+{
+    std::vector<float*> p(bufs.size());
+    for (size_t i = 0; i < bufs.size(); i++)
+        p[i] = bufs[i].f();
+    return p;
+}
+
+void h2d(void* d, const void* h, size_t n, hipStream_t s)
+{
+    PETL_HIP_CHECK(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s));
+}
+void d2h(void* h, const void* d, size_t n, hipStream_t s)
+{
+    PETL_HIP_CHECK(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s));
+}
+
+// ---- device-resident MLEM / OSEM (src/petl_server.py:1534-1606, 1257-1273) ----
+bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float beta, int prior, float p, int nsub,
+              float** mask, float** response, double* loglik)
+{
+    if (!g || !f || !m.allDefined() || numIter < 0)
+        return false;
+    const size_t nP = m.plans.size();
+    for (size_t i = 0; i < nP; i++)
+        if (!g[i] || (mask && !mask[i]) || (response && !response[i]))
+            return false;
+    int Nv1_min = m.plans[0].N_v1;
+    for (const auto& pl : m.plans)
+        Nv1_min = std::min(Nv1_min, pl.N_v1);
+    nsub = std::max(1, std::min(nsub, Nv1_min));
+    hipStream_t s = m.stream();
+    m.prepare();
+    DeviceState& D = m.dev;
+    const int64_t V = m.vol_elems();
+
+    // initial estimate (src/petl_server.py:1571-1577), in place on the caller's array
+    bool all_zero = true;
+    for (int64_t i = 0; i < V; i++)
+        if (f[i] != 0.0f) {
+            all_zero = false;
+            break;
+        }
+    if (all_zero)
+        std::fill(f, f + V, 1.0f);
+    else
+        for (int64_t i = 0; i < V; i++)
+            if (f[i] < 0.0f)
+                f[i] = 0.0f;
+
+    ensure_plan_bufs(m, D.y);
+    ensure_plan_bufs(m, D.aux);
+    if (response)
+        ensure_plan_bufs(m, D.resp);
+    D.f.ensure(V * sizeof(float));
+    D.f2.ensure(V * sizeof(float));
+    D.d.ensure(V * sizeof(float));
+    D.S.ensure(size_t(nsub) * V * sizeof(float));
+    std::vector<float*> y = ptrs(D.y), aux = ptrs(D.aux);
+    std::vector<float*> resp;
+    if (response)
+        resp = ptrs(D.resp);
+    for (size_t i = 0; i < nP; i++) {
+        const size_t nb = size_t(m.plans[i].elems()) * sizeof(float);
+        h2d(y[i], g[i], nb, s);
+        if (response)
+            h2d(resp[i], response[i], nb, s);
+    }
+    h2d(D.f.p, f, V * sizeof(float), s);
+
+    // sensitivity per subset: back(mask or ones, times response), zeros -> 1
+    for (int sub = 0; sub < nsub; sub++) {
+        for (size_t i = 0; i < nP; i++) {
+            const int64_t n = m.plans[i].elems();
+            if (mask)
+                h2d(aux[i], mask[i], n * sizeof(float), s);
+            else
+                launch_fill(aux[i], 1.0f, n, s);
+            if (response)
+                launch_mul(aux[i], resp[i], n, s);
+        }
+        float* Ss = D.S.f() + int64_t(sub) * V;
+        m.back(aux.data(), Ss, m.back_mode, s, sub, nsub);
+        launch_sens_fix(Ss, V, s);
+    }
+
+    double* dll = nullptr;
+    double* ll_scratch = nullptr;
+    int64_t max_el = 0;
+    for (const auto& pl : m.plans)
+        max_el = std::max<int64_t>(max_el, pl.elems());
+    if (loglik) {
+        D.red.ensure(sizeof(double) * (size_t(numIter) + 1 + size_t(ll_scratch_doubles(max_el))));
+        dll = static_cast<double*>(D.red.p);
+        ll_scratch = dll + numIter + 1;
+        ensure_plan_bufs(m, D.g);
+    }
+    std::vector<float*> q = loglik ? ptrs(D.g) : std::vector<float*>();
+
+    float* fc = D.f.f();
+    float* fn = D.f2.f();
+    for (int it = 0; it < numIter; it++) {
+        for (int sub = 0; sub < nsub; sub++) {
+            if (loglik) {
+                m.forward(q.data(), fc, EPI_NONE, nullptr, response ? resp.data() : nullptr, s, sub, nsub);
+                for (size_t i = 0; i < nP; i++) {
+                    const int64_t n = m.plans[i].elems();
+                    launch_loglik(y[i], q[i], n, ll_scratch, dll + it, !(i == 0 && sub == 0), s);
+                    launch_ratio(aux[i], y[i], q[i], response ? resp[i] : nullptr, n, s);
+                }
+            } else {
+                m.forward(aux.data(), fc, EPI_RATIO, y.data(), response ? resp.data() : nullptr, s, sub, nsub);
+            }
+            m.back(aux.data(), D.d.f(), m.back_mode, s, sub, nsub);
+            const float* Ss = D.S.f() + int64_t(sub) * V;
+            if (beta <= 0.0f || prior == 0) {
+                launch_mlem_update(fc, D.d.f(), Ss, V, s);
+            } else {
+                launch_mlem_update_prior(fn, fc, D.d.f(), Ss, m.numZ, m.numY, m.numX, prior == 2 ? 1 : 0, delta,
+                                         beta, p, s);
+                std::swap(fc, fn);
+            }
+        }
+    }
+    d2h(f, fc, V * sizeof(float), s);
+    if (loglik && numIter > 0)
+        d2h(loglik, dll, sizeof(double) * numIter, s);
+    PETL_HIP_CHECK(hipStreamSynchronize(s));
+    return true;
+}
+
+}  // namespace
+}  // namespace petl
+
+using namespace petl;
+
+// =================================================================================================
+// Part 1: the reference ABI (src/petl_c_interface.cpp)
+// =================================================================================================
+extern "C" {
+
+bool set_model(int i)
+{
+    g_which = i;
+    return true;
+}
+
+int create_new_model()
+{
+    g_which = append_model();
+    return g_which;
+}
+
+void about()
+{
+    printf("***********************************************************\n");
+    printf("       Planogram Emission Tomography Library (PETL)\n");
+    printf("           MI355X (gfx950) HIP implementation %s\n", PETL_VERSION);
+    printf("***********************************************************\n");
+}
+
+bool copy_parameters(int param_id)
+{
+    if (0 <= param_id && param_id < int(g_models.size())) {
+        Model* cur = M();
+        Model* src = g_models[param_id].get();
+        if (cur != src)
+            cur->copy_params_from(*src);
+        return true;
+    }
+    return false;
+}
+
+void clearAll()
+{
+    Model* m = M();
+    m->plans.clear();
+    m->numX = m->numY = m->numZ = 0;
+    m->voxelWidth = m->voxelHeight = 0.0f;
+    m->offsetX = m->offsetY = m->offsetZ = 0.0f;
+    m->touch();
+}
+
+void print_parameters()
+{
+    Model* m = M();
+    printf("\n======== PET Volume ========\n");
+    printf("number of voxels (x, y, z): %d x %d x %d\n", m->numX, m->numY, m->numZ);
+    printf("voxel size: %f mm x %f mm x %f mm\n", m->voxelWidth, m->voxelWidth, m->voxelHeight);
+    if (m->offsetX != 0.0f || m->offsetY != 0.0f || m->offsetZ != 0.0f)
+        printf("volume offset: %f mm, %f mm, %f mm\n", m->offsetX, m->offsetY, m->offsetZ);
+    printf("\n======== PET Geometry ========\n");
+    for (size_t i = 0; i < m->plans.size(); i++) {
+        const Planogram& p = m->plans[i];
+        printf("Planogram %d\n", int(i));
+        printf("  angle = %f deg\n", p.psi * 180.0 / 3.1415926535897932385);
+        printf("  number of samples: %d x %d x %d x %d\n", p.N_v1, p.N_v0, p.N_u1, p.N_u0);
+        printf("  detector size: %f mm x %f mm\n", p.T_u1, p.T_u0);
+        printf("  full detector length: %f mm\n", 2.0 * p.L);
+        printf("  full detector height: %f mm\n", 2.0 * p.H);
+        printf("  distance between panels: %f mm\n", 2.0 * p.R);
+    }
+    printf("\n");
+}
+
+bool volume_defined() { return M()->volumeDefined(); }
+bool geometry_defined() { return M()->geometryDefined(); }
+
+int get_numX() { return M()->numX; }
+int get_numY() { return M()->numY; }
+int get_numZ() { return M()->numZ; }
+float get_voxelWidth() { return M()->voxelWidth; }
+float get_voxelHeight() { return M()->voxelHeight; }
+float get_offsetX() { return M()->offsetX; }
+float get_offsetY() { return M()->offsetY; }
+float get_offsetZ() { return M()->offsetZ; }
+
+#define PETL_SETTER(name, field, type, cond) \
+    bool name(type v)                       \
+    {                                       \
+        if (cond) {                         \
+            M()->field = v;                 \
+            M()->touch();                   \
+            return true;                    \
+        }                                   \
+        return false;                       \
+    }
+PETL_SETTER(set_numX, numX, int, v > 0)
+PETL_SETTER(set_numY, numY, int, v > 0)
+PETL_SETTER(set_numZ, numZ, int, v > 0)
+PETL_SETTER(set_voxelWidth, voxelWidth, float, v > 0.0f)
+PETL_SETTER(set_voxelHeight, voxelHeight, float, v > 0.0f)
+// src/petl_c_interface.cpp:154-195 accepts only positive offsets
+PETL_SETTER(set_offsetX, offsetX, float, v > 0.0f)
+PETL_SETTER(set_offsetY, offsetY, float, v > 0.0f)
+PETL_SETTER(set_offsetZ, offsetZ, float, v > 0.0f)
+#undef PETL_SETTER
+
+int get_numPlanograms() { return int(M()->plans.size()); }
+
+bool get_planogramSize(int which, int* shape)
+{
+    Model* m = M();
+    if (!m->geometryDefined() || which < 0 || which >= int(m->plans.size()) || shape == nullptr)
+        return false;
+    const Planogram& p = m->plans[which];
+    shape[0] = p.N_v1;
+    shape[1] = p.N_v0;
+    shape[2] = p.N_u1;
+    shape[3] = p.N_u0;
+    return true;
+}
+
+// src/petl_c_interface.cpp:216-219: degrees -> radians and full -> half sizes, in double
+bool add_planogram(float psi, float R, float L, float H, float v_m0, float v_m1, float T)
+{
+    Planogram p;
+    if (!p.init(float(double(psi) * 3.1415926535897932385 / 180.0), float(0.5 * double(R)), float(0.5 * double(L)),
+                float(0.5 * double(H)), v_m0, v_m1, T))
+        return false;
+    M()->plans.push_back(p);
+    M()->touch();
+    return true;
+}
+
+bool remove_planogram(int which)
+{
+    Model* m = M();
+    if (which >= 0 && which < int(m->plans.size())) {
+        m->plans.erase(m->plans.begin() + which);
+        m->touch();
+        return true;
+    }
+    return false;
+}
+
+bool keep_only_planogram(int which)
+{
+    Model* m = M();
+    if (which >= 0 && which < int(m->plans.size())) {
+        Planogram keep = m->plans[which];
+        m->plans.clear();
+        m->plans.push_back(keep);
+        m->touch();
+        return true;
+    }
+    return false;
+}
+
+// src/parameters.cpp:205-248
+bool set_default_volume(float scale)
+{
+    Model* m = M();
+    if (!m->geometryDefined())
+        return false;
+    m->offsetX = m->offsetY = m->offsetZ = 0.0f;
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    for (size_t i = 0; i < m->plans.size(); i++) {
+        const Planogram& p = m->plans[i];
+        const float W = p.L, Hh = p.R, psi = p.psi;
+        const float X_new = std::fabs(W * std::cos(psi)) + std::fabs(Hh * std::sin(psi));
+        const float Y_new = std::fabs(W * std::sin(psi)) + std::fabs(Hh * std::cos(psi));
+        if (i == 0) {
+            m->voxelWidth = p.T_u0;
+            m->voxelHeight = p.T_u1;
+            Z = p.H;
+            X = X_new;
+            Y = Y_new;
+        } else {
+            m->voxelWidth = std::min(m->voxelWidth, p.T_u0);
+            m->voxelHeight = std::min(m->voxelHeight, p.T_u1);
+            Z = std::min(Z, p.H);
+            X = std::min(X, X_new);
+            Y = std::min(Y, Y_new);
+        }
+    }
+    m->voxelWidth *= scale;
+    m->voxelHeight *= scale;
+    m->numZ = 2 * int(Z / m->voxelHeight);
+    m->numY = 2 * int(Y / m->voxelWidth);
+    m->numX = 2 * int(X / m->voxelWidth);
+    m->touch();
+    return true;
+}
+
+// src/parameters.cpp:250-269
+bool set_volume(int numX, int numY, int numZ, float voxelWidth, float voxelHeight, float offsetX, float offsetY,
+                float offsetZ)
+{
+    if (numX > 0 && numY > 0 && numZ > 0 && voxelWidth > 0.0f && voxelHeight > 0.0f) {
+        Model* m = M();
+        m->numX = numX;
+        m->numY = numY;
+        m->numZ = numZ;
+        m->voxelWidth = voxelWidth;
+        m->voxelHeight = voxelHeight;
+        m->offsetX = offsetX;
+        m->offsetY = offsetY;
+        m->offsetZ = offsetZ;
+        m->touch();
+        return true;
+    }
+    return false;
+}
+
+bool project(float** g, float* f)
+{
+    Model* m = M();
+    if (g == nullptr || f == nullptr || !m->allDefined())
+        return false;
+    for (size_t i = 0; i < m->plans.size(); i++)
+        if (!g[i])
+            return false;
+    return guarded("project", [&] {
+        hipStream_t s = m->stream();
+        m->prepare();
+        const int64_t V = m->vol_elems();
+        m->dev.f.ensure(V * sizeof(float));
+        ensure_plan_bufs(*m, m->dev.g);
+        std::vector<float*> dg = ptrs(m->dev.g);
+        h2d(m->dev.f.p, f, V * sizeof(float), s);
+        m->forward(dg.data(), m->dev.f.f(), EPI_NONE, nullptr, nullptr, s);
+        for (size_t i = 0; i < m->plans.size(); i++)
+            d2h(g[i], dg[i], size_t(m->plans[i].elems()) * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool backproject(float** g, float* f)
+{
+    Model* m = M();
+    if (g == nullptr || f == nullptr || !m->allDefined())
+        return false;
+    for (size_t i = 0; i < m->plans.size(); i++)
+        if (!g[i])
+            return false;
+    return guarded("backproject", [&] {
+        hipStream_t s = m->stream();
+        m->prepare();
+        const int64_t V = m->vol_elems();
+        m->dev.f.ensure(V * sizeof(float));
+        ensure_plan_bufs(*m, m->dev.g);
+        std::vector<float*> dg = ptrs(m->dev.g);
+        for (size_t i = 0; i < m->plans.size(); i++)
+            h2d(dg[i], g[i], size_t(m->plans[i].elems()) * sizeof(float), s);
+        m->back(dg.data(), m->dev.f.f(), m->back_mode, s);
+        d2h(f, m->dev.f.p, V * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool multiply3D(float* x, float* y, int N_1, int N_2, int N_3)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3}))
+        return false;
+    binary_op(x, y, int64_t(N_1) * N_2 * N_3, OP_MULT);
+    return true;
+}
+bool multiply4D(float* x, float* y, int N_1, int N_2, int N_3, int N_4)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3, N_4}))
+        return false;
+    binary_op(x, y, int64_t(N_1) * N_2 * N_3 * N_4, OP_MULT);
+    return true;
+}
+bool divide3D(float* x, float* y, int N_1, int N_2, int N_3)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3}))
+        return false;
+    binary_op(x, y, int64_t(N_1) * N_2 * N_3, OP_DIV);
+    return true;
+}
+bool divide4D(float* x, float* y, int N_1, int N_2, int N_3, int N_4)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3, N_4}))
+        return false;
+    binary_op(x, y, int64_t(N_1) * N_2 * N_3 * N_4, OP_DIV);
+    return true;
+}
+// denom = num / denom (denom == 0 -> 0)
+bool rdivide3D(float* num, float* denom, int N_1, int N_2, int N_3)
+{
+    if (!num || !denom || !dims_ok({N_1, N_2, N_3}))
+        return false;
+    binary_op(denom, num, int64_t(N_1) * N_2 * N_3, OP_RDIV);
+    return true;
+}
+bool rdivide4D(float* num, float* denom, int N_1, int N_2, int N_3, int N_4)
+{
+    if (!num || !denom || !dims_ok({N_1, N_2, N_3, N_4}))
+        return false;
+    binary_op(denom, num, int64_t(N_1) * N_2 * N_3 * N_4, OP_RDIV);
+    return true;
+}
+static void reciprocal_n(float* x, int64_t n, float dz)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        x[i] = (x[i] == 0.0f) ? dz : float(1.0 / x[i]);
+}
+bool reciprocal3D(float* x, int N_1, int N_2, int N_3, float divide_by_zero_value)
+{
+    if (!x || !dims_ok({N_1, N_2, N_3}))
+        return false;
+    reciprocal_n(x, int64_t(N_1) * N_2 * N_3, divide_by_zero_value);
+    return true;
+}
+bool reciprocal4D(float* x, int N_1, int N_2, int N_3, int N_4, float divide_by_zero_value)
+{
+    if (!x || !dims_ok({N_1, N_2, N_3, N_4}))
+        return false;
+    reciprocal_n(x, int64_t(N_1) * N_2 * N_3 * N_4, divide_by_zero_value);
+    return true;
+}
+float inner_product3D(float* x, float* y, float* w, int N_1, int N_2, int N_3)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3}))
+        return 0.0f;
+    return inner3(x, y, w, N_1, N_2, N_3);
+}
+// 4D: float partials per N_1 slab accumulated in double (src/data_cube.cpp:339-356)
+float inner_product4D(float* x, float* y, float* w, int N_1, int N_2, int N_3, int N_4)
+{
+    if (!x || !y || !dims_ok({N_1, N_2, N_3, N_4}))
+        return 0.0f;
+    const int64_t slab = int64_t(N_2) * N_3 * N_4;
+    double acc = 0.0;
+    for (int i = 0; i < N_1; i++)
+        acc += double(inner3(x + i * slab, y + i * slab, w ? w + i * slab : nullptr, N_2, N_3, N_4));
+    return float(acc);
+}
+
+bool TV_gradient(float* f, float* Df, int N_1, int N_2, int N_3, float delta, float beta, float p, bool data_on_cpu)
+{
+    if (!f || !Df || !dims_ok({N_1, N_2, N_3}))
+        return false;
+    return guarded("TV_gradient", [&] {
+        Model* m = M();
+        hipStream_t s = m->stream();
+        const int64_t n = int64_t(N_1) * N_2 * N_3;
+        if (data_on_cpu) {
+            DevBuf a, b;
+            a.ensure(n * sizeof(float));
+            b.ensure(n * sizeof(float));
+            h2d(a.p, f, n * sizeof(float), s);
+            launch_tv_gradient(a.f(), b.f(), N_1, N_2, N_3, delta, beta, p, s);
+            d2h(Df, b.p, n * sizeof(float), s);
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            launch_tv_gradient(f, Df, N_1, N_2, N_3, delta, beta, p, s);
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return true;
+    });
+}
+
+// src/relative_differences.cu:481-532 (Df == NULL writes into f)
+bool relativeDifferences_gradient(float* f, float* Df, int N_1, int N_2, int N_3, float delta, float beta)
+{
+    if (f == nullptr || N_1 <= 0 || N_2 <= 0 || N_3 <= 0)
+        return false;
+    if (Df == nullptr)
+        Df = f;
+    return guarded("relativeDifferences_gradient", [&] {
+        Model* m = M();
+        hipStream_t s = m->stream();
+        const int64_t n = int64_t(N_1) * N_2 * N_3;
+        DevBuf a, b;
+        a.ensure(n * sizeof(float));
+        b.ensure(n * sizeof(float));
+        h2d(a.p, f, n * sizeof(float), s);
+        launch_rd_gradient(a.f(), b.f(), N_1, N_2, N_3, delta, beta, s);
+        d2h(Df, b.p, n * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+// ---- outside the projector / MLEM path ----
+bool bin(float**, char*) { return unsupported("bin"); }
+bool simulate_scatter(float**, float*) { return unsupported("simulate_scatter"); }
+bool FBP(float**, float*) { return unsupported("FBP"); }
+bool PFDR(float**, float**) { return unsupported("PFDR"); }
+bool add_object(int, float*, float*, float, float*, float*) { return unsupported("add_object"); }
+bool add_module(int, float*, float*, float, float*, float*) { return unsupported("add_module"); }
+bool clear_phantom() { return unsupported("clear_phantom"); }
+bool clear_modules() { return unsupported("clear_modules"); }
+bool scale_phantom(float, float, float) { return unsupported("scale_phantom"); }
+bool voxelize(float*, int) { return unsupported("voxelize"); }
+bool ray_trace(float**, int) { return unsupported("ray_trace"); }
+bool stopping_power(float**, int) { return unsupported("stopping_power"); }
+bool set_solid_angle_correction(float**, bool) { return unsupported("set_solid_angle_correction"); }
+bool apply_corrections(float**, float**, float) { return unsupported("apply_corrections"); }
+float TV_cost(float*, int, int, int, float, float, float, bool)
+{
+    unsupported("TV_cost");
+    return 0.0f;
+}
+float TV_quadForm(float*, float*, int, int, int, float, float, float, bool)
+{
+    unsupported("TV_quadForm");
+    return 0.0f;
+}
+bool diffuse(float*, int, int, int, float, float, int, bool) { return unsupported("diffuse"); }
+bool TV_denoise(float*, int, int, int, float, float, float, int, bool, bool) { return unsupported("TV_denoise"); }
+bool gaussian_filter(float*, int, int, int, float, int) { return unsupported("gaussian_filter"); }
+
+// =================================================================================================
+// Part 2: additions
+// =================================================================================================
+bool set_backprojector_mode(int mode)
+{
+    if (mode != BACK_PETL && mode != BACK_MATCHED)
+        return false;
+    M()->back_mode = mode;
+    return true;
+}
+int get_backprojector_mode() { return M()->back_mode; }
+
+bool set_projector_path(int path)
+{
+    if (path != 0 && path != 1)
+        return false;
+    M()->path = path;
+    return true;
+}
+int get_projector_path() { return M()->path; }
+
+int planogram_uses_fast_path(int which)
+{
+    Model* m = M();
+    if (!m->allDefined())
+        return 0;
+    bool r = false;
+    guarded("planogram_uses_fast_path", [&] {
+        r = m->uses_fast(which);
+        return true;
+    });
+    return r ? 1 : 0;
+}
+
+bool project_gpu(float** d_g, float* d_f, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || !m->allDefined())
+        return false;
+    return guarded("project_gpu", [&] {
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream();
+        m->forward(d_g, d_f, EPI_NONE, nullptr, nullptr, s);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || !m->allDefined())
+        return false;
+    return guarded("backproject_gpu", [&] {
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream();
+        m->back(d_g, d_f, m->back_mode, s);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int prior, float p, int numSubsets,
+              float** mask, float** response, double* loglik)
+{
+    Model* m = M();
+    return guarded("MLEM_gpu",
+                   [&] { return run_mlem(*m, g, f, numIter, delta, beta, prior, p, numSubsets, mask, response, loglik); });
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Host model: planogram/volume parameters of one PETL "model" (src/parameters.h:20-115,
+// src/planogram.h:11-67) plus the device state that keeps buffers, kernel tables and a stream alive
+// between calls.
+#pragma once
+#include <memory>
+#include <vector>
+#include "petl_common.h"
+#include "proj_fast.h"
+
+namespace petl {
+
+struct Planogram {
+    float psi = 0, R = 0, L = 0, H = 0, v_m0 = 0, v_m1 = 0;
+    int N_u0 = 0, N_u1 = 0, N_v0 = 0, N_v1 = 0;
+    float T_u0 = 0, T_u1 = 0, T_v0 = 0, T_v1 = 0;
+    float u0_0 = 0, u1_0 = 0, v0_0 = 0, v1_0 = 0;
+    float cos_psi = 1, sin_psi = 0;
+
+    // src/planogram.cpp:143-186 (psi in radians, half sizes)
+    bool init(float psi_in, float R_in, float L_in, float H_in, float v_m0_in, float v_m1_in, float T);
+    bool defined() const;
+    PlanGeom geom() const;
+    int64_t elems() const { return int64_t(N_v1) * N_v0 * N_u1 * N_u0; }
+};
+
+// Owned device allocation that grows on demand.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    float* f() const { return static_cast<float*>(p); }
+    ~DevBuf() { release(); }
+};
+
+struct DeviceState {
+    hipStream_t stream = nullptr;
+    uint64_t built_version = 0;                   // geometry version the fast plans were made for
+    int built_path = -1;
+    std::vector<std::unique_ptr<FastPlan>> fast;  // per planogram; null = generic kernels
+    DevBuf f, f2, S, d;                           // volume-sized
+    std::vector<DevBuf> g, y, aux, resp;          // planogram-sized
+    DevBuf h2;                                    // fast back projector workspace
+    DevBuf red;                                   // reduction scratch
+    ~DeviceState();
+};
+
+struct Model {
+    std::vector<Planogram> plans;
+    int numX = 0, numY = 0, numZ = 0;
+    float voxelWidth = 0, voxelHeight = 0, offsetX = 0, offsetY = 0, offsetZ = 0;
+    int back_mode = BACK_PETL;
+    int path = 0;  // 0 automatic, 1 generic kernels only
+    uint64_t version = 1;
+    DeviceState dev;
+
+    bool volumeDefined() const;
+    bool geometryDefined() const;
+    bool allDefined() const { return geometryDefined() && volumeDefined(); }
+    float x_0() const;
+    float y_0() const;
+    float z_0() const;
+    VolGeom vgeom() const;
+    int64_t vol_elems() const { return int64_t(numX) * numY * numZ; }
+    void touch() { version++; }
+    void copy_params_from(const Model& o);
+
+    // device helpers (petl_device.hip)
+    hipStream_t stream();
+    void prepare();  // (re)build per-planogram kernel plans when geometry changed
+    bool uses_fast(int i);
+    // g_i = A_i f for all planograms (device pointers), with optional ratio epilogue (y, resp lists may
+    // be null).  v1 subset (first, stride) restricts the views; (0,1) = all.
+    void forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
+                 int subset = 0, int nsubsets = 1);
+    // f = sum_i A_i^T g_i (overwrites f)
+    void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
+};
+
+Model* current_model();
+
+}  // namespace petl

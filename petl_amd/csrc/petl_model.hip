@@ -1,0 +1,220 @@
+// Host model (geometry/volume parameters, registry) and device dispatch of the projectors.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include "petl_model.h"
+#include "proj_kernels.h"
+#include "vec_kernels.h"
+
+namespace petl {
+
+// ---------------------------------------------------------------------------------------------
+// planogram geometry: src/planogram.cpp:143-186.  Float/double promotion as in the C++ source.
+// ---------------------------------------------------------------------------------------------
+bool Planogram::init(float psi_in, float R_in, float L_in, float H_in, float v_m0_in, float v_m1_in, float T)
+{
+    if (R_in <= 0.0f || L_in <= 0.0f || H_in <= 0.0f || v_m0_in <= 0.0f || v_m1_in < 0.0f || T <= 0.0f)
+        return false;
+    psi = psi_in;
+    R = R_in;
+    L = L_in;
+    H = H_in;
+    v_m0 = v_m0_in;
+    const float h_over_r = H / R;
+    v_m1 = (v_m1_in < h_over_r) ? v_m1_in : h_over_r;
+    T_u0 = T;
+    T_u1 = T;
+    volatile float tv0 = T_u0 / R;
+    volatile float tv1 = T_u1 / R;
+    T_v0 = tv0;
+    T_v1 = tv1;
+    volatile float lr = R * v_m0;
+    volatile float lsum = L + lr;
+    N_u0 = 2 * int(lsum / T_u0);
+    N_u1 = 2 * int(H / T_u1);
+    N_v0 = 2 * int(v_m0 / T_v0) + 1;
+    N_v1 = 2 * int(v_m1 / T_v1) + 1;
+    T_v0 = float(double(v_m0) * 2.0 / double(float(N_v0)));
+    volatile float a0 = -T_u0 * float(N_u0 - 1);
+    volatile float a1 = -T_u1 * float(N_u1 - 1);
+    volatile float a2 = -T_v0 * float(N_v0 - 1);
+    volatile float a3 = -T_v1 * float(N_v1 - 1);
+    u0_0 = float(double(a0) / 2.0);
+    u1_0 = float(double(a1) / 2.0);
+    v0_0 = float(double(a2) / 2.0);
+    v1_0 = float(double(a3) / 2.0);
+    cos_psi = float(std::cos(double(psi)));
+    sin_psi = float(std::sin(double(psi)));
+    return true;
+}
+
+// src/planogram.cpp:91-97
+bool Planogram::defined() const
+{
+    return R > 0.0f && L > 0.0f && H > 0.0f && v_m0 > 0.0f && v_m1 >= 0.0f && T_u0 > 0.0f && T_u1 > 0.0f &&
+           T_v0 > 0.0f && T_v1 > 0.0f && N_u0 > 0 && N_u1 > 0 && N_v0 > 0 && N_v1 > 0;
+}
+
+PlanGeom Planogram::geom() const
+{
+    PlanGeom g;
+    g.Nv1 = N_v1;
+    g.Nv0 = N_v0;
+    g.Nu1 = N_u1;
+    g.Nu0 = N_u0;
+    g.Tv1 = T_v1;
+    g.Tv0 = T_v0;
+    g.Tu1 = T_u1;
+    g.Tu0 = T_u0;
+    g.v1_0 = v1_0;
+    g.v0_0 = v0_0;
+    g.u1_0 = u1_0;
+    g.u0_0 = u0_0;
+    g.cos_psi = cos_psi;
+    g.sin_psi = sin_psi;
+    g.R = R;
+    g.L = L;
+    g.H = H;
+    g.psi = psi;
+    return g;
+}
+
+// src/parameters.cpp:175-203
+bool Model::geometryDefined() const
+{
+    if (plans.empty())
+        return false;
+    for (const auto& p : plans)
+        if (!p.defined())
+            return false;
+    return true;
+}
+
+bool Model::volumeDefined() const
+{
+    return !(numX <= 0 || numY <= 0 || numZ <= 0 || voxelWidth <= 0.0f || voxelHeight <= 0.0f);
+}
+
+// src/parameters.cpp:271-284
+float Model::x_0() const { return float((0.0 - 0.5 * (numX - 1)) * double(voxelWidth) + double(offsetX)); }
+float Model::y_0() const { return float((0.0 - 0.5 * (numY - 1)) * double(voxelWidth) + double(offsetY)); }
+float Model::z_0() const { return float((0.0 - 0.5 * (numZ - 1)) * double(voxelHeight) + double(offsetZ)); }
+
+VolGeom Model::vgeom() const
+{
+    VolGeom v;
+    v.Nx = numX;
+    v.Ny = numY;
+    v.Nz = numZ;
+    v.Tx = voxelWidth;
+    v.Ty = voxelWidth;
+    v.Tz = voxelHeight;
+    v.x0 = x_0();
+    v.y0 = y_0();
+    v.z0 = z_0();
+    return v;
+}
+
+void Model::copy_params_from(const Model& o)
+{
+    plans = o.plans;
+    numX = o.numX;
+    numY = o.numY;
+    numZ = o.numZ;
+    voxelWidth = o.voxelWidth;
+    voxelHeight = o.voxelHeight;
+    offsetX = o.offsetX;
+    offsetY = o.offsetY;
+    offsetZ = o.offsetZ;
+    touch();
+}
+
+// ---------------------------------------------------------------------------------------------
+// device state
+// ---------------------------------------------------------------------------------------------
+void DevBuf::ensure(size_t n)
+{
+    if (n <= bytes && p)
+        return;
+    release();
+    PETL_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 256)));
+    bytes = n;
+}
+
+void DevBuf::release()
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+DeviceState::~DeviceState()
+{
+    // buffers free themselves; the stream is left to process teardown (destroying it after the HIP
+    // runtime has shut down is not safe)
+}
+
+hipStream_t Model::stream()
+{
+    if (!dev.stream)
+        PETL_HIP_CHECK(hipStreamCreateWithFlags(&dev.stream, hipStreamNonBlocking));
+    return dev.stream;
+}
+
+void Model::prepare()
+{
+    if (dev.built_version == version && dev.built_path == path && dev.fast.size() == plans.size())
+        return;
+    dev.fast.clear();
+    const VolGeom V = vgeom();
+    size_t ws = 0;
+    for (const auto& p : plans) {
+        std::unique_ptr<FastPlan> fp;
+        if (path == 0)
+            fp = make_fast_plan(p.geom(), V);
+        if (fp)
+            ws = std::max(ws, fast_back_workspace_bytes(*fp));
+        dev.fast.push_back(std::move(fp));
+    }
+    if (ws)
+        dev.h2.ensure(ws);
+    dev.built_version = version;
+    dev.built_path = path;
+}
+
+bool Model::uses_fast(int i)
+{
+    prepare();
+    return i >= 0 && i < int(dev.fast.size()) && dev.fast[i] != nullptr;
+}
+
+void Model::forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
+                    int subset, int nsubsets)
+{
+    prepare();
+    const VolGeom V = vgeom();
+    for (size_t i = 0; i < plans.size(); i++) {
+        const float* yi = y ? y[i] : nullptr;
+        const float* ri = resp ? resp[i] : nullptr;
+        if (dev.fast[i])
+            fast_forward(*dev.fast[i], g[i], f, epilogue, yi, ri, s, subset, nsubsets);
+        else
+            launch_fwd_generic(g[i], f, plans[i].geom(), V, epilogue, yi, ri, s, subset, nsubsets);
+    }
+}
+
+void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets)
+{
+    prepare();
+    const VolGeom V = vgeom();
+    launch_fill(f, 0.0f, vol_elems(), s);
+    for (size_t i = 0; i < plans.size(); i++) {
+        if (dev.fast[i])
+            fast_back(*dev.fast[i], g[i], f, mode, dev.h2.f(), s, subset, nsubsets);
+        else
+            launch_back_generic(g[i], f, plans[i].geom(), V, mode, s, subset, nsubsets);
+    }
+}
+
+}  // namespace petl

@@ -1,0 +1,720 @@
+// Fast separable SF projector and atomic-free gather back projector for gfx950 (see proj_fast.h).
+//
+// Axes.  For a psi = 0 panel the slices run along y and the u0 interpolation runs along x; for psi = 90
+// deg the roles of x and y swap (src/projectors.cu:77 picks x slices when |r.x| >= |r.y|).  Below, "s"
+// is the slice axis (Ns voxels), "a" the u0-interpolated axis (Na voxels) and z the u1-interpolated axis.
+//
+// Forward (fwd_fast_kernel), one workgroup per (u1 row, v1 view row, group of v0 views):
+//   1. For a block of SB slices, interpolate the volume along z onto this u1 row,
+//      H[s][a] = wa * f[iz_a][s][a] + wb * f[iz_a+1][s][a], into LDS (zero-padded rows).
+//   2. Each wave owns up to J v0 views; lane p of chunk c owns output bin u0 = 64c + p.  With the slice's
+//      u0 interpolation out[p] = wq*H[p+k] + wp*H[p+k+1], every lane reads ONE value v = H[p+k] per
+//      (view, slice) and accumulates Q[p] += wq*v, P[p] += wp*v; at the end out[p] = Q[p] + P[p+1].
+//      wq, wp, k are wave-uniform (scalar loads from the FP64-built table), so the inner step is one
+//      ds_read_b32 and two v_fmac_f32 per bin per slice.  Chunks that cannot touch the volume's
+//      footprint for the whole slice block are skipped with a uniform branch.
+//
+// Back projection (two kernels, no atomics):
+//   back_h2_kernel, one workgroup per (u1 row, v1 view row, group of a): the exact transpose of step 2.
+//      Lane = slice; the workgroup streams the v0 rows g[v1][:, u1, :] (scaled, masked) through LDS and
+//      each lane accumulates its slice's column H2[s][a] = sum_v0 wp*S[a+m] + wq*S[a+m+1] in registers
+//      (sliding window: one LDS read and two FMAs per (view, voxel)).  The slab is written transposed
+//      through LDS to workspace[v1][u1][s][a].
+//   back_z_kernel, one wave per (a chunk, slice, z chunk): the transpose of step 1, gathering the two
+//      u1 rows of every v1 for each voxel, f[iz][s][a] += sum_v1 wb*H2[iz+m1] + wa*H2[iz+m1+1].  In PETL
+//      mode the reference's voxel-centre u1 test (src/projectors.cu:284-289) is applied per voxel here.
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include <cstring>
+#include "petl_common.h"
+#include "proj_kernels.h"
+#include "proj_fast.h"
+
+namespace petl {
+
+namespace {
+
+constexpr int FWD_THREADS = 512;
+constexpr int FWD_WAVES = FWD_THREADS / 64;
+constexpr int FWD_SB = 8;     // slices per LDS block in the forward
+constexpr int BACK_RB = 8;    // v0 rows per LDS block in the back projector
+constexpr int BACK_AG = 128;  // a-values per lane in the back projector
+constexpr int BACK_TW = 32;   // transpose tile width of the back epilogue
+
+__device__ __forceinline__ float coordf(int i, float T, float s)
+{
+#pragma clang fp contract(off)
+    return float(i) * T + s;
+}
+__device__ __forceinline__ bool u0_masked(const PlanGeom& P, float u0, float v0)
+{
+#pragma clang fp contract(off)
+    return fabsf(u0) - 0.5f * P.Tu0 > P.L + P.R * fabsf(v0);
+}
+__device__ __forceinline__ bool u1_masked(const PlanGeom& P, float u1, float v1)
+{
+#pragma clang fp contract(off)
+    return fabsf(u1) - 0.5f * P.Tu1 > P.H - P.R * fabsf(v1);
+}
+
+struct FwdArgs {
+    PlanGeom P;
+    int Ns, Na, Nz;
+    int64_t fs_s, fs_a, fs_z;
+    int hpad, hrow;
+    int psi90;
+    int v1_first, v1_stride;
+};
+
+// =============================================================================================
+// forward
+// =============================================================================================
+template <int J, int C>
+__global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict__ g, const float* __restrict__ f,
+                                                               FwdArgs A, const int4* __restrict__ xtab,
+                                                               const int4* __restrict__ ztab,
+                                                               const float* __restrict__ fscale, int epilogue,
+                                                               const float* __restrict__ yv,
+                                                               const float* __restrict__ resp)
+{
+    extern __shared__ float hbuf[];
+    __shared__ int s_lo, s_hi;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int grp = blockIdx.x, iu1 = blockIdx.y, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const PlanGeom& P = A.P;
+    const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1;
+    const int Ns = A.Ns, Na = A.Na, Nz = A.Nz;
+    const int iv0_base = grp * (FWD_WAVES * J) + wave;
+
+    const float v1 = coordf(iv1, P.Tv1, P.v1_0);
+    const float u1 = coordf(iu1, P.Tu1, P.u1_0);
+    if (u1_masked(P, u1, v1)) {
+        // whole u1 row is masked (src/projectors.cu:38-43); a ratio of a zero projection is zero too
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int iv0 = iv0_base + FWD_WAVES * j;
+            if (iv0 < Nv0) {
+                float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0;
+                for (int p = lane; p < Nu0; p += 64)
+                    gr[p] = 0.0f;
+            }
+        }
+        return;
+    }
+
+    // slices whose z-interpolated row can be non-zero
+    if (tid == 0) {
+        s_lo = Ns;
+        s_hi = -1;
+    }
+    const int hbuf_elems = 2 * FWD_SB * A.hrow;
+    for (int i = tid; i < hbuf_elems; i += FWD_THREADS)
+        hbuf[i] = 0.0f;
+    __syncthreads();
+    for (int is = tid; is < Ns; is += FWD_THREADS) {
+        const int4 e = ztab[int64_t(iv1) * Ns + is];
+        const int iza = iu1 - e.x - 1;
+        const bool nz = (iza >= 0 && iza < Nz && __int_as_float(e.y) > 0.0f) ||
+                        (iza + 1 >= 0 && iza + 1 < Nz && __int_as_float(e.z) > 0.0f);
+        if (nz) {
+            atomicMin(&s_lo, is);
+            atomicMax(&s_hi, is);
+        }
+    }
+    __syncthreads();
+    const int is_lo = __builtin_amdgcn_readfirstlane(s_lo), is_hi = __builtin_amdgcn_readfirstlane(s_hi);
+
+    float Q[J][C], Pa[J][C];
+#pragma unroll
+    for (int j = 0; j < J; j++)
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            Q[j][c] = 0.0f;
+            Pa[j][c] = 0.0f;
+        }
+
+    if (is_lo <= is_hi) {
+        const int nblk = (is_hi - is_lo) / FWD_SB + 1;
+        // z interpolation of slices [is0, is0+SB) into LDS buffer b
+        auto build = [&](int blk, int b) {
+            const int is0 = is_lo + blk * FWD_SB;
+            float* hb = hbuf + b * FWD_SB * A.hrow + A.hpad;
+            const int n = FWD_SB * Na;
+            for (int idx = tid; idx < n; idx += FWD_THREADS) {
+                int sl, ia;
+                if (A.psi90) {
+                    sl = idx % FWD_SB;
+                    ia = idx / FWD_SB;
+                } else {
+                    sl = idx / Na;
+                    ia = idx - sl * Na;
+                }
+                const int is = is0 + sl;
+                float val = 0.0f;
+                if (is <= is_hi) {
+                    const int4 e = ztab[int64_t(iv1) * Ns + is];
+                    const int iza = iu1 - e.x - 1;
+                    const float* fp = f + int64_t(is) * A.fs_s + int64_t(ia) * A.fs_a;
+                    if (iza >= 0 && iza < Nz)
+                        val = __int_as_float(e.y) * fp[int64_t(iza) * A.fs_z];
+                    if (iza + 1 >= 0 && iza + 1 < Nz)
+                        val += __int_as_float(e.z) * fp[int64_t(iza + 1) * A.fs_z];
+                }
+                hb[sl * A.hrow + ia] = val;
+            }
+        };
+        build(0, 0);
+        __syncthreads();
+        for (int blk = 0; blk < nblk; blk++) {
+            if (blk + 1 < nblk)
+                build(blk + 1, (blk + 1) & 1);
+            const int is0 = is_lo + blk * FWD_SB;
+            const float* hb = hbuf + (blk & 1) * FWD_SB * A.hrow + A.hpad + lane;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                const int iv0 = iv0_base + FWD_WAVES * j;
+                if (iv0 < Nv0) {
+                    int k[FWD_SB];
+                    float wq[FWD_SB], wp[FWD_SB];
+                    const int4* xt = xtab + int64_t(iv0) * Ns;
+#pragma unroll
+                    for (int s = 0; s < FWD_SB; s++) {
+                        const int4 e = xt[min(is0 + s, is_hi)];
+                        k[s] = e.x;
+                        wq[s] = __int_as_float(e.y);
+                        wp[s] = __int_as_float(e.z);
+                    }
+                    int kmin = k[0], kmax = k[0];
+#pragma unroll
+                    for (int s = 1; s < FWD_SB; s++) {
+                        kmin = min(kmin, k[s]);
+                        kmax = max(kmax, k[s]);
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        if (64 * c <= Na - 1 - kmin && 64 * c + 63 >= -kmax) {
+#pragma unroll
+                            for (int s = 0; s < FWD_SB; s++) {
+                                const float v = hb[s * A.hrow + 64 * c + k[s]];
+                                Q[j][c] += wq[s] * v;
+                                Pa[j][c] += wp[s] * v;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // epilogue: out[p] = (Q[p] + P[p+1]) * scale, masks, optional ratio
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int iv0 = iv0_base + FWD_WAVES * j;
+        if (iv0 < Nv0) {
+            const float sc = fscale[int64_t(iv1) * Nv0 + iv0];
+            const float v0 = coordf(iv0, P.Tv0, P.v0_0);
+            float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float pn = __shfl_down(Pa[j][c], 1);
+                const float nxt = (c + 1 < C) ? __shfl(Pa[j][(c + 1 < C) ? c + 1 : c], 0) : 0.0f;
+                if (lane == 63)
+                    pn = nxt;
+                const int p = 64 * c + lane;
+                if (p < Nu0) {
+                    float out = (Q[j][c] + pn) * sc;
+                    if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
+                        out = 0.0f;
+                    const int64_t gi = int64_t(gr - g) + p;
+                    if (resp)
+                        out = out * resp[gi];
+                    if (epilogue == EPI_RATIO) {
+                        out = (out == 0.0f) ? 0.0f : yv[gi] / out;
+                        if (resp)
+                            out = out * resp[gi];
+                    }
+                    gr[p] = out;
+                }
+            }
+        }
+    }
+}
+
+// =============================================================================================
+// back projection, stage 1: u0 gather of every v0 row into per-slice columns
+// =============================================================================================
+struct BackArgs {
+    PlanGeom P;
+    VolGeom V;
+    int Ns, Na, Nz;
+    int64_t fs_s, fs_a, fs_z;
+    int spad_lo, srow;
+    int mode;
+    int psi90;
+    int v1_first, v1_stride;
+};
+
+template <int AG>
+__global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ g, float* __restrict__ h2, BackArgs A,
+                                                      const int4* __restrict__ xtab,
+                                                      const float* __restrict__ scale,
+                                                      const int2* __restrict__ rows)
+{
+    extern __shared__ float sbuf[];
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int ag = blockIdx.x, iu1 = blockIdx.y, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const PlanGeom& P = A.P;
+    const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1, Ns = A.Ns, Na = A.Na;
+    const int2 rr = rows[iv1];
+    if (iu1 < rr.x || iu1 > rr.y)
+        return;
+    const int a0 = ag * AG;
+    const int is = min(tid, Ns - 1);
+
+    float acc[AG];
+#pragma unroll
+    for (int a = 0; a < AG; a++)
+        acc[a] = 0.0f;
+
+    const int buf_elems = 2 * BACK_RB * A.srow;
+    for (int i = tid; i < buf_elems; i += nthr)
+        sbuf[i] = 0.0f;
+    __syncthreads();
+
+    auto load = [&](int blk, int b) {
+        float* sb = sbuf + b * BACK_RB * A.srow + A.spad_lo;
+        const int n = BACK_RB * Nu0;
+        for (int idx = tid; idx < n; idx += nthr) {
+            const int r = idx / Nu0, p = idx - r * Nu0;
+            const int iv0 = blk * BACK_RB + r;
+            float val = 0.0f;
+            if (iv0 < Nv0) {
+                val = g[((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0 + p] * scale[int64_t(iv1) * Nv0 + iv0];
+                if (A.mode == BACK_MATCHED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
+                    val = 0.0f;
+            }
+            sb[r * A.srow + p] = val;
+        }
+    };
+
+    const int nblk = (Nv0 + BACK_RB - 1) / BACK_RB;
+    load(0, 0);
+    __syncthreads();
+    for (int blk = 0; blk < nblk; blk++) {
+        if (blk + 1 < nblk)
+            load(blk + 1, (blk + 1) & 1);
+        const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
+#pragma unroll 1
+        for (int r = 0; r < BACK_RB; r++) {
+            const int iv0 = blk * BACK_RB + r;
+            if (iv0 < Nv0) {
+                const int4 e = xtab[int64_t(iv0) * Ns + is];
+                const int m = -e.x - 1;
+                const float wq = __int_as_float(e.y), wp = __int_as_float(e.z);
+                const float* s = sb + r * A.srow + m;
+                float xp = s[0];
+#pragma unroll
+                for (int a = 0; a < AG; a++) {
+                    const float x = s[a + 1];
+                    acc[a] += wp * xp;
+                    acc[a] += wq * x;
+                    xp = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // transposed write: workspace[iv1][iu1][s][a]
+    float* tile = sbuf;
+    float* out = h2 + (int64_t(iv1) * Nu1 + iu1) * Ns * int64_t(Na);
+#pragma unroll
+    for (int q = 0; q < AG / BACK_TW; q++) {
+        if (tid < Ns) {
+#pragma unroll
+            for (int t = 0; t < BACK_TW; t++)
+                tile[tid * (BACK_TW + 1) + t] = acc[q * BACK_TW + t];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < Ns * BACK_TW; idx += nthr) {
+            const int s = idx / BACK_TW, t = idx - s * BACK_TW;
+            const int a = a0 + q * BACK_TW + t;
+            if (a < Na)
+                out[int64_t(s) * Na + a] = tile[s * (BACK_TW + 1) + t];
+        }
+        __syncthreads();
+    }
+}
+
+// =============================================================================================
+// back projection, stage 2: u1 gather over all v1 rows, per voxel
+// =============================================================================================
+template <int ZC>
+__global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2, float* __restrict__ f, BackArgs A,
+                                                    const int4* __restrict__ ztab, const int2* __restrict__ rows)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x;
+    const int ia = blockIdx.x * 64 + lane;
+    const int is = blockIdx.y;
+    const int z0 = blockIdx.z * ZC;
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int Ns = A.Ns, Na = A.Na, Nz = A.Nz, Nu1 = P.Nu1;
+    const int iac = min(ia, Na - 1);
+    const bool petl = (A.mode == BACK_PETL);
+
+    // voxel-centre coordinates for the PETL-mode test
+    const int ix = A.psi90 ? is : iac;
+    const int iy = A.psi90 ? iac : is;
+    const float xc = coordf(ix, V.Tx, V.x0), yc = coordf(iy, V.Ty, V.y0);
+    const float tv = yc * P.cos_psi - xc * P.sin_psi;
+
+    float acc[ZC];
+#pragma unroll
+    for (int i = 0; i < ZC; i++)
+        acc[i] = 0.0f;
+
+    const int64_t slab = int64_t(Ns) * Na;
+    const float* hb = h2 + int64_t(is) * Na + iac;
+    for (int iv1 = A.v1_first; iv1 < P.Nv1; iv1 += A.v1_stride) {
+        const int4 e = ztab[int64_t(iv1) * Ns + is];
+        const int2 rr = rows[iv1];
+        const int r0 = z0 + e.x;
+        const float wa = __int_as_float(e.y), wb = __int_as_float(e.z);
+        const float* hv = hb + int64_t(iv1) * Nu1 * slab;
+        float x[ZC + 1];
+#pragma unroll
+        for (int i = 0; i <= ZC; i++) {
+            const int r = r0 + i;
+            x[i] = (r >= rr.x && r <= rr.y) ? hv[int64_t(r) * slab] : 0.0f;
+        }
+        if (petl) {
+            const float v1 = coordf(iv1, P.Tv1, P.v1_0);
+            const float thr = P.H - P.R * fabsf(v1);
+            const float v1t = v1 * tv;
+#pragma unroll
+            for (int i = 0; i < ZC; i++) {
+                const float zc = coordf(z0 + i, V.Tz, V.z0);
+                const float u1 = zc + v1t;
+                const float c = wb * x[i] + wa * x[i + 1];
+                acc[i] += (fabsf(u1) - 0.5f * P.Tu1 <= thr) ? c : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < ZC; i++)
+                acc[i] += wb * x[i] + wa * x[i + 1];
+        }
+    }
+    if (ia < Na) {
+        float* fp = f + int64_t(is) * A.fs_s + int64_t(ia) * A.fs_a;
+#pragma unroll
+        for (int i = 0; i < ZC; i++)
+            if (z0 + i < Nz)
+                fp[int64_t(z0 + i) * A.fs_z] += acc[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// host-side kernel selection
+// ---------------------------------------------------------------------------------------------
+typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int4*, const int4*, const float*, int, const float*,
+                       const float*);
+
+template <int J>
+fwd_fn pick_fwd_c(int C)
+{
+    switch (C) {
+    case 1: return fwd_fast_kernel<J, 1>;
+    case 2: return fwd_fast_kernel<J, 2>;
+    case 3: return fwd_fast_kernel<J, 3>;
+    case 4: return fwd_fast_kernel<J, 4>;
+    case 5: return fwd_fast_kernel<J, 5>;
+    case 6: return fwd_fast_kernel<J, 6>;
+    case 7: return fwd_fast_kernel<J, 7>;
+    case 8: return fwd_fast_kernel<J, 8>;
+    default: return nullptr;
+    }
+}
+
+fwd_fn pick_fwd(int J, int C)
+{
+    if (J == 13)
+        return pick_fwd_c<13>(C);
+    if (J == 7)
+        return pick_fwd_c<7>(C);
+    return nullptr;
+}
+
+inline float fbits(int i)
+{
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
+inline int ibits(float f)
+{
+    int i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+
+inline float coordh(int i, float T, float s)
+{
+    volatile float a = float(i) * T;
+    return a + s;
+}
+
+}  // namespace
+
+FastPlan::~FastPlan()
+{
+    if (dev_block)
+        (void)hipFree(dev_block);
+}
+
+std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
+{
+    int psi90;
+    if (P.sin_psi == 0.0f && P.cos_psi == 1.0f)
+        psi90 = 0;
+    else if (P.sin_psi == 1.0f && std::fabs(P.cos_psi) < 1.0e-6f)
+        psi90 = 1;
+    else
+        return nullptr;
+    if (!(V.Tx == P.Tu0 && V.Ty == P.Tu0 && V.Tz == P.Tu1))
+        return nullptr;
+    // every view must take the same slice branch as this kernel assumes (src/projectors.cu:77)
+    for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+        const float v0 = coordh(iv0, P.Tv0, P.v0_0);
+        volatile float t1 = v0 * P.cos_psi;
+        volatile float t2 = v0 * P.sin_psi;
+        const float rx = -(t1 + P.sin_psi), ry = -(t2 - P.cos_psi);
+        const bool xs = std::fabs(rx) >= std::fabs(ry);
+        if (xs != (psi90 == 1))
+            return nullptr;
+    }
+
+    auto fp = std::make_unique<FastPlan>();
+    fp->P = P;
+    fp->V = V;
+    fp->psi90 = psi90;
+    fp->Nz = V.Nz;
+    fp->fs_z = int64_t(V.Nx) * V.Ny;
+    if (!psi90) {
+        fp->Ns = V.Ny;
+        fp->Na = V.Nx;
+        fp->fs_s = V.Nx;
+        fp->fs_a = 1;
+    } else {
+        fp->Ns = V.Nx;
+        fp->Na = V.Ny;
+        fp->fs_s = 1;
+        fp->fs_a = V.Nx;
+    }
+    const int Ns = fp->Ns, Na = fp->Na;
+    // forward shape: C chunks of 64 lanes cover the Nu0+1 positions the P/Q split needs
+    fp->C = (P.Nu0 + 1 + 63) / 64;
+    if (fp->C > 8 || Ns > 1024)
+        return nullptr;
+    fp->J = (2 * 13 * fp->C <= 112) ? 13 : 7;
+    fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
+    fp->SB = FWD_SB;
+    fp->AG = BACK_AG;
+    fp->nagroups = (Na + BACK_AG - 1) / BACK_AG;
+    fp->RB = BACK_RB;
+    fp->ZC = 32;
+    if (Ns > 256)  // back_h2_kernel has one lane per slice, at most 256
+        return nullptr;
+
+    // ---- FP64 tables ----
+    const double x0 = V.x0, y0 = V.y0, z0 = V.z0;
+    const double cs = P.cos_psi;
+    std::vector<int4> xtab(size_t(P.Nv0) * Ns), ztab(size_t(P.Nv1) * Ns);
+    std::vector<float> fscale(size_t(P.Nv1) * P.Nv0), pscale(size_t(P.Nv1) * P.Nv0);
+    std::vector<int2> rows(2 * size_t(P.Nv1));
+    int kspread = 0, mmin = 1 << 30, mmax = -(1 << 30);
+    for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+        const double v0 = coordh(iv0, P.Tv0, P.v0_0);
+        double alpha, beta;
+        if (!psi90) {
+            alpha = (x0 + y0 * v0 - double(P.u0_0)) / double(P.Tu0);
+            beta = double(V.Ty) * v0 / double(P.Tu0);
+        } else {
+            const double kap = -(v0 - cs) / (1.0 + v0 * cs);
+            alpha = (y0 + x0 * kap - double(P.u0_0)) / double(P.Tu0);
+            beta = double(V.Tx) * kap / double(P.Tu0);
+        }
+        for (int is = 0; is < Ns; is++) {
+            const double mu = alpha + double(is) * beta;
+            const double m = std::floor(mu);
+            const double phi = mu - m;
+            const int mi = int(m);
+            xtab[size_t(iv0) * Ns + is] = make_int4(-mi - 1, ibits(float(phi)), ibits(float(1.0 - phi)), 0);
+            mmin = std::min(mmin, mi);
+            mmax = std::max(mmax, mi);
+        }
+        for (int is = 0; is < Ns; is++) {
+            const int lo = std::max(0, is - FWD_SB + 1);
+            const int kd = std::abs(xtab[size_t(iv0) * Ns + is].x - xtab[size_t(iv0) * Ns + lo].x);
+            kspread = std::max(kspread, kd);
+        }
+    }
+    for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
+        const double v1 = coordh(iv1, P.Tv1, P.v1_0);
+        double gam, del;
+        if (!psi90) {
+            gam = (z0 + y0 * v1 - double(P.u1_0)) / double(P.Tu1);
+            del = double(V.Ty) * v1 / double(P.Tu1);
+        } else {
+            gam = (z0 - x0 * v1 - double(P.u1_0)) / double(P.Tu1);
+            del = -double(V.Tx) * v1 / double(P.Tu1);
+        }
+        for (int is = 0; is < Ns; is++) {
+            const double nu = gam + double(is) * del;
+            const double m = std::floor(nu);
+            const double ps = nu - m;
+            ztab[size_t(iv1) * Ns + is] = make_int4(int(m), ibits(float(ps)), ibits(float(1.0 - ps)), 0);
+        }
+        // active u1 rows.  matched: unmasked rows.  PETL: rows a voxel passing the centre test can touch.
+        const float v1f = coordh(iv1, P.Tv1, P.v1_0);
+        volatile float rv = P.R * std::fabs(v1f);
+        const float thr = P.H - rv;
+        int lo_m = P.Nu1, hi_m = -1, lo_p = P.Nu1, hi_p = -1;
+        for (int iu1 = 0; iu1 < P.Nu1; iu1++) {
+            const float u1 = coordh(iu1, P.Tu1, P.u1_0);
+            volatile float d = std::fabs(u1) - 0.5f * P.Tu1;
+            if (!(d > thr)) {
+                lo_m = std::min(lo_m, iu1);
+                hi_m = std::max(hi_m, iu1);
+            }
+            if (std::fabs(u1) <= thr + P.Tu1 + 0.5f * V.Tz + 0.01f * P.Tu1) {
+                lo_p = std::min(lo_p, iu1);
+                hi_p = std::max(hi_p, iu1);
+            }
+        }
+        rows[iv1] = make_int2(lo_m, hi_m);
+        rows[P.Nv1 + iv1] = make_int2(lo_p, hi_p);
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const float v0 = coordh(iv0, P.Tv0, P.v0_0);
+            volatile float q0 = v0 * v0;
+            volatile float q1 = v1f * v1f;
+            volatile float sum = 1.0f + q0;
+            sum = sum + q1;
+            const float obl = 1.0f / std::sqrt(float(sum));
+            const float l_phi = psi90 ? V.Tx * std::fabs(1.0f / P.sin_psi) : V.Tx * std::fabs(1.0f / P.cos_psi);
+            fscale[size_t(iv1) * P.Nv0 + iv0] = l_phi * obl;
+            volatile float a1 = v0 * P.cos_psi;
+            volatile float a2 = v0 * P.sin_psi;
+            const float wy = a1 + P.sin_psi, wx = P.cos_psi - a2;
+            const float lp = 1.0f / std::max(std::fabs(wy), std::fabs(wx)) * obl;
+            pscale[size_t(iv1) * P.Nv0 + iv0] = lp * V.Tx;
+        }
+    }
+    fp->hpad = 64 + kspread + 1;
+    fp->hrow = Na + 2 * fp->hpad;
+    if ((fp->hrow & 1) == 0)
+        fp->hrow += 1;
+    // back: S rows read at indices [m, m + NaPad] for m in [mmin, mmax]
+    const int na_pad = fp->nagroups * BACK_AG;
+    fp->spad_lo = std::max(0, -mmin);
+    fp->srow = fp->spad_lo + std::max(P.Nu0, mmax + na_pad + 1) + 1;
+    if ((fp->srow & 1) == 0)
+        fp->srow += 1;
+    if (size_t(2) * FWD_SB * fp->hrow * 4 > 64 * 1024 || size_t(2) * BACK_RB * fp->srow * 4 > 64 * 1024)
+        return nullptr;
+
+    // ---- upload ----
+    const size_t nx = xtab.size() * sizeof(int4), nzb = ztab.size() * sizeof(int4);
+    const size_t nf = fscale.size() * sizeof(float), nr = rows.size() * sizeof(int2);
+    auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
+    const size_t total = al(nx) + al(nzb) + 2 * al(nf) + al(nr);
+    char* blk = nullptr;
+    PETL_HIP_CHECK(hipMalloc(&blk, total));
+    fp->dev_block = blk;
+    size_t off = 0;
+    fp->dev.xtab = reinterpret_cast<int4*>(blk + off);
+    off += al(nx);
+    fp->dev.ztab = reinterpret_cast<int4*>(blk + off);
+    off += al(nzb);
+    fp->dev.fscale = reinterpret_cast<float*>(blk + off);
+    off += al(nf);
+    fp->dev.pscale = reinterpret_cast<float*>(blk + off);
+    off += al(nf);
+    fp->dev.rows = reinterpret_cast<int2*>(blk + off);
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.xtab, xtab.data(), nx, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.ztab, ztab.data(), nzb, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.fscale, fscale.data(), nf, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.pscale, pscale.data(), nf, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.rows, rows.data(), nr, hipMemcpyHostToDevice));
+    return fp;
+}
+
+void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
+                  hipStream_t s, int v1_first, int v1_stride)
+{
+    FwdArgs A;
+    A.P = fp.P;
+    A.Ns = fp.Ns;
+    A.Na = fp.Na;
+    A.Nz = fp.Nz;
+    A.fs_s = fp.fs_s;
+    A.fs_a = fp.fs_a;
+    A.fs_z = fp.fs_z;
+    A.hpad = fp.hpad;
+    A.hrow = fp.hrow;
+    A.psi90 = fp.psi90;
+    A.v1_first = v1_first;
+    A.v1_stride = v1_stride;
+    const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    fwd_fn fn = pick_fwd(fp.J, fp.C);
+    if (!fn)
+        throw hip_error("fast forward: no kernel instance for this shape");
+    const size_t lds = size_t(2) * FWD_SB * fp.hrow * sizeof(float);
+    hipLaunchKernelGGL(fn, dim3(fp.ngroups, fp.P.Nu1, nv1), dim3(FWD_THREADS), lds, s, g, f, A, fp.dev.xtab,
+                       fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
+    PETL_LAUNCH_CHECK();
+}
+
+size_t fast_back_workspace_bytes(const FastPlan& fp)
+{
+    return size_t(fp.P.Nv1) * fp.P.Nu1 * size_t(fp.Ns) * fp.Na * sizeof(float);
+}
+
+void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, hipStream_t s, int v1_first,
+               int v1_stride)
+{
+    BackArgs A;
+    A.P = fp.P;
+    A.V = fp.V;
+    A.Ns = fp.Ns;
+    A.Na = fp.Na;
+    A.Nz = fp.Nz;
+    A.fs_s = fp.fs_s;
+    A.fs_a = fp.fs_a;
+    A.fs_z = fp.fs_z;
+    A.spad_lo = fp.spad_lo;
+    A.srow = fp.srow;
+    A.mode = mode;
+    A.psi90 = fp.psi90;
+    A.v1_first = v1_first;
+    A.v1_stride = v1_stride;
+    const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    const int2* rows = fp.dev.rows + (mode == BACK_PETL ? fp.P.Nv1 : 0);
+    const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
+    const int nthr = ((fp.Ns + 63) / 64) * 64;
+    const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
+    hipLaunchKernelGGL(back_h2_kernel<BACK_AG>, dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s, g,
+                       workspace, A, fp.dev.xtab, scale, rows);
+    PETL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(back_z_kernel<32>, dim3((fp.Na + 63) / 64, fp.Ns, (fp.Nz + 31) / 32), dim3(64), 0, s,
+                       workspace, f, A, fp.dev.ztab, rows);
+    PETL_LAUNCH_CHECK();
+}
+
+}  // namespace petl

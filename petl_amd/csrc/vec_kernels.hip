@@ -1,0 +1,177 @@
+// Elementwise and reduction kernels of the device-resident reconstruction loop.
+#include "petl_common.h"
+#include "vec_kernels.h"
+#include "priors_dev.h"
+
+namespace petl {
+
+namespace {
+
+constexpr int VT = 256;
+
+inline int grid_for(int64_t n) { return int(std::min<int64_t>((n + VT - 1) / VT, 65536)); }
+
+__global__ void fill_kernel(float* __restrict__ x, float v, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = v;
+}
+
+__global__ void mul_kernel(float* __restrict__ x, const float* __restrict__ y, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = x[i] * y[i];
+}
+
+__global__ void sens_fix_kernel(float* __restrict__ S, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        if (S[i] == 0.0f)
+            S[i] = 1.0f;
+}
+
+__global__ void mlem_update_kernel(float* __restrict__ f, const float* __restrict__ d, const float* __restrict__ S,
+                                   int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+        const float s = S[i];
+        const float q = (s == 0.0f) ? 0.0f : d[i] / s;
+        f[i] = f[i] * q;
+    }
+}
+
+__global__ void ratio_kernel(float* __restrict__ r, const float* __restrict__ y, const float* __restrict__ q,
+                             const float* __restrict__ resp, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+        const float qq = q[i];
+        float out = (qq == 0.0f) ? 0.0f : y[i] / qq;
+        if (resp)
+            out = out * resp[i];
+        r[i] = out;
+    }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64) mlem_update_prior_kernel(float* __restrict__ fo, const float* __restrict__ f,
+                                                               const float* __restrict__ d,
+                                                               const float* __restrict__ S, int N1, int N2, int N3,
+                                                               float delta, float beta, HuberParams h)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= N3)
+        return;
+    const int64_t v = (int64_t(i) * N2 + j) * N3 + k;
+    const float s = S[v];
+    float G = prior_grad_at<KIND>(f, i, j, k, N1, N2, N3, delta, beta, h);
+    G = G + s;
+    G = fmaxf(0.1f * s, G);
+    const float q = (G == 0.0f) ? 0.0f : d[v] / G;
+    fo[v] = f[v] * q;
+}
+
+// pass 1: per-block partial sums in FP64
+__global__ void __launch_bounds__(VT) loglik_partial_kernel(const float* __restrict__ y, const float* __restrict__ q,
+                                                            int64_t n, double* __restrict__ part)
+{
+    __shared__ double red[VT];
+    double acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+        const double qq = q[i];
+        if (qq > 0.0)
+            acc += double(y[i]) * log(qq) - qq;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = VT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0];
+}
+
+// pass 2: one block sums the partials in a fixed order
+__global__ void __launch_bounds__(VT) loglik_final_kernel(const double* __restrict__ part, int nparts,
+                                                          double* __restrict__ out, int accumulate)
+{
+    __shared__ double red[VT];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += VT)
+        acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = VT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *out = accumulate ? (*out + red[0]) : red[0];
+}
+
+}  // namespace
+
+void launch_fill(float* x, float v, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(VT), 0, s, x, v, n);
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_mul(float* x, const float* y, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(mul_kernel, dim3(grid_for(n)), dim3(VT), 0, s, x, y, n);
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_sens_fix(float* S, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(sens_fix_kernel, dim3(grid_for(n)), dim3(VT), 0, s, S, n);
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_mlem_update(float* f, const float* d, const float* S, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(mlem_update_kernel, dim3(grid_for(n)), dim3(VT), 0, s, f, d, S, n);
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_mlem_update_prior(float* f_out, const float* f, const float* d, const float* S, int N1, int N2, int N3,
+                              int kind, float delta, float beta, float p, hipStream_t s)
+{
+    const dim3 grid(ceil_div(N3, 64), N2, N1);
+    if (kind == 0) {
+        HuberParams h{};
+        hipLaunchKernelGGL(mlem_update_prior_kernel<0>, grid, dim3(64), 0, s, f_out, f, d, S, N1, N2, N3, delta, beta,
+                           h);
+    } else {
+        const HuberParams h = huber_params(delta, beta, p);
+        hipLaunchKernelGGL(mlem_update_prior_kernel<1>, grid, dim3(64), 0, s, f_out, f, d, S, N1, N2, N3, delta, beta,
+                           h);
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_ratio(float* r, const float* y, const float* q, const float* resp, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(ratio_kernel, dim3(grid_for(n)), dim3(VT), 0, s, r, y, q, resp, n);
+    PETL_LAUNCH_CHECK();
+}
+
+int64_t ll_scratch_doubles(int64_t n) { return std::min<int64_t>((n + VT - 1) / VT, 4096); }
+
+void launch_loglik(const float* y, const float* q, int64_t n, double* scratch, double* out, bool accumulate,
+                   hipStream_t s)
+{
+    const int nparts = int(ll_scratch_doubles(n));
+    hipLaunchKernelGGL(loglik_partial_kernel, dim3(nparts), dim3(VT), 0, s, y, q, n, scratch);
+    PETL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(loglik_final_kernel, dim3(1), dim3(VT), 0, s, scratch, nparts, out, accumulate ? 1 : 0);
+    PETL_LAUNCH_CHECK();
+}
+
+}  // namespace petl

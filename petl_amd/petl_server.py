@@ -1,0 +1,531 @@
+"""PETLserver: ctypes bindings of the MI355X PETL library with the reference's Python API.
+
+This mirrors the part of the reference's ``PETLserver`` class (src/petl_server.py) that sits on the
+projector / MLEM path -- geometry, volume, allocation, project, backproject, sensitivity, elementwise
+helpers, MLEM and the TV / relative-difference prior gradients -- with the same method names, argument
+meanings, defaults and error behaviour (``TypeError('project failed')`` etc.), so PETL scripts that use
+that path run unchanged.  The compute runs in libpetl.so (HIP kernels for gfx950); the library is
+required and loading fails loudly without it.
+
+Additions beyond the reference: ``set_backprojector_mode``, ``project_gpu`` / ``backproject_gpu`` on
+torch tensors, and a device-resident ``MLEM`` (with optional OSEM subsets and log-likelihood trace).
+"""
+import ctypes
+import glob
+import os
+
+import numpy as np
+from numpy.ctypeslib import ndpointer
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_f32 = ndpointer(ctypes.c_float, flags="C_CONTIGUOUS")
+
+_SIGNATURES = {
+    "set_model": (ctypes.c_bool, [ctypes.c_int]),
+    "create_new_model": (ctypes.c_int, []),
+    "about": (None, []),
+    "copy_parameters": (ctypes.c_bool, [ctypes.c_int]),
+    "clearAll": (None, []),
+    "print_parameters": (None, []),
+    "volume_defined": (ctypes.c_bool, []),
+    "geometry_defined": (ctypes.c_bool, []),
+    "get_numX": (ctypes.c_int, []), "set_numX": (ctypes.c_bool, [ctypes.c_int]),
+    "get_numY": (ctypes.c_int, []), "set_numY": (ctypes.c_bool, [ctypes.c_int]),
+    "get_numZ": (ctypes.c_int, []), "set_numZ": (ctypes.c_bool, [ctypes.c_int]),
+    "get_voxelWidth": (ctypes.c_float, []), "set_voxelWidth": (ctypes.c_bool, [ctypes.c_float]),
+    "get_voxelHeight": (ctypes.c_float, []), "set_voxelHeight": (ctypes.c_bool, [ctypes.c_float]),
+    "get_offsetX": (ctypes.c_float, []), "set_offsetX": (ctypes.c_bool, [ctypes.c_float]),
+    "get_offsetY": (ctypes.c_float, []), "set_offsetY": (ctypes.c_bool, [ctypes.c_float]),
+    "get_offsetZ": (ctypes.c_float, []), "set_offsetZ": (ctypes.c_bool, [ctypes.c_float]),
+    "get_numPlanograms": (ctypes.c_int, []),
+    "get_planogramSize": (ctypes.c_bool, [ctypes.c_int, ndpointer(ctypes.c_int, flags="C_CONTIGUOUS")]),
+    "add_planogram": (ctypes.c_bool, [ctypes.c_float] * 7),
+    "remove_planogram": (ctypes.c_bool, [ctypes.c_int]),
+    "keep_only_planogram": (ctypes.c_bool, [ctypes.c_int]),
+    "set_default_volume": (ctypes.c_bool, [ctypes.c_float]),
+    "set_volume": (ctypes.c_bool, [ctypes.c_int] * 3 + [ctypes.c_float] * 5),
+    "project": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32]),
+    "backproject": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32]),
+    "multiply3D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 3),
+    "multiply4D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 4),
+    "divide3D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 3),
+    "divide4D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 4),
+    "rdivide3D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 3),
+    "rdivide4D": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 4),
+    "reciprocal3D": (ctypes.c_bool, [_f32] + [ctypes.c_int] * 3 + [ctypes.c_float]),
+    "reciprocal4D": (ctypes.c_bool, [_f32] + [ctypes.c_int] * 4 + [ctypes.c_float]),
+    "inner_product3D": (ctypes.c_float, [_c_float_p, _c_float_p, _c_float_p] + [ctypes.c_int] * 3),
+    "inner_product4D": (ctypes.c_float, [_c_float_p, _c_float_p, _c_float_p] + [ctypes.c_int] * 4),
+    "TV_gradient": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 +
+                    [ctypes.c_float] * 3 + [ctypes.c_bool]),
+    "relativeDifferences_gradient": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 3 + [ctypes.c_float] * 2),
+    "set_backprojector_mode": (ctypes.c_bool, [ctypes.c_int]),
+    "get_backprojector_mode": (ctypes.c_int, []),
+    "set_projector_path": (ctypes.c_bool, [ctypes.c_int]),
+    "get_projector_path": (ctypes.c_int, []),
+    "planogram_uses_fast_path": (ctypes.c_int, [ctypes.c_int]),
+    "project_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _c_float_p, ctypes.c_void_p, ctypes.c_bool]),
+    "backproject_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _c_float_p, ctypes.c_void_p, ctypes.c_bool]),
+    "MLEM_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(_c_float_p),
+                                 ctypes.POINTER(_c_float_p), ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def library_path():
+    return os.path.join(_HERE, "libpetl.so")
+
+
+def load_library(path=None):
+    """Loads libpetl.so (once per process) and declares every signature."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or library_path()
+    if not os.path.isfile(path):
+        raise RuntimeError(f"PETL library not found at {path}; build it with `python -m petl_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PETLserver:
+    """Python interface of the PETL projector library (one instance = one PETL model)."""
+
+    BACK_PETL = 0
+    BACK_MATCHED = 1
+
+    def __init__(self, param_id=None, lib_dir=""):
+        if len(lib_dir) > 0:
+            cands = glob.glob(os.path.join(lib_dir, "*petl*.so"))
+            self.libprojectors = load_library(cands[0] if cands else None)
+        else:
+            self.libprojectors = load_library()
+        if param_id is not None:
+            self.param_id = param_id
+        else:
+            self.param_id = self.create_new_model()
+        self.set_model()
+        self.print_cost = False
+        self.print_warnings = True
+        self.response = None
+        self.include_response_with_projectors = False
+
+    # ------------------------------------------------------------------ model / geometry
+    def about(self):
+        self.set_model()
+        self.libprojectors.about()
+
+    def set_response(self, R):
+        self.response = R
+
+    def clear_response(self):
+        self.response = None
+
+    def create_new_model(self):
+        return self.libprojectors.create_new_model()
+
+    def set_model(self, i=None):
+        return self.libprojectors.set_model(self.param_id if i is None else i)
+
+    def volume_defined(self):
+        self.set_model()
+        return self.libprojectors.volume_defined()
+
+    def geometry_defined(self):
+        self.set_model()
+        return self.libprojectors.geometry_defined()
+
+    def all_defined(self):
+        return bool(self.volume_defined() and self.geometry_defined())
+
+    def print_parameters(self):
+        self.set_model()
+        self.libprojectors.print_parameters()
+
+    def _get_set(self, name, value):
+        self.set_model()
+        if value is None:
+            return getattr(self.libprojectors, "get_" + name)()
+        if getattr(self.libprojectors, "set_" + name)(value):
+            return getattr(self.libprojectors, "get_" + name)()
+        return None
+
+    def numX(self, N=None):
+        return self._get_set("numX", N)
+
+    def numY(self, N=None):
+        return self._get_set("numY", N)
+
+    def numZ(self, N=None):
+        return self._get_set("numZ", N)
+
+    def voxelWidth(self, w=None):
+        return self._get_set("voxelWidth", w)
+
+    def voxelHeight(self, h=None):
+        return self._get_set("voxelHeight", h)
+
+    def offsetX(self, x_0=None):
+        return self._get_set("offsetX", x_0)
+
+    def offsetY(self, y_0=None):
+        return self._get_set("offsetY", y_0)
+
+    def offsetZ(self, z_0=None):
+        return self._get_set("offsetZ", z_0)
+
+    def get_numPlanograms(self):
+        self.set_model()
+        return self.libprojectors.get_numPlanograms()
+
+    def get_planogramSize(self, n):
+        if self.geometry_defined():
+            shape = np.zeros(4, dtype=np.int32)
+            self.set_model()
+            self.libprojectors.get_planogramSize(n, shape)
+            return shape
+        return None
+
+    def allocate_projections(self):
+        if self.geometry_defined():
+            return [np.zeros(tuple(int(s) for s in self.get_planogramSize(n)), dtype=np.float32)
+                    for n in range(self.get_numPlanograms())]
+        return None
+
+    def copy_data(self, x):
+        if isinstance(x, list):
+            return [a.copy() for a in x]
+        return x.copy()
+
+    def copyData(self, x):
+        return self.copy_data(x)
+
+    def allocate_volume(self):
+        if self.volume_defined():
+            return np.zeros((self.numZ(), self.numY(), self.numX()), dtype=np.float32)
+        print('Error: volume is not defined')
+        return None
+
+    def clearAll(self):
+        self.set_model()
+        self.libprojectors.clearAll()
+
+    def add_planogram(self, psi, D, L, H, T, v_m0=None, v_m1=None):
+        """Adds a pair of parallel flat panels (src/petl_server.py:484-507 defaults)."""
+        if v_m0 is None:
+            v_m0 = 1.0
+        if v_m1 is None:
+            v_m1 = min(1.0, (H - 1.5 * T) / D)
+        self.set_model()
+        if not self.libprojectors.add_planogram(psi, D, L, H, v_m0, v_m1, T):
+            raise ValueError('Error: add_planogram failed')
+
+    def remove_planogram(self, n):
+        self.set_model()
+        if not self.libprojectors.remove_planogram(n):
+            raise ValueError('Error: remove_planogram failed')
+
+    def keep_only_planogram(self, n):
+        self.set_model()
+        if not self.libprojectors.keep_only_planogram(n):
+            raise ValueError('Error: keep_only_planogram failed')
+
+    def set_default_volume(self, scale=1.0):
+        self.set_model()
+        if not self.libprojectors.set_default_volume(scale):
+            raise ValueError('Error: set_default_volume failed')
+
+    def set_volume(self, numX, numY, numZ, voxelWidth, voxelHeight=0.0, offsetX=0.0, offsetY=0.0, offsetZ=0.0):
+        if voxelHeight == 0.0:
+            voxelHeight = voxelWidth
+        self.set_model()
+        if not self.libprojectors.set_volume(numX, numY, numZ, voxelWidth, voxelHeight, offsetX, offsetY, offsetZ):
+            raise ValueError('Error: set_volume failed')
+
+    # ------------------------------------------------------------------ projectors
+    def floatstarstar(self, g):
+        ptrs = [x.ctypes.data_as(_c_float_p) for x in g]
+        return (_c_float_p * len(ptrs))(*ptrs)
+
+    @staticmethod
+    def _check_arrays(g, f):
+        for x in list(g) + [f]:
+            if x.dtype != np.float32 or not x.flags["C_CONTIGUOUS"]:
+                raise TypeError("arrays must be C-contiguous float32")
+
+    def project(self, g, f):
+        if not self.all_defined():
+            raise ValueError('Error: volume and PET geometry must be defined')
+        self._check_arrays(g, f)
+        self.set_model()
+        if not self.libprojectors.project(self.floatstarstar(g), f):
+            raise TypeError('project failed')
+        if self.response is not None and self.include_response_with_projectors:
+            self.multiply(g, self.response)
+        return g
+
+    def backproject(self, g, f=None):
+        if not self.all_defined():
+            raise ValueError('Error: volume and PET geometry must be defined')
+        if f is None:
+            f = self.allocate_volume()
+        if self.response is not None and self.include_response_with_projectors:
+            Rg = self.copyData(g)
+            self.multiply(Rg, self.response)
+        else:
+            Rg = g
+        self._check_arrays(Rg, f)
+        self.set_model()
+        if not self.libprojectors.backproject(self.floatstarstar(Rg), f):
+            raise TypeError('backproject failed')
+        return f
+
+    def set_backprojector_mode(self, mode):
+        """'petl' (reference semantics, default) or 'matched' (exact transpose of project)."""
+        m = {"petl": 0, "matched": 1}.get(mode, mode)
+        self.set_model()
+        if not self.libprojectors.set_backprojector_mode(int(m)):
+            raise ValueError('Error: unknown back projector mode')
+
+    def get_backprojector_mode(self):
+        self.set_model()
+        return ["petl", "matched"][self.libprojectors.get_backprojector_mode()]
+
+    def set_projector_path(self, path):
+        """'auto' (fast separable kernels where the geometry allows) or 'generic'."""
+        p = {"auto": 0, "generic": 1}.get(path, path)
+        self.set_model()
+        if not self.libprojectors.set_projector_path(int(p)):
+            raise ValueError('Error: unknown projector path')
+
+    def uses_fast_path(self, n):
+        self.set_model()
+        return bool(self.libprojectors.planogram_uses_fast_path(n))
+
+    def _dev_ptrs(self, g):
+        ptrs = [ctypes.cast(ctypes.c_void_p(x.data_ptr()), _c_float_p) for x in g]
+        return (_c_float_p * len(ptrs))(*ptrs)
+
+    def project_gpu(self, g, f, stream=None, sync=True):
+        """Forward projection on device tensors (torch, float32, contiguous) without host copies."""
+        self.set_model()
+        s = None if stream is None else ctypes.c_void_p(stream)
+        if not self.libprojectors.project_gpu(self._dev_ptrs(g), ctypes.cast(ctypes.c_void_p(f.data_ptr()),
+                                                                                _c_float_p), s, sync):
+            raise TypeError('project failed')
+        return g
+
+    def backproject_gpu(self, g, f, stream=None, sync=True):
+        self.set_model()
+        s = None if stream is None else ctypes.c_void_p(stream)
+        if not self.libprojectors.backproject_gpu(self._dev_ptrs(g), ctypes.cast(ctypes.c_void_p(f.data_ptr()),
+                                                                                    _c_float_p), s, sync):
+            raise TypeError('backproject failed')
+        return f
+
+    def sensitivity(self, Pstar1=None):
+        """P*1, the back projection of all-ones planograms (src/petl_server.py:1257-1273)."""
+        if Pstar1 is None:
+            Pstar1 = self.allocate_volume()
+        ones = self.allocate_projections()
+        for n in range(len(ones)):
+            ones[n][:] = 1.0
+        self.backproject(ones, Pstar1)
+        return Pstar1
+
+    # ------------------------------------------------------------------ elementwise helpers
+    def clip(self, x, min_val=0.0, max_val=None):
+        if isinstance(x, list):
+            for a in x:
+                np.clip(a, min_val, max_val, out=a)
+        else:
+            np.clip(x, min_val, max_val, out=x)
+        return x
+
+    def scale(self, x, c=1.0):
+        if isinstance(x, list):
+            for a in x:
+                a[:] *= c
+        else:
+            x[:] *= c
+        return x
+
+    def square(self, x):
+        return self.multiply(x, x)
+
+    def subtract(self, x, y):
+        if isinstance(x, list):
+            for a, b in zip(x, y):
+                a[:] -= b[:]
+        else:
+            x[:] -= y[:]
+        return x
+
+    def scalarAdd(self, x, c, y):
+        if isinstance(x, list):
+            for a, b in zip(x, y):
+                a[:] += c * b[:]
+        else:
+            x[:] += c * y[:]
+        return x
+
+    def _binary(self, op, x, y):
+        self.set_model()
+        xs = x if isinstance(x, list) else [x]
+        ys = y if isinstance(y, list) else [y]
+        for a, b in zip(xs, ys):
+            if a.ndim == 4:
+                ok = getattr(self.libprojectors, op + "4D")(a, b, *a.shape)
+            elif a.ndim == 3:
+                ok = getattr(self.libprojectors, op + "3D")(a, b, *a.shape)
+            else:
+                raise ValueError('inputs must be 3D or 4D numpy arrays')
+            if not ok:
+                print(op + ' failed!')
+                return None
+        return x
+
+    def multiply(self, x, y):
+        """x = x * y"""
+        return self._binary("multiply", x, y)
+
+    def divide(self, num, denom):
+        """num = num / denom (0 where denom == 0)"""
+        return self._binary("divide", num, denom)
+
+    def rdivide(self, denom, num):
+        """denom = num / denom (0 where denom == 0)"""
+        self._binary("rdivide", num, denom)
+        return denom
+
+    def reciprocal(self, x, divide_by_zero_value=0.0):
+        self.set_model()
+        for a in (x if isinstance(x, list) else [x]):
+            fn = self.libprojectors.reciprocal4D if a.ndim == 4 else self.libprojectors.reciprocal3D
+            if not fn(a, *a.shape, divide_by_zero_value):
+                print('reciprocal failed!')
+                return None
+        return x
+
+    def as_float_ptr(self, arr):
+        if arr is None:
+            return None
+        return np.asarray(arr, dtype=np.float32, order="C").ctypes.data_as(_c_float_p)
+
+    def inner_product_helper(self, x, y, w=None):
+        self.set_model()
+        if x.ndim == 4:
+            return self.libprojectors.inner_product4D(self.as_float_ptr(x), self.as_float_ptr(y),
+                                                      self.as_float_ptr(w), *x.shape)
+        if x.ndim == 3:
+            return self.libprojectors.inner_product3D(self.as_float_ptr(x), self.as_float_ptr(y),
+                                                      self.as_float_ptr(w), *x.shape)
+        return np.sum(x * y * w)
+
+    def inner_product(self, x, y, w=None):
+        if isinstance(x, list):
+            acc = 0.0
+            for n in range(len(x)):
+                if w is None:
+                    acc += self.inner_product_helper(x[n], y[n])
+                elif isinstance(w, (int, float)):
+                    acc += self.inner_product_helper(x[n], y[n]) * w
+                else:
+                    acc += self.inner_product_helper(x[n], y[n], w[n])
+            return acc
+        return self.inner_product_helper(x, y, w)
+
+    def innerProd(self, x, y, w=None):
+        return self.inner_product(x, y, w)
+
+    def isAllZeros(self, f):
+        return not np.any(f)
+
+    def setToConstant(self, x, c=0.0):
+        if isinstance(x, list):
+            for a in x:
+                a[:] = c
+        else:
+            x[:] = c
+
+    def setToZero(self, x):
+        self.setToConstant(x, 0.0)
+
+    def setToOne(self, x):
+        self.setToConstant(x, 1.0)
+
+    # ------------------------------------------------------------------ reconstruction
+    def MLEM(self, g, f, numIter, delta=2.0, beta=0.0, mask=None, prior="rd", p=1.0, num_subsets=1,
+             return_loglik=False):
+        r"""Maximum Likelihood-Expectation Maximization, f <- f / (P^T 1 + R'(f)) * P^T[g / Pf].
+
+        Same arguments and result as the reference's PETLserver.MLEM (src/petl_server.py:1534-1606):
+        f None -> ones, all-zero f -> ones, negatives clipped; beta > 0 adds the relative-difference
+        prior gradient (delta) one-step-late; the response, when set, weights the projections.  The
+        whole loop runs on the GPU.  Extras: prior='tv' uses the p-Huber TV gradient instead,
+        num_subsets > 1 runs OSEM, return_loglik=True also returns sum(y log Pf - Pf) per iteration.
+        """
+        if mask is not None and (len(mask) != len(g) or any(m.shape != a.shape for m, a in zip(mask, g))):
+            print('Error: mask must be the same shape as the projection data!')
+            return None
+        if f is None:
+            f = self.allocate_volume()
+            self.setToOne(f)
+        self._check_arrays(g, f)
+        if self.print_warnings:
+            print('MLEM: ' + str(numIter) + ' iterations on the GPU')
+        kind = 0 if beta <= 0.0 else (2 if prior == "tv" else 1)
+        ll = np.zeros(max(numIter, 1), dtype=np.float64)
+        resp = self.floatstarstar(self.response) if self.response is not None else None
+        msk = self.floatstarstar(mask) if mask is not None else None
+        self.set_model()
+        ok = self.libprojectors.MLEM_gpu(self.floatstarstar(g), f, int(numIter), float(delta), float(beta), kind,
+                                         float(p), int(num_subsets), msk, resp,
+                                         ll.ctypes.data_as(ctypes.c_void_p) if return_loglik else None)
+        if not ok:
+            raise TypeError('MLEM failed')
+        if return_loglik:
+            return f, ll[:numIter]
+        return f
+
+    def OSEM(self, g, f, numIter, numSubsets, delta=2.0, beta=0.0, mask=None):
+        return self.MLEM(g, f, numIter, delta=delta, beta=beta, mask=mask, num_subsets=numSubsets)
+
+    # ------------------------------------------------------------------ priors
+    @staticmethod
+    def _dims3(f):
+        if f.ndim == 3:
+            return f.shape
+        if f.ndim == 2:
+            return (1,) + f.shape
+        raise ValueError('input must be 2D or 3D')
+
+    def TVgradient(self, f, delta, beta=0.0, p=1.0):
+        """Gradient of the anisotropic p-Huber TV functional (src/petl_server.py:1924-1974)."""
+        N_1, N_2, N_3 = self._dims3(f)
+        self.set_model()
+        if hasattr(f, "data_ptr"):
+            Df = f.clone()
+            self.libprojectors.TV_gradient(f.data_ptr(), Df.data_ptr(), N_1, N_2, N_3, delta, beta, p, not f.is_cuda)
+            return Df
+        Df = np.zeros(f.shape, dtype=np.float32)
+        fc = np.ascontiguousarray(f, dtype=np.float32)
+        self.libprojectors.TV_gradient(fc.ctypes.data, Df.ctypes.data, N_1, N_2, N_3, delta, beta, p, True)
+        return Df
+
+    def relative_differences_grad(self, f, delta, beta):
+        """Gradient of the relative-difference penalty (src/petl_server.py:2112-2127)."""
+        N_1, N_2, N_3 = self._dims3(f)
+        Df = np.zeros(f.shape, dtype=np.float32)
+        self.set_model()
+        self.libprojectors.relativeDifferences_gradient(np.ascontiguousarray(f, dtype=np.float32), Df,
+                                                        N_1, N_2, N_3, delta, beta)
+        return Df
