@@ -132,8 +132,8 @@ PETL_EXTERN int get_projector_path();
 PETL_EXTERN int planogram_uses_fast_path(int which);
 
 /* Device-resident forward / back projection on caller-owned device buffers (e.g. torch tensors).
- * d_g: host array of get_numPlanograms() device pointers; d_f: device pointer; stream: hipStream_t or
- * NULL for the library's stream.  Work is enqueued on the stream; the call returns after enqueueing
+ * d_g: host array of get_numPlanograms() device pointers; d_f: device pointer; stream: the hipStream_t
+ * to enqueue on (NULL = the default stream).  Work is enqueued on the stream; the call returns after enqueueing
  * unless sync is true.  backproject_gpu overwrites d_f. */
 PETL_EXTERN bool project_gpu(float** d_g, float* d_f, void* stream, bool sync);
 PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync);

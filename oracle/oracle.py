@@ -78,6 +78,8 @@ def lib():
         L.orc_tv_cost_f64.argtypes = [dp, c_int, c_int, c_int, c_float, c_float, c_float]
         L.orc_tv_cost_f64.restype = c_double
         L.orc_max_threads.restype = c_int
+        L.orc_project_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
+        L.orc_backproject_petl_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         _lib = L
     return _lib
 
@@ -135,6 +137,16 @@ def backproject(plans, vol, g, mode="petl", dtype=np.float32, nthreads=None):
     for p, gi in zip(plans, g):
         fn(ctypes.byref(p), ctypes.byref(vol), np.ascontiguousarray(gi, dtype=dtype), f, m, _nt(nthreads))
     return f
+
+
+def project_rows(plan, vol, f, g, v1_begin, v1_end, nthreads=None):
+    """fp32 forward of v1 rows [v1_begin, v1_end) into g (full planogram array)."""
+    lib().orc_project_rows_f32(ctypes.byref(plan), ctypes.byref(vol), f, g, v1_begin, v1_end, _nt(nthreads))
+
+
+def backproject_petl_rows(plan, vol, g, f, v1_begin, v1_end, nthreads=None):
+    """fp32 PETL-mode back projection of v1 rows [v1_begin, v1_end), accumulated into f."""
+    lib().orc_backproject_petl_rows_f32(ctypes.byref(plan), ctypes.byref(vol), g, f, v1_begin, v1_end, _nt(nthreads))
 
 
 def bin_taps(plan, vol, iv1, iv0, iu1, iu0, cap=1 << 16):

@@ -256,12 +256,16 @@ R enumerate_bin(const orc_plan& P, const orc_vol& V, int iv1, int iv0, int iu1, 
 }
 
 template <typename R>
-void project_plan(const orc_plan& P, const orc_vol& V, const R* f, R* g, int nthreads)
+void project_plan(const orc_plan& P, const orc_vol& V, const R* f, R* g, int nthreads, int v1_begin = 0,
+                  int v1_end = -1)
 {
-    const int NV = P.N_v1 * P.N_v0;
+    if (v1_end < 0)
+        v1_end = P.N_v1;
+    const int NV = (v1_end - v1_begin) * P.N_v0;
     const int64_t row = int64_t(P.N_u1) * P.N_u0;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-    for (int v = 0; v < NV; v++) {
+    for (int vv = 0; vv < NV; vv++) {
+        const int v = v1_begin * P.N_v0 + vv;
         const int iv1 = v / P.N_v0, iv0 = v % P.N_v0;
         R* gv = g + int64_t(v) * row;
         for (int iu1 = 0; iu1 < P.N_u1; iu1++)
@@ -318,8 +322,11 @@ void backproject_matched_plan(const orc_plan& P, const orc_vol& V, const R* g, R
 
 // PETL-mode back projector: src/projectors.cu:240-377 with exact bilinear weights.  Accumulates into f.
 template <typename R>
-void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f, int nthreads)
+void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f, int nthreads, int v1_begin = 0,
+                           int v1_end = -1)
 {
+    if (v1_end < 0)
+        v1_end = P.N_v1;
     const int Nx = V.numX, Ny = V.numY, Nz = V.numZ;
     const R cs = R(P.cos_psi), sn = R(P.sin_psi), half = R(0.5);
     const R Tx = R(V.voxelWidth), Ty = R(V.voxelWidth), Tz = R(V.voxelHeight);
@@ -337,7 +344,7 @@ void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f
                 const float xf = coord_f(ix, V.voxelWidth, V.x0), yf = coord_f(iy, V.voxelWidth, V.y0);
                 const float zf = coord_f(iz, V.voxelHeight, V.z0);
                 R fsum = R(0);
-                for (int iv1 = 0; iv1 < P.N_v1; iv1++) {
+                for (int iv1 = v1_begin; iv1 < v1_end; iv1++) {
                     R val = R(0);
                     // u1 test at the voxel centre (src/projectors.cu:284-289), evaluated in float
                     const float v1f = coord_f(iv1, P.T_v1, P.v1_0);
@@ -673,5 +680,18 @@ double orc_tv_cost_f64(const double* f, int N1, int N2, int N3, float delta, flo
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+// Forward / PETL-mode back projection restricted to v1 rows [v1_begin, v1_end) (bounded timing samples).
+void orc_project_rows_f32(const orc_plan* P, const orc_vol* V, const float* f, float* g, int v1_begin, int v1_end,
+                          int nthreads)
+{
+    project_plan<float>(*P, *V, f, g, nthreads, v1_begin, v1_end);
+}
+
+void orc_backproject_petl_rows_f32(const orc_plan* P, const orc_vol* V, const float* g, float* f, int v1_begin,
+                                   int v1_end, int nthreads)
+{
+    backproject_petl_plan<float>(*P, *V, g, f, nthreads, v1_begin, v1_end);
+}
 
 }  // extern "C"

@@ -725,7 +725,7 @@ bool project_gpu(float** d_g, float* d_f, void* stream, bool sync)
     if (!d_g || !d_f || !m->allDefined())
         return false;
     return guarded("project_gpu", [&] {
-        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream();
+        hipStream_t s = static_cast<hipStream_t>(stream);  // NULL = the default stream
         m->forward(d_g, d_f, EPI_NONE, nullptr, nullptr, s);
         if (sync)
             PETL_HIP_CHECK(hipStreamSynchronize(s));
@@ -739,7 +739,7 @@ bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync)
     if (!d_g || !d_f || !m->allDefined())
         return false;
     return guarded("backproject_gpu", [&] {
-        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream();
+        hipStream_t s = static_cast<hipStream_t>(stream);  // NULL = the default stream
         m->back(d_g, d_f, m->back_mode, s);
         if (sync)
             PETL_HIP_CHECK(hipStreamSynchronize(s));

@@ -18,7 +18,7 @@
 namespace petl {
 
 struct FastTables {
-    int4* xtab = nullptr;     // [Nv0][Ns]  {k, wq, wp, 0}: u0 interpolation of slice s in view v0
+    int2* xtab = nullptr;     // [Nv0][Ns + pad]  {k, wq}: u0 interpolation of slice s in view v0
     int4* ztab = nullptr;     // [Nv1][Ns]  {m1, wa, wb, 0}: u1 interpolation of slice s in view v1
     float* fscale = nullptr;  // [Nv1][Nv0] forward / matched scale  l_phi / sqrt(1+v0^2+v1^2)
     float* pscale = nullptr;  // [Nv1][Nv0] PETL back scale  T_x / max(|wy|,|wx|) / sqrt(1+v0^2+v1^2)
@@ -32,7 +32,7 @@ struct FastPlan {
     int Ns = 0, Na = 0, Nz = 0;
     int64_t fs_s = 0, fs_a = 0, fs_z = 0;  // element strides of the slice / row / z axes in f[z][y][x]
     // forward kernel shape
-    int J = 0, C = 0, ngroups = 0, SB = 0, hpad = 0, hrow = 0;
+    int J = 0, C = 0, ngroups = 0, SB = 0, hpad = 0, hrow = 0, xrow = 0;
     // back kernel shape
     int AG = 0, nagroups = 0, RB = 0, spad_lo = 0, spad_hi = 0, srow = 0, ZC = 0;
     FastTables dev;

@@ -27,6 +27,7 @@
 #include <vector>
 #include <algorithm>
 #include <cstring>
+#include <cstdlib>
 #include "petl_common.h"
 #include "proj_kernels.h"
 #include "proj_fast.h"
@@ -37,10 +38,11 @@ namespace {
 
 constexpr int FWD_THREADS = 512;
 constexpr int FWD_WAVES = FWD_THREADS / 64;
-constexpr int FWD_SB = 8;     // slices per LDS block in the forward
+constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
 constexpr int BACK_RB = 8;    // v0 rows per LDS block in the back projector
 constexpr int BACK_AG = 128;  // a-values per lane in the back projector
 constexpr int BACK_TW = 32;   // transpose tile width of the back epilogue
+
 
 __device__ __forceinline__ float coordf(int i, float T, float s)
 {
@@ -65,14 +67,66 @@ struct FwdArgs {
     int hpad, hrow;
     int psi90;
     int v1_first, v1_stride;
+    int xrow;  // padded row length of the u0 table
 };
 
 // =============================================================================================
 // forward
 // =============================================================================================
-template <int J, int C>
+// z interpolation of one block of slices, software-pipelined: the global loads of block b+1 are issued
+// before the block-b accumulation and written to LDS after it.
+template <int MAXE>
+struct ZLoads {
+    float a[MAXE], b[MAXE], wa[MAXE], wb[MAXE];
+    int dst[MAXE];
+};
+
+template <int SB, int MAXE>
+__device__ __forceinline__ void zload(ZLoads<MAXE>& L, const FwdArgs& A, const float* __restrict__ f,
+                                      const int4* __restrict__ ztab, int iv1, int iu1, int is0, int is_hi, int tid)
+{
+    // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
+    // the loads stay in flight until zstore consumes them
+    const int n = SB * A.Na;
+#pragma unroll
+    for (int e = 0; e < MAXE; e++) {
+        const int idx0 = tid + e * FWD_THREADS;
+        const int idx = min(idx0, n - 1);
+        int sl, ia;
+        if (A.psi90) {
+            sl = idx % SB;
+            ia = idx / SB;
+        } else {
+            sl = idx / A.Na;
+            ia = idx - sl * A.Na;
+        }
+        L.dst[e] = (idx0 < n) ? sl * A.hrow + ia : -1;
+        const int is = is0 + sl;
+        const bool live = is <= is_hi;
+        const int isc = min(is, is_hi);
+        const int4 z = ztab[int64_t(iv1) * A.Ns + isc];
+        const int iza = iu1 - z.x - 1;
+        const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
+        const float* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
+        L.a[e] = fp[int64_t(za) * A.fs_z];
+        L.b[e] = fp[int64_t(zb) * A.fs_z];
+        L.wa[e] = (live && iza >= 0 && iza < A.Nz) ? __int_as_float(z.y) : 0.0f;
+        L.wb[e] = (live && iza + 1 >= 0 && iza + 1 < A.Nz) ? __int_as_float(z.z) : 0.0f;
+    }
+}
+
+template <int MAXE>
+__device__ __forceinline__ void zstore(const ZLoads<MAXE>& L, float* hb)
+{
+#pragma unroll
+    for (int e = 0; e < MAXE; e++)
+        if (L.dst[e] >= 0)
+            hb[L.dst[e]] = L.wa[e] * L.a[e] + L.wb[e] * L.b[e];
+}
+
+template <int J, int C, int SB>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict__ g, const float* __restrict__ f,
-                                                               FwdArgs A, const int4* __restrict__ xtab,
+                                                               FwdArgs A, const int2* __restrict__ xtab,
                                                                const int4* __restrict__ ztab,
                                                                const float* __restrict__ fscale, int epilogue,
                                                                const float* __restrict__ yv,
@@ -110,7 +164,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
         s_lo = Ns;
         s_hi = -1;
     }
-    const int hbuf_elems = 2 * FWD_SB * A.hrow;
+    const int hbuf_elems = 2 * SB * A.hrow;
     for (int i = tid; i < hbuf_elems; i += FWD_THREADS)
         hbuf[i] = 0.0f;
     __syncthreads();
@@ -127,90 +181,74 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
     __syncthreads();
     const int is_lo = __builtin_amdgcn_readfirstlane(s_lo), is_hi = __builtin_amdgcn_readfirstlane(s_hi);
 
-    float Q[J][C], Pa[J][C];
+    // Q[p] = sum wq*v, T[p] = sum v over the values v = H[s][p+k] read by lane p;  out[p] = Q[p] + T[p+1] - Q[p+1]
+    float Q[J][C], T[J][C];
 #pragma unroll
     for (int j = 0; j < J; j++)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             Q[j][c] = 0.0f;
-            Pa[j][c] = 0.0f;
+            T[j][c] = 0.0f;
         }
 
     if (is_lo <= is_hi) {
-        const int nblk = (is_hi - is_lo) / FWD_SB + 1;
-        // z interpolation of slices [is0, is0+SB) into LDS buffer b
-        auto build = [&](int blk, int b) {
-            const int is0 = is_lo + blk * FWD_SB;
-            float* hb = hbuf + b * FWD_SB * A.hrow + A.hpad;
-            const int n = FWD_SB * Na;
-            for (int idx = tid; idx < n; idx += FWD_THREADS) {
-                int sl, ia;
-                if (A.psi90) {
-                    sl = idx % FWD_SB;
-                    ia = idx / FWD_SB;
-                } else {
-                    sl = idx / Na;
-                    ia = idx - sl * Na;
-                }
-                const int is = is0 + sl;
-                float val = 0.0f;
-                if (is <= is_hi) {
-                    const int4 e = ztab[int64_t(iv1) * Ns + is];
-                    const int iza = iu1 - e.x - 1;
-                    const float* fp = f + int64_t(is) * A.fs_s + int64_t(ia) * A.fs_a;
-                    if (iza >= 0 && iza < Nz)
-                        val = __int_as_float(e.y) * fp[int64_t(iza) * A.fs_z];
-                    if (iza + 1 >= 0 && iza + 1 < Nz)
-                        val += __int_as_float(e.z) * fp[int64_t(iza + 1) * A.fs_z];
-                }
-                hb[sl * A.hrow + ia] = val;
-            }
-        };
-        build(0, 0);
+        const int nblk = (is_hi - is_lo) / SB + 1;
+        constexpr int MAXE = (SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128 per slice row
+        ZLoads<MAXE> L;
+        zload<SB>(L, A, f, ztab, iv1, iu1, is_lo, is_hi, tid);
+        zstore(L, hbuf + A.hpad);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
-            if (blk + 1 < nblk)
-                build(blk + 1, (blk + 1) & 1);
-            const int is0 = is_lo + blk * FWD_SB;
-            const float* hb = hbuf + (blk & 1) * FWD_SB * A.hrow + A.hpad + lane;
+            const bool more = blk + 1 < nblk;
+            if (more)
+                zload<SB>(L, A, f, ztab, iv1, iu1, is_lo + (blk + 1) * SB, is_hi, tid);
+            const int is0 = is_lo + blk * SB;
+            const float* hb = hbuf + (blk & 1) * SB * A.hrow + A.hpad + lane;
 #pragma unroll
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
-                    int k[FWD_SB];
-                    float wq[FWD_SB], wp[FWD_SB];
-                    const int4* xt = xtab + int64_t(iv0) * Ns;
+                    // table rows are padded by SB entries, so the block's entries are contiguous
+                    const int2* xt = xtab + int64_t(iv0) * A.xrow + is0;
+                    int k[SB];
+                    float wq[SB];
 #pragma unroll
-                    for (int s = 0; s < FWD_SB; s++) {
-                        const int4 e = xt[min(is0 + s, is_hi)];
+                    for (int s = 0; s < SB; s++) {
+                        const int2 e = xt[s];
                         k[s] = e.x;
                         wq[s] = __int_as_float(e.y);
-                        wp[s] = __int_as_float(e.z);
                     }
-                    int kmin = k[0], kmax = k[0];
+                    // k is monotone in the slice index, so the block's extremes are its end values
+                    const int kmin = min(k[0], k[SB - 1]), kmax = max(k[0], k[SB - 1]);
+                    const int c_lo = max(0, (-kmax) / 64);  // first chunk with 64c + 63 >= -kmax
+                    const int c_hi = min(C - 1, (Na - 1 - kmin) >= 0 ? (Na - 1 - kmin) / 64 : -1);
+                    const float* hp[SB];
 #pragma unroll
-                    for (int s = 1; s < FWD_SB; s++) {
-                        kmin = min(kmin, k[s]);
-                        kmax = max(kmax, k[s]);
-                    }
+                    for (int s = 0; s < SB; s++)
+                        hp[s] = hb + s * A.hrow + k[s];
 #pragma unroll
                     for (int c = 0; c < C; c++) {
-                        if (64 * c <= Na - 1 - kmin && 64 * c + 63 >= -kmax) {
+                        if (c >= c_lo && c <= c_hi) {
+                            float v[SB];
 #pragma unroll
-                            for (int s = 0; s < FWD_SB; s++) {
-                                const float v = hb[s * A.hrow + 64 * c + k[s]];
-                                Q[j][c] += wq[s] * v;
-                                Pa[j][c] += wp[s] * v;
+                            for (int s = 0; s < SB; s++)
+                                v[s] = hp[s][64 * c];
+#pragma unroll
+                            for (int s = 0; s < SB; s++) {
+                                T[j][c] += v[s];
+                                Q[j][c] += wq[s] * v[s];
                             }
                         }
                     }
                 }
             }
+            if (more)
+                zstore(L, hbuf + ((blk + 1) & 1) * SB * A.hrow + A.hpad);
             __syncthreads();
         }
     }
 
-    // epilogue: out[p] = (Q[p] + P[p+1]) * scale, masks, optional ratio
+    // epilogue: out[p] = (Q[p] + T[p+1] - Q[p+1]) * scale, masks, optional ratio
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int iv0 = iv0_base + FWD_WAVES * j;
@@ -220,13 +258,18 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
             float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0;
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                float pn = __shfl_down(Pa[j][c], 1);
-                const float nxt = (c + 1 < C) ? __shfl(Pa[j][(c + 1 < C) ? c + 1 : c], 0) : 0.0f;
-                if (lane == 63)
-                    pn = nxt;
+                float tn = __shfl_down(T[j][c], 1);
+                float qn = __shfl_down(Q[j][c], 1);
+                const int cn = (c + 1 < C) ? c + 1 : c;
+                const float t0 = (c + 1 < C) ? __shfl(T[j][cn], 0) : 0.0f;
+                const float q0 = (c + 1 < C) ? __shfl(Q[j][cn], 0) : 0.0f;
+                if (lane == 63) {
+                    tn = t0;
+                    qn = q0;
+                }
                 const int p = 64 * c + lane;
                 if (p < Nu0) {
-                    float out = (Q[j][c] + pn) * sc;
+                    float out = (Q[j][c] + (tn - qn)) * sc;
                     if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
                         out = 0.0f;
                     const int64_t gi = int64_t(gr - g) + p;
@@ -260,7 +303,7 @@ struct BackArgs {
 
 template <int AG>
 __global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ g, float* __restrict__ h2, BackArgs A,
-                                                      const int4* __restrict__ xtab,
+                                                      const int2* __restrict__ xtab, int xrow,
                                                       const float* __restrict__ scale,
                                                       const int2* __restrict__ rows)
 {
@@ -309,13 +352,17 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ 
         if (blk + 1 < nblk)
             load(blk + 1, (blk + 1) & 1);
         const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
+        // the next view's table entry is loaded while the current view's window runs
+        int2 e_next = xtab[int64_t(min(blk * BACK_RB, Nv0 - 1)) * xrow + is];
 #pragma unroll 1
         for (int r = 0; r < BACK_RB; r++) {
             const int iv0 = blk * BACK_RB + r;
+            const int2 e = e_next;
+            if (r + 1 < BACK_RB)
+                e_next = xtab[int64_t(min(iv0 + 1, Nv0 - 1)) * xrow + is];
             if (iv0 < Nv0) {
-                const int4 e = xtab[int64_t(iv0) * Ns + is];
                 const int m = -e.x - 1;
-                const float wq = __int_as_float(e.y), wp = __int_as_float(e.z);
+                const float wq = __int_as_float(e.y), wp = 1.0f - wq;
                 const float* s = sb + r * A.srow + m;
                 float xp = s[0];
 #pragma unroll
@@ -423,31 +470,31 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
 // ---------------------------------------------------------------------------------------------
 // host-side kernel selection
 // ---------------------------------------------------------------------------------------------
-typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int4*, const int4*, const float*, int, const float*,
+typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int2*, const int4*, const float*, int, const float*,
                        const float*);
 
-template <int J>
+template <int J, int SB>
 fwd_fn pick_fwd_c(int C)
 {
     switch (C) {
-    case 1: return fwd_fast_kernel<J, 1>;
-    case 2: return fwd_fast_kernel<J, 2>;
-    case 3: return fwd_fast_kernel<J, 3>;
-    case 4: return fwd_fast_kernel<J, 4>;
-    case 5: return fwd_fast_kernel<J, 5>;
-    case 6: return fwd_fast_kernel<J, 6>;
-    case 7: return fwd_fast_kernel<J, 7>;
-    case 8: return fwd_fast_kernel<J, 8>;
+    case 1: return fwd_fast_kernel<J, 1, SB>;
+    case 2: return fwd_fast_kernel<J, 2, SB>;
+    case 3: return fwd_fast_kernel<J, 3, SB>;
+    case 4: return fwd_fast_kernel<J, 4, SB>;
+    case 5: return fwd_fast_kernel<J, 5, SB>;
+    case 6: return fwd_fast_kernel<J, 6, SB>;
+    case 7: return fwd_fast_kernel<J, 7, SB>;
+    case 8: return fwd_fast_kernel<J, 8, SB>;
     default: return nullptr;
     }
 }
 
-fwd_fn pick_fwd(int J, int C)
+fwd_fn pick_fwd(int J, int C, int SB)
 {
-    if (J == 13)
-        return pick_fwd_c<13>(C);
-    if (J == 7)
-        return pick_fwd_c<7>(C);
+    if (SB == 16)
+        return J == 7 ? pick_fwd_c<7, 16>(C) : (J == 4 ? pick_fwd_c<4, 16>(C) : nullptr);
+    if (SB == 8)
+        return J == 7 ? pick_fwd_c<7, 8>(C) : (J == 4 ? pick_fwd_c<4, 8>(C) : nullptr);
     return nullptr;
 }
 
@@ -522,20 +569,31 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->C = (P.Nu0 + 1 + 63) / 64;
     if (fp->C > 8 || Ns > 1024)
         return nullptr;
-    fp->J = (2 * 13 * fp->C <= 112) ? 13 : 7;
+    // views per wave: fewer views = fewer accumulators = more waves per SIMD to hide table latency
+    fp->J = (2 * 7 * fp->C <= 112) ? 7 : 4;
+    if (const char* e = std::getenv("PETL_FWD_J")) {
+        const int j = std::atoi(e);
+        if ((j == 4 || j == 7) && 2 * j * fp->C <= 112)
+            fp->J = j;
+    }
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
-    fp->SB = FWD_SB;
+    fp->SB = 8;  // measured: 8-slice blocks beat 16 (more waves per CU)
+    if (const char* e = std::getenv("PETL_FWD_SB"))
+        fp->SB = (std::atoi(e) == 16) ? 16 : 8;
     fp->AG = BACK_AG;
     fp->nagroups = (Na + BACK_AG - 1) / BACK_AG;
     fp->RB = BACK_RB;
     fp->ZC = 32;
-    if (Ns > 256)  // back_h2_kernel has one lane per slice, at most 256
+    if (Ns > 256 || Na > 128)  // one lane per slice in back_h2_kernel; z-interpolated row length
         return nullptr;
 
     // ---- FP64 tables ----
     const double x0 = V.x0, y0 = V.y0, z0 = V.z0;
     const double cs = P.cos_psi;
-    std::vector<int4> xtab(size_t(P.Nv0) * Ns), ztab(size_t(P.Nv1) * Ns);
+    const int xrow = Ns + FWD_SBMAX;  // padded so a block of slice entries never leaves the row
+    fp->xrow = xrow;
+    std::vector<int2> xtab(size_t(P.Nv0) * xrow);
+    std::vector<int4> ztab(size_t(P.Nv1) * Ns);
     std::vector<float> fscale(size_t(P.Nv1) * P.Nv0), pscale(size_t(P.Nv1) * P.Nv0);
     std::vector<int2> rows(2 * size_t(P.Nv1));
     int kspread = 0, mmin = 1 << 30, mmax = -(1 << 30);
@@ -555,13 +613,15 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
             const double m = std::floor(mu);
             const double phi = mu - m;
             const int mi = int(m);
-            xtab[size_t(iv0) * Ns + is] = make_int4(-mi - 1, ibits(float(phi)), ibits(float(1.0 - phi)), 0);
+            xtab[size_t(iv0) * xrow + is] = make_int2(-mi - 1, ibits(float(phi)));
             mmin = std::min(mmin, mi);
             mmax = std::max(mmax, mi);
         }
-        for (int is = 0; is < Ns; is++) {
-            const int lo = std::max(0, is - FWD_SB + 1);
-            const int kd = std::abs(xtab[size_t(iv0) * Ns + is].x - xtab[size_t(iv0) * Ns + lo].x);
+        for (int is = Ns; is < xrow; is++)
+            xtab[size_t(iv0) * xrow + is] = xtab[size_t(iv0) * xrow + Ns - 1];
+        for (int is = 0; is < xrow; is++) {
+            const int lo = std::max(0, is - fp->SB + 1);
+            const int kd = std::abs(xtab[size_t(iv0) * xrow + is].x - xtab[size_t(iv0) * xrow + lo].x);
             kspread = std::max(kspread, kd);
         }
     }
@@ -626,11 +686,11 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->srow = fp->spad_lo + std::max(P.Nu0, mmax + na_pad + 1) + 1;
     if ((fp->srow & 1) == 0)
         fp->srow += 1;
-    if (size_t(2) * FWD_SB * fp->hrow * 4 > 64 * 1024 || size_t(2) * BACK_RB * fp->srow * 4 > 64 * 1024)
+    if (size_t(2) * fp->SB * fp->hrow * 4 > 64 * 1024 || size_t(2) * BACK_RB * fp->srow * 4 > 64 * 1024)
         return nullptr;
 
     // ---- upload ----
-    const size_t nx = xtab.size() * sizeof(int4), nzb = ztab.size() * sizeof(int4);
+    const size_t nx = xtab.size() * sizeof(int2), nzb = ztab.size() * sizeof(int4);
     const size_t nf = fscale.size() * sizeof(float), nr = rows.size() * sizeof(int2);
     auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
     const size_t total = al(nx) + al(nzb) + 2 * al(nf) + al(nr);
@@ -638,7 +698,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     PETL_HIP_CHECK(hipMalloc(&blk, total));
     fp->dev_block = blk;
     size_t off = 0;
-    fp->dev.xtab = reinterpret_cast<int4*>(blk + off);
+    fp->dev.xtab = reinterpret_cast<int2*>(blk + off);
     off += al(nx);
     fp->dev.ztab = reinterpret_cast<int4*>(blk + off);
     off += al(nzb);
@@ -671,11 +731,12 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
     A.psi90 = fp.psi90;
     A.v1_first = v1_first;
     A.v1_stride = v1_stride;
+    A.xrow = fp.xrow;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn fn = pick_fwd(fp.J, fp.C);
+    fwd_fn fn = pick_fwd(fp.J, fp.C, fp.SB);
     if (!fn)
         throw hip_error("fast forward: no kernel instance for this shape");
-    const size_t lds = size_t(2) * FWD_SB * fp.hrow * sizeof(float);
+    const size_t lds = size_t(2) * fp.SB * fp.hrow * sizeof(float);
     hipLaunchKernelGGL(fn, dim3(fp.ngroups, fp.P.Nu1, nv1), dim3(FWD_THREADS), lds, s, g, f, A, fp.dev.xtab,
                        fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
     PETL_LAUNCH_CHECK();
@@ -710,7 +771,7 @@ void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* wo
     const int nthr = ((fp.Ns + 63) / 64) * 64;
     const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
     hipLaunchKernelGGL(back_h2_kernel<BACK_AG>, dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s, g,
-                       workspace, A, fp.dev.xtab, scale, rows);
+                       workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     PETL_LAUNCH_CHECK();
     hipLaunchKernelGGL(back_z_kernel<32>, dim3((fp.Na + 63) / 64, fp.Ns, (fp.Nz + 31) / 32), dim3(64), 0, s,
                        workspace, f, A, fp.dev.ztab, rows);
