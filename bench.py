@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=4, help="v1 rows of each planogram in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--mode", default="petl", choices=["petl", "matched"])
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (reference for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -100,6 +102,12 @@ def main():
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
+    if a.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()

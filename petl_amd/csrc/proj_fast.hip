@@ -39,6 +39,7 @@ namespace {
 constexpr int FWD_THREADS = 512;
 constexpr int FWD_WAVES = FWD_THREADS / 64;
 constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
+constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows
 constexpr int BACK_RB = 8;    // v0 rows per LDS block in the back projector
 constexpr int BACK_AG = 128;  // a-values per lane in the back projector
 constexpr int BACK_TW = 32;   // transpose tile width of the back epilogue
@@ -73,39 +74,43 @@ struct FwdArgs {
 // =============================================================================================
 // forward
 // =============================================================================================
-// z interpolation of one block of slices, software-pipelined: the global loads of block b+1 are issued
-// before the block-b accumulation and written to LDS after it.
+// z interpolation of R u1 rows x SB slices, software-pipelined: the global loads of block b+1 are issued
+// before the block-b accumulation and written to LDS after it.  LDS layout per buffer: [R][SB][FWD_HP].
 template <int MAXE>
 struct ZLoads {
     float a[MAXE], b[MAXE], wa[MAXE], wb[MAXE];
     int dst[MAXE];
 };
 
-template <int SB, int MAXE>
+template <int SB, int R, int MAXE>
 __device__ __forceinline__ void zload(ZLoads<MAXE>& L, const FwdArgs& A, const float* __restrict__ f,
-                                      const int4* __restrict__ ztab, int iv1, int iu1, int is0, int is_hi, int tid)
+                                      const int4* __restrict__ ztab, int iv1, int iu1_0, int nrows, int is0,
+                                      int tid)
 {
     // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
     // the loads stay in flight until zstore consumes them
-    const int n = SB * A.Na;
+    const int per_row = SB * A.Na;
+    const int n = R * per_row;
 #pragma unroll
     for (int e = 0; e < MAXE; e++) {
         const int idx0 = tid + e * FWD_THREADS;
         const int idx = min(idx0, n - 1);
+        const int r = idx / per_row;
+        const int rem = idx - r * per_row;
         int sl, ia;
         if (A.psi90) {
-            sl = idx % SB;
-            ia = idx / SB;
+            sl = rem % SB;
+            ia = rem / SB;
         } else {
-            sl = idx / A.Na;
-            ia = idx - sl * A.Na;
+            sl = rem / A.Na;
+            ia = rem - sl * A.Na;
         }
-        L.dst[e] = (idx0 < n) ? sl * A.hrow + ia : -1;
+        L.dst[e] = (idx0 < n) ? (r * SB + sl) * FWD_HP + ia : -1;
         const int is = is0 + sl;
-        const bool live = is <= is_hi;
-        const int isc = min(is, is_hi);
+        const bool live = is < A.Ns && r < nrows;
+        const int isc = min(is, A.Ns - 1);
         const int4 z = ztab[int64_t(iv1) * A.Ns + isc];
-        const int iza = iu1 - z.x - 1;
+        const int iza = iu1_0 + r - z.x - 1;
         const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
         const float* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
         L.a[e] = fp[int64_t(za) * A.fs_z];
@@ -124,7 +129,8 @@ __device__ __forceinline__ void zstore(const ZLoads<MAXE>& L, float* hb)
             hb[L.dst[e]] = L.wa[e] * L.a[e] + L.wb[e] * L.b[e];
 }
 
-template <int J, int C, int SB>
+// J views per wave, C output chunks of 64 bins, SB-slice LDS blocks, R u1 rows per workgroup.
+template <int J, int C, int SB, int R>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict__ g, const float* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ xtab,
                                                                const int4* __restrict__ ztab,
@@ -137,42 +143,59 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int grp = blockIdx.x, iu1 = blockIdx.y, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const int grp = blockIdx.x, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const int iu1_0 = blockIdx.y * R;
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz;
+    const int nrows = min(R, Nu1 - iu1_0);
     const int iv0_base = grp * (FWD_WAVES * J) + wave;
 
     const float v1 = coordf(iv1, P.Tv1, P.v1_0);
-    const float u1 = coordf(iu1, P.Tu1, P.u1_0);
-    if (u1_masked(P, u1, v1)) {
-        // whole u1 row is masked (src/projectors.cu:38-43); a ratio of a zero projection is zero too
+    bool rmask[R];
+    bool all_masked = true;
 #pragma unroll
-        for (int j = 0; j < J; j++) {
-            const int iv0 = iv0_base + FWD_WAVES * j;
-            if (iv0 < Nv0) {
-                float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0;
-                for (int p = lane; p < Nu0; p += 64)
-                    gr[p] = 0.0f;
+    for (int r = 0; r < R; r++) {
+        rmask[r] = (r >= nrows) || u1_masked(P, coordf(iu1_0 + r, P.Tu1, P.u1_0), v1);
+        all_masked = all_masked && rmask[r];
+    }
+    if (all_masked) {
+        // every u1 row is masked (src/projectors.cu:38-43); a ratio of a zero projection is zero too
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r >= nrows)
+                continue;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                const int iv0 = iv0_base + FWD_WAVES * j;
+                if (iv0 < Nv0) {
+                    float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                    for (int p = lane; p < Nu0; p += 64)
+                        gr[p] = 0.0f;
+                }
             }
         }
         return;
     }
 
-    // slices whose z-interpolated row can be non-zero
+    // slices whose z-interpolated rows can be non-zero (union over the R rows)
     if (tid == 0) {
         s_lo = Ns;
         s_hi = -1;
     }
-    const int hbuf_elems = 2 * SB * A.hrow;
+    const int hbuf_elems = 2 * R * SB * FWD_HP;
     for (int i = tid; i < hbuf_elems; i += FWD_THREADS)
         hbuf[i] = 0.0f;
     __syncthreads();
     for (int is = tid; is < Ns; is += FWD_THREADS) {
         const int4 e = ztab[int64_t(iv1) * Ns + is];
-        const int iza = iu1 - e.x - 1;
-        const bool nz = (iza >= 0 && iza < Nz && __int_as_float(e.y) > 0.0f) ||
-                        (iza + 1 >= 0 && iza + 1 < Nz && __int_as_float(e.z) > 0.0f);
+        bool nz = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int iza = iu1_0 + r - e.x - 1;
+            nz = nz || (r < nrows && ((iza >= 0 && iza < Nz && __int_as_float(e.y) > 0.0f) ||
+                                      (iza + 1 >= 0 && iza + 1 < Nz && __int_as_float(e.z) > 0.0f)));
+        }
         if (nz) {
             atomicMin(&s_lo, is);
             atomicMax(&s_hi, is);
@@ -182,105 +205,118 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
     const int is_lo = __builtin_amdgcn_readfirstlane(s_lo), is_hi = __builtin_amdgcn_readfirstlane(s_hi);
 
     // Q[p] = sum wq*v, T[p] = sum v over the values v = H[s][p+k] read by lane p;  out[p] = Q[p] + T[p+1] - Q[p+1]
-    float Q[J][C], T[J][C];
+    float Q[R][J][C], T[R][J][C];
 #pragma unroll
-    for (int j = 0; j < J; j++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            Q[j][c] = 0.0f;
-            T[j][c] = 0.0f;
-        }
+        for (int j = 0; j < J; j++)
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                Q[r][j][c] = 0.0f;
+                T[r][j][c] = 0.0f;
+            }
 
     if (is_lo <= is_hi) {
-        const int nblk = (is_hi - is_lo) / SB + 1;
-        constexpr int MAXE = (SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128 per slice row
+        // blocks start at multiples of SB so the table can hold each slice's LDS row offset
+        const int is_first = (is_lo / SB) * SB;
+        const int nblk = (is_hi - is_first) / SB + 1;
+        constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
         ZLoads<MAXE> L;
-        zload<SB>(L, A, f, ztab, iv1, iu1, is_lo, is_hi, tid);
+        zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first, tid);
         zstore(L, hbuf + A.hpad);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
             const bool more = blk + 1 < nblk;
             if (more)
-                zload<SB>(L, A, f, ztab, iv1, iu1, is_lo + (blk + 1) * SB, is_hi, tid);
-            const int is0 = is_lo + blk * SB;
-            const float* hb = hbuf + (blk & 1) * SB * A.hrow + A.hpad + lane;
+                zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
+            const int is0 = is_first + blk * SB;
+            const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
 #pragma unroll
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
-                    // table rows are padded by SB entries, so the block's entries are contiguous
+                    // entry: {(is % SB) * FWD_HP + k, wq}; rows are padded so a block never leaves the row
                     const int2* xt = xtab + int64_t(iv0) * A.xrow + is0;
-                    int k[SB];
+                    int off[SB];
                     float wq[SB];
 #pragma unroll
                     for (int s = 0; s < SB; s++) {
                         const int2 e = xt[s];
-                        k[s] = e.x;
+                        off[s] = e.x;
                         wq[s] = __int_as_float(e.y);
                     }
                     // k is monotone in the slice index, so the block's extremes are its end values
-                    const int kmin = min(k[0], k[SB - 1]), kmax = max(k[0], k[SB - 1]);
+                    const int k0 = off[0], k1 = off[SB - 1] - (SB - 1) * FWD_HP;
+                    const int kmin = min(k0, k1), kmax = max(k0, k1);
                     const int c_lo = max(0, (-kmax) / 64);  // first chunk with 64c + 63 >= -kmax
                     const int c_hi = min(C - 1, (Na - 1 - kmin) >= 0 ? (Na - 1 - kmin) / 64 : -1);
                     const float* hp[SB];
 #pragma unroll
                     for (int s = 0; s < SB; s++)
-                        hp[s] = hb + s * A.hrow + k[s];
+                        hp[s] = hb + off[s];
 #pragma unroll
                     for (int c = 0; c < C; c++) {
                         if (c >= c_lo && c <= c_hi) {
-                            float v[SB];
 #pragma unroll
-                            for (int s = 0; s < SB; s++)
-                                v[s] = hp[s][64 * c];
+                            for (int r = 0; r < R; r++) {
+                                float v[SB];
 #pragma unroll
-                            for (int s = 0; s < SB; s++) {
-                                T[j][c] += v[s];
-                                Q[j][c] += wq[s] * v[s];
+                                for (int s = 0; s < SB; s++)
+                                    v[s] = hp[s][r * SB * FWD_HP + 64 * c];
+#pragma unroll
+                                for (int s = 0; s < SB; s++) {
+                                    T[r][j][c] += v[s];
+                                    Q[r][j][c] += wq[s] * v[s];
+                                }
                             }
                         }
                     }
                 }
             }
             if (more)
-                zstore(L, hbuf + ((blk + 1) & 1) * SB * A.hrow + A.hpad);
+                zstore(L, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad);
             __syncthreads();
         }
     }
 
     // epilogue: out[p] = (Q[p] + T[p+1] - Q[p+1]) * scale, masks, optional ratio
 #pragma unroll
-    for (int j = 0; j < J; j++) {
-        const int iv0 = iv0_base + FWD_WAVES * j;
-        if (iv0 < Nv0) {
-            const float sc = fscale[int64_t(iv1) * Nv0 + iv0];
-            const float v0 = coordf(iv0, P.Tv0, P.v0_0);
-            float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0;
+    for (int r = 0; r < R; r++) {
+        if (r >= nrows)
+            continue;
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                float tn = __shfl_down(T[j][c], 1);
-                float qn = __shfl_down(Q[j][c], 1);
-                const int cn = (c + 1 < C) ? c + 1 : c;
-                const float t0 = (c + 1 < C) ? __shfl(T[j][cn], 0) : 0.0f;
-                const float q0 = (c + 1 < C) ? __shfl(Q[j][cn], 0) : 0.0f;
-                if (lane == 63) {
-                    tn = t0;
-                    qn = q0;
-                }
-                const int p = 64 * c + lane;
-                if (p < Nu0) {
-                    float out = (Q[j][c] + (tn - qn)) * sc;
-                    if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
-                        out = 0.0f;
-                    const int64_t gi = int64_t(gr - g) + p;
-                    if (resp)
-                        out = out * resp[gi];
-                    if (epilogue == EPI_RATIO) {
-                        out = (out == 0.0f) ? 0.0f : yv[gi] / out;
+        for (int j = 0; j < J; j++) {
+            const int iv0 = iv0_base + FWD_WAVES * j;
+            if (iv0 < Nv0) {
+                const float sc = rmask[r] ? 0.0f : fscale[int64_t(iv1) * Nv0 + iv0];
+                const float v0 = coordf(iv0, P.Tv0, P.v0_0);
+                float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    float tn = __shfl_down(T[r][j][c], 1);
+                    float qn = __shfl_down(Q[r][j][c], 1);
+                    const int cn = (c + 1 < C) ? c + 1 : c;
+                    const float t0 = (c + 1 < C) ? __shfl(T[r][j][cn], 0) : 0.0f;
+                    const float q0 = (c + 1 < C) ? __shfl(Q[r][j][cn], 0) : 0.0f;
+                    if (lane == 63) {
+                        tn = t0;
+                        qn = q0;
+                    }
+                    const int p = 64 * c + lane;
+                    if (p < Nu0) {
+                        float out = (Q[r][j][c] + (tn - qn)) * sc;
+                        if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
+                            out = 0.0f;
+                        const int64_t gi = int64_t(gr - g) + p;
                         if (resp)
                             out = out * resp[gi];
+                        if (epilogue == EPI_RATIO) {
+                            out = (out == 0.0f) ? 0.0f : yv[gi] / out;
+                            if (resp)
+                                out = out * resp[gi];
+                        }
+                        gr[p] = out;
                     }
-                    gr[p] = out;
                 }
             }
         }
@@ -299,6 +335,7 @@ struct BackArgs {
     int mode;
     int psi90;
     int v1_first, v1_stride;
+    int sb;  // slice block of the forward table layout
 };
 
 template <int AG>
@@ -352,6 +389,7 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ 
         if (blk + 1 < nblk)
             load(blk + 1, (blk + 1) & 1);
         const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
+        const int row_off = (is % A.sb) * FWD_HP;
         // the next view's table entry is loaded while the current view's window runs
         int2 e_next = xtab[int64_t(min(blk * BACK_RB, Nv0 - 1)) * xrow + is];
 #pragma unroll 1
@@ -361,7 +399,7 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ 
             if (r + 1 < BACK_RB)
                 e_next = xtab[int64_t(min(iv0 + 1, Nv0 - 1)) * xrow + is];
             if (iv0 < Nv0) {
-                const int m = -e.x - 1;
+                const int m = -(e.x - row_off) - 1;  // the table's .x carries the forward's LDS row offset
                 const float wq = __int_as_float(e.y), wp = 1.0f - wq;
                 const float* s = sb + r * A.srow + m;
                 float xp = s[0];
@@ -473,28 +511,31 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
 typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int2*, const int4*, const float*, int, const float*,
                        const float*);
 
-template <int J, int SB>
+template <int J, int SB, int R>
 fwd_fn pick_fwd_c(int C)
 {
     switch (C) {
-    case 1: return fwd_fast_kernel<J, 1, SB>;
-    case 2: return fwd_fast_kernel<J, 2, SB>;
-    case 3: return fwd_fast_kernel<J, 3, SB>;
-    case 4: return fwd_fast_kernel<J, 4, SB>;
-    case 5: return fwd_fast_kernel<J, 5, SB>;
-    case 6: return fwd_fast_kernel<J, 6, SB>;
-    case 7: return fwd_fast_kernel<J, 7, SB>;
-    case 8: return fwd_fast_kernel<J, 8, SB>;
+    case 1: return fwd_fast_kernel<J, 1, SB, R>;
+    case 2: return fwd_fast_kernel<J, 2, SB, R>;
+    case 3: return fwd_fast_kernel<J, 3, SB, R>;
+    case 4: return fwd_fast_kernel<J, 4, SB, R>;
+    case 5: return fwd_fast_kernel<J, 5, SB, R>;
+    case 6: return fwd_fast_kernel<J, 6, SB, R>;
+    case 7: return fwd_fast_kernel<J, 7, SB, R>;
+    case 8: return fwd_fast_kernel<J, 8, SB, R>;
     default: return nullptr;
     }
 }
 
-fwd_fn pick_fwd(int J, int C, int SB)
+// (views per wave, u1 rows per workgroup): (7, 1), (4, 2), (2, 2); slice blocks of 8
+fwd_fn pick_fwd(int J, int C, int R)
 {
-    if (SB == 16)
-        return J == 7 ? pick_fwd_c<7, 16>(C) : (J == 4 ? pick_fwd_c<4, 16>(C) : nullptr);
-    if (SB == 8)
-        return J == 7 ? pick_fwd_c<7, 8>(C) : (J == 4 ? pick_fwd_c<4, 8>(C) : nullptr);
+    if (R == 1 && J == 7)
+        return pick_fwd_c<7, 8, 1>(C);
+    if (R == 2 && J == 4)
+        return pick_fwd_c<4, 8, 2>(C);
+    if (R == 2 && J == 2)
+        return pick_fwd_c<2, 8, 2>(C);
     return nullptr;
 }
 
@@ -569,17 +610,23 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->C = (P.Nu0 + 1 + 63) / 64;
     if (fp->C > 8 || Ns > 1024)
         return nullptr;
-    // views per wave: fewer views = fewer accumulators = more waves per SIMD to hide table latency
-    fp->J = (2 * 7 * fp->C <= 112) ? 7 : 4;
+    // u1 rows per workgroup and views per wave: R rows share each (view, slice) table entry and LDS
+    // address; the accumulator count 2*R*J*C stays at or below 64 so several waves fit per SIMD
+    fp->R = 2;
+    fp->J = (fp->C <= 4) ? 4 : 2;
+    if (const char* e = std::getenv("PETL_FWD_R")) {
+        if (std::atoi(e) == 1) {
+            fp->R = 1;
+            fp->J = 7;
+        }
+    }
     if (const char* e = std::getenv("PETL_FWD_J")) {
         const int j = std::atoi(e);
-        if ((j == 4 || j == 7) && 2 * j * fp->C <= 112)
+        if (fp->R == 2 && (j == 2 || j == 4) && 2 * fp->R * j * fp->C <= 64)
             fp->J = j;
     }
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
-    fp->SB = 8;  // measured: 8-slice blocks beat 16 (more waves per CU)
-    if (const char* e = std::getenv("PETL_FWD_SB"))
-        fp->SB = (std::atoi(e) == 16) ? 16 : 8;
+    fp->SB = 8;
     fp->AG = BACK_AG;
     fp->nagroups = (Na + BACK_AG - 1) / BACK_AG;
     fp->RB = BACK_RB;
@@ -613,7 +660,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
             const double m = std::floor(mu);
             const double phi = mu - m;
             const int mi = int(m);
-            xtab[size_t(iv0) * xrow + is] = make_int2(-mi - 1, ibits(float(phi)));
+            xtab[size_t(iv0) * xrow + is] = make_int2(-mi - 1, ibits(float(phi)));  // k for now
             mmin = std::min(mmin, mi);
             mmax = std::max(mmax, mi);
         }
@@ -624,6 +671,9 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
             const int kd = std::abs(xtab[size_t(iv0) * xrow + is].x - xtab[size_t(iv0) * xrow + lo].x);
             kspread = std::max(kspread, kd);
         }
+        // the forward reads entry .x as the slice's LDS offset within an SB-aligned block
+        for (int is = 0; is < xrow; is++)
+            xtab[size_t(iv0) * xrow + is].x += (is % fp->SB) * FWD_HP;
     }
     for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
         const double v1 = coordh(iv1, P.Tv1, P.v1_0);
@@ -677,16 +727,16 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         }
     }
     fp->hpad = 64 + kspread + 1;
-    fp->hrow = Na + 2 * fp->hpad;
-    if ((fp->hrow & 1) == 0)
-        fp->hrow += 1;
+    fp->hrow = FWD_HP;
+    if (fp->hpad + Na + 63 + kspread >= FWD_HP)  // reads of an active chunk must stay inside the LDS row
+        return nullptr;
     // back: S rows read at indices [m, m + NaPad] for m in [mmin, mmax]
     const int na_pad = fp->nagroups * BACK_AG;
     fp->spad_lo = std::max(0, -mmin);
     fp->srow = fp->spad_lo + std::max(P.Nu0, mmax + na_pad + 1) + 1;
     if ((fp->srow & 1) == 0)
         fp->srow += 1;
-    if (size_t(2) * fp->SB * fp->hrow * 4 > 64 * 1024 || size_t(2) * BACK_RB * fp->srow * 4 > 64 * 1024)
+    if (size_t(2) * BACK_RB * fp->srow * 4 > 64 * 1024)
         return nullptr;
 
     // ---- upload ----
@@ -733,11 +783,12 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
     A.v1_stride = v1_stride;
     A.xrow = fp.xrow;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn fn = pick_fwd(fp.J, fp.C, fp.SB);
+    fwd_fn fn = pick_fwd(fp.J, fp.C, fp.R);
     if (!fn)
         throw hip_error("fast forward: no kernel instance for this shape");
-    const size_t lds = size_t(2) * fp.SB * fp.hrow * sizeof(float);
-    hipLaunchKernelGGL(fn, dim3(fp.ngroups, fp.P.Nu1, nv1), dim3(FWD_THREADS), lds, s, g, f, A, fp.dev.xtab,
+    const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
+    hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s, g, f, A,
+                       fp.dev.xtab,
                        fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
     PETL_LAUNCH_CHECK();
 }
@@ -763,6 +814,7 @@ void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* wo
     A.srow = fp.srow;
     A.mode = mode;
     A.psi90 = fp.psi90;
+    A.sb = fp.SB;
     A.v1_first = v1_first;
     A.v1_stride = v1_stride;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
