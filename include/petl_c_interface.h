@@ -143,8 +143,9 @@ PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool syn
  * clipped) and overwritten with the result.  prior: 0 none, 1 relative differences (OSL, as the
  * reference's beta > 0 branch), 2 p-Huber TV (OSL).  mask / response: optional host planogram lists
  * (NULL for none), used as PETLserver.MLEM uses them.  numSubsets > 1 runs OSEM over interleaved v1
- * subsets.  loglik: optional host array of numIter doubles receiving sum(y log Af - Af) of each
- * iteration's forward projection (bins with Af > 0). */
+ * subsets.  loglik: optional host array of numIter doubles receiving sum(y log Af - Af) over bins with
+ * Af > 0, accumulated in FP64 over the iteration's forward projections (for OSEM: each subset's views,
+ * projected from the estimate in force when that subset is processed). */
 PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
                           int numSubsets, float** mask, float** response, double* loglik);
 

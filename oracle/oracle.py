@@ -3,7 +3,7 @@
 ctypes bindings for ``libpetl_oracle.so`` (the C++ restatement of the reference hot path, see
 petl_oracle.cpp) and for ``_ref/libpetl_ref_geom.so`` (the reference's own planogram/dataCube classes,
 built from the reference sources when they are present).  Only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg import this module, as the checker.  The product package never does.
+bench.py's cpu_baseline leg import this module, to check or time results.  The product package never does.
 """
 import ctypes
 import os
@@ -80,6 +80,8 @@ def lib():
         L.orc_max_threads.restype = c_int
         L.orc_project_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         L.orc_backproject_petl_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
+        L.orc_project_subset_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
+        L.orc_backproject_petl_subset_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         _lib = L
     return _lib
 
@@ -237,6 +239,33 @@ def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0
             f = f * divide(d, G)
         if callback is not None:
             callback(n, f, Pd)
+    return f
+
+
+def osem(plans, vol, g, num_iter, num_subsets, f0=None, nthreads=None):
+    """OSEM with interleaved v1-row subsets (rows s, s+n, ...), fp32, PETL-mode back projection; one
+    iteration = one pass over the subsets.  f0: start image (default ones).  With one subset and the
+    default start this is mlem()."""
+    L = lib()
+    f = np.ones(vol.shape, np.float32) if f0 is None else np.array(f0, dtype=np.float32)
+    S = []
+    for sub in range(num_subsets):
+        Ss = np.zeros(vol.shape, np.float32)
+        for p in plans:
+            L.orc_backproject_petl_subset_f32(ctypes.byref(p), ctypes.byref(vol), np.ones(p.shape, np.float32), Ss,
+                                              sub, num_subsets, _nt(nthreads))
+        Ss[Ss == 0] = 1
+        S.append(Ss)
+    for _ in range(num_iter):
+        for sub in range(num_subsets):
+            d = np.zeros(vol.shape, np.float32)
+            for p, gi in zip(plans, g):
+                q = np.zeros(p.shape, np.float32)
+                L.orc_project_subset_f32(ctypes.byref(p), ctypes.byref(vol), f, q, sub, num_subsets, _nt(nthreads))
+                r = rdivide(q, np.asarray(gi, np.float32))
+                L.orc_backproject_petl_subset_f32(ctypes.byref(p), ctypes.byref(vol), r, d, sub, num_subsets,
+                                                  _nt(nthreads))
+            f = f * divide(d, S[sub])
     return f
 
 

@@ -4,7 +4,7 @@
 //  This file is a plain C++17/OpenMP restatement of the reference algorithm for the separable-footprint
 //  (SF) planogram projector path.  It exists to CHECK the HIP product path, and to be timed as the CPU
 //  baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
-//  only as the checker.  Nothing in petl_amd/ links, loads or calls it.
+//  only to check or time results.  Nothing in petl_amd/ links, loads or calls it.
 //
 //  Every routine is templated on the arithmetic type R:
 //     R = float  : the fp32 restatement ("fp32 oracle").  Built with -ffp-contract=off so every
@@ -23,7 +23,7 @@
 //     relative-difference grad    src/relative_differences.cu:21-28, 107-173, 481-532
 //     p-Huber TV gradient         src/total_variation.cu:21-78, 434-660
 //
-//  Deliberate differences from the reference, each documented in DESIGN.md:
+//  Deliberate differences from the reference:
 //     * texture fetches are replaced by exact array reads: the forward samples texel centres (exact in
 //       CUDA as well); the backprojector uses exact bilinear weights instead of the hardware's 8-bit
 //       fractional filter weights (parity for that rounding is unpinned),
@@ -257,16 +257,17 @@ R enumerate_bin(const orc_plan& P, const orc_vol& V, int iv1, int iv0, int iu1, 
 
 template <typename R>
 void project_plan(const orc_plan& P, const orc_vol& V, const R* f, R* g, int nthreads, int v1_begin = 0,
-                  int v1_end = -1)
+                  int v1_end = -1, int v1_stride = 1)
 {
     if (v1_end < 0)
         v1_end = P.N_v1;
-    const int NV = (v1_end - v1_begin) * P.N_v0;
+    const int nrows = (v1_end - v1_begin + v1_stride - 1) / v1_stride;
+    const int NV = nrows * P.N_v0;
     const int64_t row = int64_t(P.N_u1) * P.N_u0;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int vv = 0; vv < NV; vv++) {
-        const int v = v1_begin * P.N_v0 + vv;
-        const int iv1 = v / P.N_v0, iv0 = v % P.N_v0;
+        const int iv1 = v1_begin + (vv / P.N_v0) * v1_stride, iv0 = vv % P.N_v0;
+        const int v = iv1 * P.N_v0 + iv0;
         R* gv = g + int64_t(v) * row;
         for (int iu1 = 0; iu1 < P.N_u1; iu1++)
             for (int iu0 = 0; iu0 < P.N_u0; iu0++) {
@@ -323,7 +324,7 @@ void backproject_matched_plan(const orc_plan& P, const orc_vol& V, const R* g, R
 // PETL-mode back projector: src/projectors.cu:240-377 with exact bilinear weights.  Accumulates into f.
 template <typename R>
 void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f, int nthreads, int v1_begin = 0,
-                           int v1_end = -1)
+                           int v1_end = -1, int v1_stride = 1)
 {
     if (v1_end < 0)
         v1_end = P.N_v1;
@@ -344,7 +345,7 @@ void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f
                 const float xf = coord_f(ix, V.voxelWidth, V.x0), yf = coord_f(iy, V.voxelWidth, V.y0);
                 const float zf = coord_f(iz, V.voxelHeight, V.z0);
                 R fsum = R(0);
-                for (int iv1 = v1_begin; iv1 < v1_end; iv1++) {
+                for (int iv1 = v1_begin; iv1 < v1_end; iv1 += v1_stride) {
                     R val = R(0);
                     // u1 test at the voxel centre (src/projectors.cu:284-289), evaluated in float
                     const float v1f = coord_f(iv1, P.T_v1, P.v1_0);
@@ -692,6 +693,19 @@ void orc_backproject_petl_rows_f32(const orc_plan* P, const orc_vol* V, const fl
                                    int v1_end, int nthreads)
 {
     backproject_petl_plan<float>(*P, *V, g, f, nthreads, v1_begin, v1_end);
+}
+
+// OSEM subsets: v1 rows first, first + stride, ...
+void orc_project_subset_f32(const orc_plan* P, const orc_vol* V, const float* f, float* g, int first, int stride,
+                            int nthreads)
+{
+    project_plan<float>(*P, *V, f, g, nthreads, first, P->N_v1, stride);
+}
+
+void orc_backproject_petl_subset_f32(const orc_plan* P, const orc_vol* V, const float* g, float* f, int first,
+                                     int stride, int nthreads)
+{
+    backproject_petl_plan<float>(*P, *V, g, f, nthreads, first, P->N_v1, stride);
 }
 
 }  // extern "C"

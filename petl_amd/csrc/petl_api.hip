@@ -242,9 +242,11 @@ bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float bet
             if (loglik) {
                 m.forward(q.data(), fc, EPI_NONE, nullptr, response ? resp.data() : nullptr, s, sub, nsub);
                 for (size_t i = 0; i < nP; i++) {
-                    const int64_t n = m.plans[i].elems();
-                    launch_loglik(y[i], q[i], n, ll_scratch, dll + it, !(i == 0 && sub == 0), s);
-                    launch_ratio(aux[i], y[i], q[i], response ? resp[i] : nullptr, n, s);
+                    const Planogram& pl = m.plans[i];
+                    const int64_t row = int64_t(pl.N_v0) * pl.N_u1 * pl.N_u0;  // one v1 view row
+                    launch_loglik(y[i], q[i], row, sub, nsub, pl.N_v1, ll_scratch, dll + it, !(i == 0 && sub == 0),
+                                  s);
+                    launch_ratio(aux[i], y[i], q[i], response ? resp[i] : nullptr, row, sub, nsub, pl.N_v1, s);
                 }
             } else {
                 m.forward(aux.data(), fc, EPI_RATIO, y.data(), response ? resp.data() : nullptr, s, sub, nsub);

@@ -19,6 +19,7 @@ namespace petl {
 
 struct FastTables {
     int2* xtab = nullptr;     // [Nv0][Ns + pad]  {k, wq}: u0 interpolation of slice s in view v0
+    float* xwp = nullptr;     // [Nv0][Ns + pad]  1 - wq, formed in float
     int4* ztab = nullptr;     // [Nv1][Ns]  {m1, wa, wb, 0}: u1 interpolation of slice s in view v1
     float* fscale = nullptr;  // [Nv1][Nv0] forward / matched scale  l_phi / sqrt(1+v0^2+v1^2)
     float* pscale = nullptr;  // [Nv1][Nv0] PETL back scale  T_x / max(|wy|,|wx|) / sqrt(1+v0^2+v1^2)

@@ -37,6 +37,7 @@ namespace petl {
 namespace {
 
 constexpr int FWD_THREADS = 512;
+constexpr int FWD_XCDS = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
 constexpr int FWD_WAVES = FWD_THREADS / 64;
 constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
 constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows
@@ -133,6 +134,7 @@ __device__ __forceinline__ void zstore(const ZLoads<MAXE>& L, float* hb)
 template <int J, int C, int SB, int R>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict__ g, const float* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ xtab,
+                                                               const float* __restrict__ xwp,
                                                                const int4* __restrict__ ztab,
                                                                const float* __restrict__ fscale, int epilogue,
                                                                const float* __restrict__ yv,
@@ -143,8 +145,20 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int grp = blockIdx.x, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
-    const int iu1_0 = blockIdx.y * R;
+    // The dispatcher deals workgroup L to XCD L % 8.  Renumber so each XCD walks a contiguous range of
+    // (view group, u1 row, v1): the workgroups resident on one XCD then stage neighbouring u1 rows of one
+    // v1, i.e. one band of the volume, which stays in that XCD's L2.
+    int grp, iu1_0, iv1;
+    {
+        const int nwg = gridDim.x * gridDim.y * gridDim.z;
+        const int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int per = nwg / FWD_XCDS, rem = nwg % FWD_XCDS, xcd = L % FWD_XCDS;
+        const int M = xcd * per + min(xcd, rem) + L / FWD_XCDS;
+        grp = M % gridDim.x;
+        const int rest = M / gridDim.x;
+        iu1_0 = (rest % gridDim.y) * R;
+        iv1 = A.v1_first + (rest / gridDim.y) * A.v1_stride;
+    }
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz;
@@ -204,8 +218,10 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
     __syncthreads();
     const int is_lo = __builtin_amdgcn_readfirstlane(s_lo), is_hi = __builtin_amdgcn_readfirstlane(s_hi);
 
-    // Q[p] = sum wq*v, T[p] = sum v over the values v = H[s][p+k] read by lane p;  out[p] = Q[p] + T[p+1] - Q[p+1]
-    float Q[R][J][C], T[R][J][C];
+    // Q[p] = sum wq*v, W[p] = sum wp*v over the values v = H[s][p+k] read by lane p;  out[p] = Q[p] + W[p+1].
+    // Both sums hold non-negative terms for a non-negative volume, so a bin whose exact value is zero comes
+    // out as exactly zero (a ratio y/q downstream depends on that).
+    float Q[R][J][C], W[R][J][C];
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -213,7 +229,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
 #pragma unroll
             for (int c = 0; c < C; c++) {
                 Q[r][j][c] = 0.0f;
-                T[r][j][c] = 0.0f;
+                W[r][j][c] = 0.0f;
             }
 
     if (is_lo <= is_hi) {
@@ -231,20 +247,37 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
                 zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
             const int is0 = is_first + blk * SB;
             const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
+            // table entries {(is % SB) * FWD_HP + k, wq} and wp of view j; the next view's entries are
+            // requested before the current view's sums so their latency overlaps the LDS reads
+            int off_n[SB];
+            float wq_n[SB], wp_n[SB];
+            auto xload = [&](int j) {
+                const int iv0c = min(iv0_base + FWD_WAVES * j, Nv0 - 1);
+                const int2* xt = xtab + int64_t(iv0c) * A.xrow + is0;  // rows are padded past Ns
+                const float* xp = xwp + int64_t(iv0c) * A.xrow + is0;
+#pragma unroll
+                for (int s = 0; s < SB; s++) {
+                    const int2 e = xt[s];
+                    off_n[s] = e.x;
+                    wq_n[s] = __int_as_float(e.y);
+                    wp_n[s] = xp[s];
+                }
+            };
+            xload(0);
 #pragma unroll
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
-                if (iv0 < Nv0) {
-                    // entry: {(is % SB) * FWD_HP + k, wq}; rows are padded so a block never leaves the row
-                    const int2* xt = xtab + int64_t(iv0) * A.xrow + is0;
-                    int off[SB];
-                    float wq[SB];
+                int off[SB];
+                float wq[SB], wp[SB];
 #pragma unroll
-                    for (int s = 0; s < SB; s++) {
-                        const int2 e = xt[s];
-                        off[s] = e.x;
-                        wq[s] = __int_as_float(e.y);
-                    }
+                for (int s = 0; s < SB; s++) {
+                    off[s] = off_n[s];
+                    wq[s] = wq_n[s];
+                    wp[s] = wp_n[s];
+                }
+                if (j + 1 < J)
+                    xload(j + 1);
+                if (iv0 < Nv0) {
                     // k is monotone in the slice index, so the block's extremes are its end values
                     const int k0 = off[0], k1 = off[SB - 1] - (SB - 1) * FWD_HP;
                     const int kmin = min(k0, k1), kmax = max(k0, k1);
@@ -265,7 +298,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
                                     v[s] = hp[s][r * SB * FWD_HP + 64 * c];
 #pragma unroll
                                 for (int s = 0; s < SB; s++) {
-                                    T[r][j][c] += v[s];
+                                    W[r][j][c] += wp[s] * v[s];
                                     Q[r][j][c] += wq[s] * v[s];
                                 }
                             }
@@ -279,7 +312,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
         }
     }
 
-    // epilogue: out[p] = (Q[p] + T[p+1] - Q[p+1]) * scale, masks, optional ratio
+    // epilogue: out[p] = (Q[p] + W[p+1]) * scale, masks, optional ratio
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (r >= nrows)
@@ -293,18 +326,14 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
                 float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    float tn = __shfl_down(T[r][j][c], 1);
-                    float qn = __shfl_down(Q[r][j][c], 1);
+                    float wn = __shfl_down(W[r][j][c], 1);
                     const int cn = (c + 1 < C) ? c + 1 : c;
-                    const float t0 = (c + 1 < C) ? __shfl(T[r][j][cn], 0) : 0.0f;
-                    const float q0 = (c + 1 < C) ? __shfl(Q[r][j][cn], 0) : 0.0f;
-                    if (lane == 63) {
-                        tn = t0;
-                        qn = q0;
-                    }
+                    const float w0 = (c + 1 < C) ? __shfl(W[r][j][cn], 0) : 0.0f;
+                    if (lane == 63)
+                        wn = w0;
                     const int p = 64 * c + lane;
                     if (p < Nu0) {
-                        float out = (Q[r][j][c] + (tn - qn)) * sc;
+                        float out = (Q[r][j][c] + wn) * sc;
                         if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
                             out = 0.0f;
                         const int64_t gi = int64_t(gr - g) + p;
@@ -508,8 +537,8 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
 // ---------------------------------------------------------------------------------------------
 // host-side kernel selection
 // ---------------------------------------------------------------------------------------------
-typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int2*, const int4*, const float*, int, const float*,
-                       const float*);
+typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int2*, const float*, const int4*, const float*, int,
+                       const float*, const float*);
 
 template <int J, int SB, int R>
 fwd_fn pick_fwd_c(int C)
@@ -611,13 +640,14 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     if (fp->C > 8 || Ns > 1024)
         return nullptr;
     // u1 rows per workgroup and views per wave: R rows share each (view, slice) table entry and LDS
-    // address; the accumulator count 2*R*J*C stays at or below 64 so several waves fit per SIMD
-    fp->R = 2;
-    fp->J = (fp->C <= 4) ? 4 : 2;
+    // address; each lane holds 2*R*J*C accumulators (56 at 228 u0 bins)
+    fp->R = 1;
+    fp->J = 7;
+    // (4, 2) and (2, 2) stay selectable for tuning sweeps; (7, 1) measured fastest at 128^3 / 228 u0 bins
     if (const char* e = std::getenv("PETL_FWD_R")) {
-        if (std::atoi(e) == 1) {
-            fp->R = 1;
-            fp->J = 7;
+        if (std::atoi(e) == 2) {
+            fp->R = 2;
+            fp->J = (fp->C <= 4) ? 4 : 2;
         }
     }
     if (const char* e = std::getenv("PETL_FWD_J")) {
@@ -640,6 +670,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     const int xrow = Ns + FWD_SBMAX;  // padded so a block of slice entries never leaves the row
     fp->xrow = xrow;
     std::vector<int2> xtab(size_t(P.Nv0) * xrow);
+    std::vector<float> xwp(size_t(P.Nv0) * xrow);
     std::vector<int4> ztab(size_t(P.Nv1) * Ns);
     std::vector<float> fscale(size_t(P.Nv1) * P.Nv0), pscale(size_t(P.Nv1) * P.Nv0);
     std::vector<int2> rows(2 * size_t(P.Nv1));
@@ -666,6 +697,10 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         }
         for (int is = Ns; is < xrow; is++)
             xtab[size_t(iv0) * xrow + is] = xtab[size_t(iv0) * xrow + Ns - 1];
+        // wp = 1 - wq in float, the value the back projector forms from wq, so matched mode stays an exact
+        // transpose of the forward
+        for (int is = 0; is < xrow; is++)
+            xwp[size_t(iv0) * xrow + is] = 1.0f - fbits(xtab[size_t(iv0) * xrow + is].y);
         for (int is = 0; is < xrow; is++) {
             const int lo = std::max(0, is - fp->SB + 1);
             const int kd = std::abs(xtab[size_t(iv0) * xrow + is].x - xtab[size_t(iv0) * xrow + lo].x);
@@ -740,16 +775,18 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         return nullptr;
 
     // ---- upload ----
-    const size_t nx = xtab.size() * sizeof(int2), nzb = ztab.size() * sizeof(int4);
+    const size_t nx = xtab.size() * sizeof(int2), nxw = xwp.size() * sizeof(float), nzb = ztab.size() * sizeof(int4);
     const size_t nf = fscale.size() * sizeof(float), nr = rows.size() * sizeof(int2);
     auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
-    const size_t total = al(nx) + al(nzb) + 2 * al(nf) + al(nr);
+    const size_t total = al(nx) + al(nxw) + al(nzb) + 2 * al(nf) + al(nr);
     char* blk = nullptr;
     PETL_HIP_CHECK(hipMalloc(&blk, total));
     fp->dev_block = blk;
     size_t off = 0;
     fp->dev.xtab = reinterpret_cast<int2*>(blk + off);
     off += al(nx);
+    fp->dev.xwp = reinterpret_cast<float*>(blk + off);
+    off += al(nxw);
     fp->dev.ztab = reinterpret_cast<int4*>(blk + off);
     off += al(nzb);
     fp->dev.fscale = reinterpret_cast<float*>(blk + off);
@@ -758,6 +795,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     off += al(nf);
     fp->dev.rows = reinterpret_cast<int2*>(blk + off);
     PETL_HIP_CHECK(hipMemcpy(fp->dev.xtab, xtab.data(), nx, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.xwp, xwp.data(), nxw, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.ztab, ztab.data(), nzb, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.fscale, fscale.data(), nf, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.pscale, pscale.data(), nf, hipMemcpyHostToDevice));
@@ -788,8 +826,7 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
     hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s, g, f, A,
-                       fp.dev.xtab,
-                       fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
+                       fp.dev.xtab, fp.dev.xwp, fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
     PETL_LAUNCH_CHECK();
 }
 

@@ -17,12 +17,14 @@ void launch_mlem_update_prior(float* f_out, const float* f, const float* d, cons
                               int kind, float delta, float beta, float p, hipStream_t s);
 // MLEM ratio from a stored projection: r = (q == 0 ? 0 : y / q), then r *= resp when resp != null
 // (rdivide of src/data_cube.cpp:150-155 followed by the response weighting of the back projection).
-void launch_ratio(float* r, const float* y, const float* q, const float* resp, int64_t n, hipStream_t s);
+// Bins are taken in rows of row_elems, rows first, first + stride, ... (< nrows_total): a v1 subset.
+void launch_ratio(float* r, const float* y, const float* q, const float* resp, int64_t row_elems, int first,
+                  int stride, int nrows_total, hipStream_t s);
 // Poisson log-likelihood sum(y log(q) - q) over bins with q > 0, in FP64; q = projection (times response).
 // Accumulates into *out (device double) via a two-pass deterministic reduction using `scratch`
 // (at least ll_scratch_doubles(n) doubles).
 int64_t ll_scratch_doubles(int64_t n);
-void launch_loglik(const float* y, const float* q, int64_t n, double* scratch, double* out, bool accumulate,
-                   hipStream_t s);
+void launch_loglik(const float* y, const float* q, int64_t row_elems, int first, int stride, int nrows_total,
+                   double* scratch, double* out, bool accumulate, hipStream_t s);
 
 }  // namespace petl

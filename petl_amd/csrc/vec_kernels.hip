@@ -40,10 +40,18 @@ __global__ void mlem_update_kernel(float* __restrict__ f, const float* __restric
     }
 }
 
-__global__ void ratio_kernel(float* __restrict__ r, const float* __restrict__ y, const float* __restrict__ q,
-                             const float* __restrict__ resp, int64_t n)
+// index of the i-th bin of a row subset
+__device__ __forceinline__ int64_t subset_index(int64_t i, int64_t row_elems, int first, int stride)
 {
-    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+    const int64_t row = i / row_elems;
+    return (first + row * stride) * row_elems + (i - row * row_elems);
+}
+
+__global__ void ratio_kernel(float* __restrict__ r, const float* __restrict__ y, const float* __restrict__ q,
+                             const float* __restrict__ resp, int64_t n, int64_t row_elems, int first, int stride)
+{
+    for (int64_t ii = int64_t(blockIdx.x) * VT + threadIdx.x; ii < n; ii += int64_t(gridDim.x) * VT) {
+        const int64_t i = subset_index(ii, row_elems, first, stride);
         const float qq = q[i];
         float out = (qq == 0.0f) ? 0.0f : y[i] / qq;
         if (resp)
@@ -75,11 +83,13 @@ __global__ void __launch_bounds__(64) mlem_update_prior_kernel(float* __restrict
 
 // pass 1: per-block partial sums in FP64
 __global__ void __launch_bounds__(VT) loglik_partial_kernel(const float* __restrict__ y, const float* __restrict__ q,
-                                                            int64_t n, double* __restrict__ part)
+                                                            int64_t n, int64_t row_elems, int first, int stride,
+                                                            double* __restrict__ part)
 {
     __shared__ double red[VT];
     double acc = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+    for (int64_t ii = int64_t(blockIdx.x) * VT + threadIdx.x; ii < n; ii += int64_t(gridDim.x) * VT) {
+        const int64_t i = subset_index(ii, row_elems, first, stride);
         const double qq = q[i];
         if (qq > 0.0)
             acc += double(y[i]) * log(qq) - qq;
@@ -156,19 +166,28 @@ void launch_mlem_update_prior(float* f_out, const float* f, const float* d, cons
     PETL_LAUNCH_CHECK();
 }
 
-void launch_ratio(float* r, const float* y, const float* q, const float* resp, int64_t n, hipStream_t s)
+static int64_t subset_bins(int64_t row_elems, int first, int stride, int nrows_total)
 {
-    hipLaunchKernelGGL(ratio_kernel, dim3(grid_for(n)), dim3(VT), 0, s, r, y, q, resp, n);
+    return int64_t((nrows_total - first + stride - 1) / stride) * row_elems;
+}
+
+void launch_ratio(float* r, const float* y, const float* q, const float* resp, int64_t row_elems, int first,
+                  int stride, int nrows_total, hipStream_t s)
+{
+    const int64_t n = subset_bins(row_elems, first, stride, nrows_total);
+    hipLaunchKernelGGL(ratio_kernel, dim3(grid_for(n)), dim3(VT), 0, s, r, y, q, resp, n, row_elems, first, stride);
     PETL_LAUNCH_CHECK();
 }
 
 int64_t ll_scratch_doubles(int64_t n) { return std::min<int64_t>((n + VT - 1) / VT, 4096); }
 
-void launch_loglik(const float* y, const float* q, int64_t n, double* scratch, double* out, bool accumulate,
-                   hipStream_t s)
+void launch_loglik(const float* y, const float* q, int64_t row_elems, int first, int stride, int nrows_total,
+                   double* scratch, double* out, bool accumulate, hipStream_t s)
 {
+    const int64_t n = subset_bins(row_elems, first, stride, nrows_total);
     const int nparts = int(ll_scratch_doubles(n));
-    hipLaunchKernelGGL(loglik_partial_kernel, dim3(nparts), dim3(VT), 0, s, y, q, n, scratch);
+    hipLaunchKernelGGL(loglik_partial_kernel, dim3(nparts), dim3(VT), 0, s, y, q, n, row_elems, first, stride,
+                       scratch);
     PETL_LAUNCH_CHECK();
     hipLaunchKernelGGL(loglik_final_kernel, dim3(1), dim3(VT), 0, s, scratch, nparts, out, accumulate ? 1 : 0);
     PETL_LAUNCH_CHECK();
