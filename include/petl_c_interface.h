@@ -149,4 +149,45 @@ PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool syn
 PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
                           int numSubsets, float** mask, float** response, double* loglik);
 
+/* ------------------------------------------------------------------------------------------------
+ * Part 3: reconstruction service (no counterpart in the reference).  Continuous batching of MLEM/OSEM
+ * requests of different panel and volume sizes on one GPU: a submitted request waits until a slot is
+ * free and its device footprint fits the HBM budget, then gets its own device buffers and stream; each
+ * MLEM iteration is captured once as a hipGraph and replayed.  Requests that arrive while others run
+ * are admitted as soon as slots and memory free up, and their work overlaps on the device.  A request
+ * gives the same image as MLEM_gpu on the same inputs, alone or in any mix.
+ * ---------------------------------------------------------------------------------------------- */
+#define PETL_MAX_PLANOGRAMS 8
+typedef struct petl_recon_request {
+    int num_planograms;                /* 1 .. PETL_MAX_PLANOGRAMS */
+    float psi[PETL_MAX_PLANOGRAMS];    /* panel angles, degrees */
+    float D, L, H, T;                  /* add_planogram geometry (mm), shared by the planograms */
+    float v_m0, v_m1;                  /* add_planogram's v_m0, v_m1 */
+    int numX, numY, numZ;              /* volume */
+    float voxelWidth, voxelHeight;
+    float** y;                         /* host data, one planogram array each; kept alive until wait returns */
+    int numIter;
+    int numSubsets;                    /* OSEM subsets (1 = MLEM) */
+    int prior;                         /* 0 none, 1 relative differences, 2 p-Huber TV (one-step-late) */
+    float delta, beta, p;
+} petl_recon_request;
+
+/* hbm_budget_bytes: device memory the service may hold for requests; max_slots: requests on the device
+ * at once.  Returns NULL on failure. */
+PETL_EXTERN void* petl_service_create(double hbm_budget_bytes, int max_slots);
+/* Queues a request (the struct is copied; the data arrays are read when the request is admitted).
+ * Returns its id (>= 0), or -1 when the request is invalid or can never fit the budget. */
+PETL_EXTERN int petl_service_submit(void* service, const petl_recon_request* request);
+/* Device bytes a request would hold (its admission footprint), or 0 when it is invalid. */
+PETL_EXTERN double petl_service_request_bytes(const petl_recon_request* request);
+/* Blocks until request `id` is done and copies its image (numZ x numY x numX) to f.  times (optional,
+ * 3 doubles): seconds from submission to admission, from admission to completion, and in total.
+ * Returns false for an unknown id or a failed request. */
+PETL_EXTERN bool petl_service_wait(void* service, int id, float* f, double* times);
+/* stats (6 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
+ * requests currently queued. */
+PETL_EXTERN bool petl_service_stats(void* service, double* stats);
+/* Finishes every queued request, then frees the service. */
+PETL_EXTERN void petl_service_destroy(void* service);
+
 #endif /* PETL_C_INTERFACE_H */

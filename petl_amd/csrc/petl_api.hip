@@ -12,6 +12,7 @@
 #include <vector>
 #include "petl_common.h"
 #include "petl_model.h"
+#include "mlem_job.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
 #include "../../include/petl_c_interface.h"
@@ -124,147 +125,34 @@ float inner3(const float* x, const float* y, const float* w, int N1, int N2, int
     return float(acc);
 }
 
-// ---- device staging helpers ----
-struct PlanBufs {
-    std::vector<float*> ptr;
-};
-
-void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs)
-{
-    bufs.resize(m.plans.size());
-    for (size_t i = 0; i < m.plans.size(); i++)
-        bufs[i].ensure(size_t(m.plans[i].elems()) * sizeof(float));
-}
-
-std::vector<float*> ptrs(std::vector<DevBuf>& bufs)
-{
-    std::vector<float*> p(bufs.size());
-    for (size_t i = 0; i < bufs.size(); i++)
-        p[i] = bufs[i].f();
-    return p;
-}
-
-void h2d(void* d, const void* h, size_t n, hipStream_t s)
-{
-    PETL_HIP_CHECK(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s));
-}
-void d2h(void* h, const void* d, size_t n, hipStream_t s)
-{
-    PETL_HIP_CHECK(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s));
-}
-
 // ---- device-resident MLEM / OSEM (src/petl_server.py:1534-1606, 1257-1273) ----
 bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float beta, int prior, float p, int nsub,
               float** mask, float** response, double* loglik)
 {
     if (!g || !f || !m.allDefined() || numIter < 0)
         return false;
-    const size_t nP = m.plans.size();
-    for (size_t i = 0; i < nP; i++)
+    for (size_t i = 0; i < m.plans.size(); i++)
         if (!g[i] || (mask && !mask[i]) || (response && !response[i]))
             return false;
-    int Nv1_min = m.plans[0].N_v1;
-    for (const auto& pl : m.plans)
-        Nv1_min = std::min(Nv1_min, pl.N_v1);
-    nsub = std::max(1, std::min(nsub, Nv1_min));
-    hipStream_t s = m.stream();
-    m.prepare();
-    DeviceState& D = m.dev;
-    const int64_t V = m.vol_elems();
-
+    MlemParams prm;
+    prm.numIter = numIter;
+    prm.delta = delta;
+    prm.beta = beta;
+    prm.prior = prior;
+    prm.p = p;
+    prm.nsub = nsub;
+    prm.loglik = loglik != nullptr;
+    prm.mask = mask != nullptr;
+    prm.response = response != nullptr;
     // initial estimate (src/petl_server.py:1571-1577), in place on the caller's array
-    bool all_zero = true;
-    for (int64_t i = 0; i < V; i++)
-        if (f[i] != 0.0f) {
-            all_zero = false;
-            break;
-        }
-    if (all_zero)
-        std::fill(f, f + V, 1.0f);
-    else
-        for (int64_t i = 0; i < V; i++)
-            if (f[i] < 0.0f)
-                f[i] = 0.0f;
-
-    ensure_plan_bufs(m, D.y);
-    ensure_plan_bufs(m, D.aux);
-    if (response)
-        ensure_plan_bufs(m, D.resp);
-    D.f.ensure(V * sizeof(float));
-    D.f2.ensure(V * sizeof(float));
-    D.d.ensure(V * sizeof(float));
-    D.S.ensure(size_t(nsub) * V * sizeof(float));
-    std::vector<float*> y = ptrs(D.y), aux = ptrs(D.aux);
-    std::vector<float*> resp;
-    if (response)
-        resp = ptrs(D.resp);
-    for (size_t i = 0; i < nP; i++) {
-        const size_t nb = size_t(m.plans[i].elems()) * sizeof(float);
-        h2d(y[i], g[i], nb, s);
-        if (response)
-            h2d(resp[i], response[i], nb, s);
-    }
-    h2d(D.f.p, f, V * sizeof(float), s);
-
-    // sensitivity per subset: back(mask or ones, times response), zeros -> 1
-    for (int sub = 0; sub < nsub; sub++) {
-        for (size_t i = 0; i < nP; i++) {
-            const int64_t n = m.plans[i].elems();
-            if (mask)
-                h2d(aux[i], mask[i], n * sizeof(float), s);
-            else
-                launch_fill(aux[i], 1.0f, n, s);
-            if (response)
-                launch_mul(aux[i], resp[i], n, s);
-        }
-        float* Ss = D.S.f() + int64_t(sub) * V;
-        m.back(aux.data(), Ss, m.back_mode, s, sub, nsub);
-        launch_sens_fix(Ss, V, s);
-    }
-
-    double* dll = nullptr;
-    double* ll_scratch = nullptr;
-    int64_t max_el = 0;
-    for (const auto& pl : m.plans)
-        max_el = std::max<int64_t>(max_el, pl.elems());
-    if (loglik) {
-        D.red.ensure(sizeof(double) * (size_t(numIter) + 1 + size_t(ll_scratch_doubles(max_el))));
-        dll = static_cast<double*>(D.red.p);
-        ll_scratch = dll + numIter + 1;
-        ensure_plan_bufs(m, D.g);
-    }
-    std::vector<float*> q = loglik ? ptrs(D.g) : std::vector<float*>();
-
-    float* fc = D.f.f();
-    float* fn = D.f2.f();
-    for (int it = 0; it < numIter; it++) {
-        for (int sub = 0; sub < nsub; sub++) {
-            if (loglik) {
-                m.forward(q.data(), fc, EPI_NONE, nullptr, response ? resp.data() : nullptr, s, sub, nsub);
-                for (size_t i = 0; i < nP; i++) {
-                    const Planogram& pl = m.plans[i];
-                    const int64_t row = int64_t(pl.N_v0) * pl.N_u1 * pl.N_u0;  // one v1 view row
-                    launch_loglik(y[i], q[i], row, sub, nsub, pl.N_v1, ll_scratch, dll + it, !(i == 0 && sub == 0),
-                                  s);
-                    launch_ratio(aux[i], y[i], q[i], response ? resp[i] : nullptr, row, sub, nsub, pl.N_v1, s);
-                }
-            } else {
-                m.forward(aux.data(), fc, EPI_RATIO, y.data(), response ? resp.data() : nullptr, s, sub, nsub);
-            }
-            m.back(aux.data(), D.d.f(), m.back_mode, s, sub, nsub);
-            const float* Ss = D.S.f() + int64_t(sub) * V;
-            if (beta <= 0.0f || prior == 0) {
-                launch_mlem_update(fc, D.d.f(), Ss, V, s);
-            } else {
-                launch_mlem_update_prior(fn, fc, D.d.f(), Ss, m.numZ, m.numY, m.numX, prior == 2 ? 1 : 0, delta,
-                                         beta, p, s);
-                std::swap(fc, fn);
-            }
-        }
-    }
-    d2h(f, fc, V * sizeof(float), s);
-    if (loglik && numIter > 0)
-        d2h(loglik, dll, sizeof(double) * numIter, s);
+    mlem_initial_estimate(f, m.vol_elems());
+    hipStream_t s = m.stream();
+    MlemJob job(m, prm);
+    job.allocate();
+    job.setup(g, f, mask, response, s);
+    for (int it = 0; it < numIter; it++)
+        job.iterate(it, s);
+    job.fetch(f, loglik, s);
     PETL_HIP_CHECK(hipStreamSynchronize(s));
     return true;
 }
@@ -395,8 +283,7 @@ bool get_planogramSize(int which, int* shape)
 bool add_planogram(float psi, float R, float L, float H, float v_m0, float v_m1, float T)
 {
     Planogram p;
-    if (!p.init(float(double(psi) * 3.1415926535897932385 / 180.0), float(0.5 * double(R)), float(0.5 * double(L)),
-                float(0.5 * double(H)), v_m0, v_m1, T))
+    if (!p.init_panels(psi, R, L, H, v_m0, v_m1, T))
         return false;
     M()->plans.push_back(p);
     M()->touch();
@@ -497,7 +384,7 @@ bool project(float** g, float* f)
         const int64_t V = m->vol_elems();
         m->dev.f.ensure(V * sizeof(float));
         ensure_plan_bufs(*m, m->dev.g);
-        std::vector<float*> dg = ptrs(m->dev.g);
+        std::vector<float*> dg = dev_ptrs(m->dev.g);
         h2d(m->dev.f.p, f, V * sizeof(float), s);
         m->forward(dg.data(), m->dev.f.f(), EPI_NONE, nullptr, nullptr, s);
         for (size_t i = 0; i < m->plans.size(); i++)
@@ -521,7 +408,7 @@ bool backproject(float** g, float* f)
         const int64_t V = m->vol_elems();
         m->dev.f.ensure(V * sizeof(float));
         ensure_plan_bufs(*m, m->dev.g);
-        std::vector<float*> dg = ptrs(m->dev.g);
+        std::vector<float*> dg = dev_ptrs(m->dev.g);
         for (size_t i = 0; i < m->plans.size(); i++)
             h2d(dg[i], g[i], size_t(m->plans[i].elems()) * sizeof(float), s);
         m->back(dg.data(), m->dev.f.f(), m->back_mode, s);

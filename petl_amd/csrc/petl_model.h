@@ -18,6 +18,9 @@ struct Planogram {
 
     // src/planogram.cpp:143-186 (psi in radians, half sizes)
     bool init(float psi_in, float R_in, float L_in, float H_in, float v_m0_in, float v_m1_in, float T);
+    // add_planogram's arguments (src/petl_c_interface.cpp:216-219): psi in degrees, full panel separation
+    // D and full panel length/height, converted in double as the reference does
+    bool init_panels(float psi_deg, float D, float L, float H, float v_m0_in, float v_m1_in, float T);
     bool defined() const;
     PlanGeom geom() const;
     int64_t elems() const { return int64_t(N_v1) * N_v0 * N_u1 * N_u0; }

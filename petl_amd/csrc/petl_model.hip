@@ -48,6 +48,12 @@ bool Planogram::init(float psi_in, float R_in, float L_in, float H_in, float v_m
     return true;
 }
 
+bool Planogram::init_panels(float psi_deg, float D, float L, float H, float v_m0_in, float v_m1_in, float T)
+{
+    return init(float(double(psi_deg) * 3.1415926535897932385 / 180.0), float(0.5 * double(D)),
+                float(0.5 * double(L)), float(0.5 * double(H)), v_m0_in, v_m1_in, T);
+}
+
 // src/planogram.cpp:91-97
 bool Planogram::defined() const
 {

@@ -5,6 +5,8 @@
 namespace petl {
 
 void launch_fill(float* x, float v, int64_t n, hipStream_t s);
+// dst = src (a kernel rather than a copy engine call, so it can sit in a captured graph)
+void launch_copy(float* dst, const float* src, int64_t n, hipStream_t s);
 // x *= y
 void launch_mul(float* x, const float* y, int64_t n, hipStream_t s);
 // S == 0 -> 1 (src/petl_server.py:1584)

@@ -17,6 +17,12 @@ __global__ void fill_kernel(float* __restrict__ x, float v, int64_t n)
         x[i] = v;
 }
 
+__global__ void copy_kernel(float* __restrict__ x, const float* __restrict__ y, int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = y[i];
+}
+
 __global__ void mul_kernel(float* __restrict__ x, const float* __restrict__ y, int64_t n)
 {
     for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
@@ -129,6 +135,12 @@ __global__ void __launch_bounds__(VT) loglik_final_kernel(const double* __restri
 void launch_fill(float* x, float v, int64_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(VT), 0, s, x, v, n);
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_copy(float* dst, const float* src, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(VT), 0, s, dst, src, n);
     PETL_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,142 @@
+"""Python interface of the reconstruction service (petl_service_* in libpetl.so).
+
+    svc = ReconService(hbm_budget_bytes=64e9, max_slots=8)
+    rid = svc.submit(psis=(0, 90), D=200, L=128, H=128, T=2, n=64, voxel=2.0, y=data, num_iter=10)
+    image, times = svc.wait(rid)      # times = (queued s, on device s, total s)
+
+A request is one MLEM/OSEM reconstruction of its own geometry and volume.  Requests are admitted while
+others run, as soon as a slot is free and the request's device footprint fits the budget; the image is
+the same as PETLserver.MLEM gives for the same inputs (ones as the initial estimate, no mask/response).
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from .petl_server import load_library
+
+MAX_PLANOGRAMS = 8
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class ReconRequest(ctypes.Structure):
+    _fields_ = [("num_planograms", ctypes.c_int),
+                ("psi", ctypes.c_float * MAX_PLANOGRAMS),
+                ("D", ctypes.c_float), ("L", ctypes.c_float), ("H", ctypes.c_float), ("T", ctypes.c_float),
+                ("v_m0", ctypes.c_float), ("v_m1", ctypes.c_float),
+                ("numX", ctypes.c_int), ("numY", ctypes.c_int), ("numZ", ctypes.c_int),
+                ("voxelWidth", ctypes.c_float), ("voxelHeight", ctypes.c_float),
+                ("y", ctypes.POINTER(_c_float_p)),
+                ("numIter", ctypes.c_int), ("numSubsets", ctypes.c_int), ("prior", ctypes.c_int),
+                ("delta", ctypes.c_float), ("beta", ctypes.c_float), ("p", ctypes.c_float)]
+
+
+_PRIORS = {None: 0, "none": 0, "rd": 1, "tv": 2}
+
+
+def _declare(lib):
+    lib.petl_service_create.restype = ctypes.c_void_p
+    lib.petl_service_create.argtypes = [ctypes.c_double, ctypes.c_int]
+    lib.petl_service_submit.restype = ctypes.c_int
+    lib.petl_service_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(ReconRequest)]
+    lib.petl_service_request_bytes.restype = ctypes.c_double
+    lib.petl_service_request_bytes.argtypes = [ctypes.POINTER(ReconRequest)]
+    lib.petl_service_wait.restype = ctypes.c_bool
+    lib.petl_service_wait.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_float_p, ctypes.POINTER(ctypes.c_double)]
+    lib.petl_service_stats.restype = ctypes.c_bool
+    lib.petl_service_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    lib.petl_service_destroy.restype = None
+    lib.petl_service_destroy.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+def make_request(psis, D, L, H, T, n, voxel, y, num_iter, v_m0=None, v_m1=None, num_subsets=1, prior=None,
+                 delta=2.0, beta=0.0, p=1.0, shape=None):
+    """Builds the C request and the array keeping the data pointers alive.  Defaults of v_m0/v_m1 follow
+    PETLserver.add_planogram (src/petl_server.py:484-507)."""
+    psis = tuple(float(x) for x in psis)
+    if not 1 <= len(psis) <= MAX_PLANOGRAMS:
+        raise ValueError("1 to 8 planograms per request")
+    if v_m0 is None:
+        v_m0 = 1.0
+    if v_m1 is None:
+        v_m1 = min(1.0, (H - 1.5 * T) / D)
+    nz, ny, nx = shape if shape is not None else (n, n, n)
+    if len(y) != len(psis):
+        raise ValueError("one data array per planogram")
+    ys = [np.ascontiguousarray(a, dtype=np.float32) for a in y]
+    ptrs = (_c_float_p * len(ys))(*[a.ctypes.data_as(_c_float_p) for a in ys])
+    r = ReconRequest()
+    r.num_planograms = len(psis)
+    for i, v in enumerate(psis):
+        r.psi[i] = v
+    r.D, r.L, r.H, r.T = D, L, H, T
+    r.v_m0, r.v_m1 = v_m0, v_m1
+    r.numX, r.numY, r.numZ = nx, ny, nz
+    r.voxelWidth, r.voxelHeight = voxel, voxel
+    r.y = ptrs
+    r.numIter, r.numSubsets = int(num_iter), int(num_subsets)
+    r.prior = _PRIORS[prior]
+    r.delta, r.beta, r.p = delta, beta, p
+    return r, (ys, ptrs)
+
+
+class ReconService:
+    def __init__(self, hbm_budget_bytes=0.0, max_slots=8):
+        self.lib = _declare(load_library())
+        self.h = self.lib.petl_service_create(float(hbm_budget_bytes), int(max_slots))
+        if not self.h:
+            raise RuntimeError("petl_service_create failed")
+        self._keep = {}
+        self._shapes = {}
+        self._lock = threading.Lock()
+
+    def request_bytes(self, **kw):
+        r, _ = make_request(**kw)
+        return self.lib.petl_service_request_bytes(ctypes.byref(r))
+
+    def submit(self, **kw):
+        r, keep = make_request(**kw)
+        rid = self.lib.petl_service_submit(self.h, ctypes.byref(r))
+        if rid < 0:
+            raise ValueError("request rejected (invalid geometry or larger than the HBM budget)")
+        with self._lock:
+            self._keep[rid] = keep
+            self._shapes[rid] = (r.numZ, r.numY, r.numX)
+        return rid
+
+    def wait(self, rid):
+        with self._lock:
+            shape = self._shapes[rid]
+        f = np.empty(shape, np.float32)
+        t = (ctypes.c_double * 3)()
+        ok = self.lib.petl_service_wait(self.h, int(rid), f.ctypes.data_as(_c_float_p), t)
+        with self._lock:
+            self._keep.pop(rid, None)
+            self._shapes.pop(rid, None)
+        if not ok:
+            raise RuntimeError(f"request {rid} failed")
+        return f, (t[0], t[1], t[2])
+
+    def stats(self):
+        s = (ctypes.c_double * 6)()
+        self.lib.petl_service_stats(self.h, s)
+        return dict(completed=int(s[0]), failed=int(s[1]), peak_active=int(s[2]), peak_reserved_bytes=s[3],
+                    budget_bytes=s[4], queued=int(s[5]))
+
+    def close(self):
+        if self.h:
+            self.lib.petl_service_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,150 @@
+"""Reconstruction service (petl_amd.serving) on the GPU.
+
+Every request's image must be bit-identical to PETLserver.MLEM on the same inputs run alone: the service
+runs the same kernels, each request on its own buffers, and the kernels have a fixed reduction order (no
+atomics), so neither the mix nor the replay of captured graphs may change a bit.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from petl_amd.petl_server import PETLserver
+from petl_amd.serving import ReconService
+from petl_amd.workloads import ellipsoid_phantom
+
+pytestmark = pytest.mark.gpu
+
+GEOS = {
+    "p32": dict(psis=(0.0,), D=100.0, L=64.0, H=64.0, T=2.0, n=32, voxel=2.0),
+    "p48": dict(psis=(0.0, 90.0), D=200.0, L=96.0, H=96.0, T=2.0, n=48, voxel=2.0),
+    "p64": dict(psis=(0.0, 90.0), D=200.0, L=128.0, H=128.0, T=2.0, n=64, voxel=2.0),
+}
+
+
+def server_for(geo):
+    s = PETLserver()
+    s.print_warnings = False
+    for psi in geo["psis"]:
+        s.add_planogram(psi, geo["D"], geo["L"], geo["H"], geo["T"])
+    s.set_volume(geo["n"], geo["n"], geo["n"], geo["voxel"])
+    return s
+
+
+_data_cache = {}
+
+
+def data_for(name, seed, counts=1e6):
+    key = (name, seed, counts)
+    if key not in _data_cache:
+        geo = GEOS[name]
+        s = server_for(geo)
+        f = ellipsoid_phantom(geo["n"], geo["voxel"], 6, seed=seed)
+        g = s.project(s.allocate_projections(), f)
+        tot = sum(float(x.sum()) for x in g)
+        rng = np.random.default_rng(seed + 100)
+        _data_cache[key] = [rng.poisson(x * (counts / tot)).astype(np.float32) for x in g]
+    return _data_cache[key]
+
+
+def alone(name, y, num_iter, prior=None, beta=0.0, delta=2.0, p=1.0, num_subsets=1):
+    s = server_for(GEOS[name])
+    return s.MLEM(y, None, num_iter, delta=delta, beta=beta, prior=prior or "rd", p=p, num_subsets=num_subsets)
+
+
+def kw(name, y, num_iter, **extra):
+    return dict(GEOS[name], y=y, num_iter=num_iter, **extra)
+
+
+MIX = [
+    ("p32", 1, 10, {}),
+    ("p48", 2, 5, dict(prior="rd", beta=0.01, delta=2.0)),
+    ("p64", 3, 4, dict(num_subsets=4)),
+    ("p32", 4, 6, dict(prior="tv", beta=0.005, delta=0.05, p=1.0)),
+    ("p48", 5, 3, {}),
+    ("p64", 6, 2, {}),
+]
+
+
+def expected(case):
+    name, seed, it, extra = case
+    return alone(name, data_for(name, seed), it, **extra)
+
+
+def test_mix_matches_alone_bit_exact():
+    want = [expected(c) for c in MIX]
+    with ReconService(max_slots=4) as svc:
+        ids = [svc.submit(**kw(n, data_for(n, sd), it, **ex)) for (n, sd, it, ex) in MIX]
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["completed"] == len(MIX) and st["failed"] == 0
+    assert st["peak_active"] >= 2
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 4])
+def test_each_graph_replay_matches_eager(iters):
+    """k replays of the captured iteration equal k eager iterations, for k = 1..4."""
+    y = data_for("p48", 2)
+    want = alone("p48", y, iters, prior="rd", beta=0.01)
+    with ReconService(max_slots=2) as svc:
+        rid = svc.submit(**kw("p48", y, iters, prior="rd", beta=0.01))
+        got, _ = svc.wait(rid)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_hbm_budget_admits_one_at_a_time():
+    cases = MIX[:3]
+    with ReconService(max_slots=8) as probe:
+        sizes = [probe.request_bytes(**kw(n, data_for(n, sd), it, **ex)) for (n, sd, it, ex) in cases]
+    budget = 1.2 * max(sizes)
+    assert budget < sum(sorted(sizes)[-2:])  # no two of the larger ones fit together
+    want = [expected(c) for c in cases]
+    with ReconService(hbm_budget_bytes=budget, max_slots=8) as svc:
+        ids = [svc.submit(**kw(n, data_for(n, sd), it, **ex)) for (n, sd, it, ex) in cases]
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["peak_reserved_bytes"] <= budget
+
+
+def test_requests_join_while_others_run():
+    """A long request is running when short ones arrive; they are admitted beside it and finish first."""
+    y_long = data_for("p64", 7)
+    y_short = data_for("p32", 8)
+    want_long = alone("p64", y_long, 30)
+    want_short = alone("p32", y_short, 3)
+    with ReconService(max_slots=4) as svc:
+        t0 = time.perf_counter()
+        rid_long = svc.submit(**kw("p64", y_long, 30))
+        done = {}
+
+        def wait_long():
+            done["long"] = svc.wait(rid_long)
+            done["t_long"] = time.perf_counter() - t0
+
+        th = threading.Thread(target=wait_long)
+        th.start()
+        time.sleep(0.05)
+        short_ids = [svc.submit(**kw("p32", y_short, 3)) for _ in range(3)]
+        shorts = [svc.wait(i) for i in short_ids]
+        t_short = time.perf_counter() - t0
+        th.join()
+        st = svc.stats()
+    np.testing.assert_array_equal(done["long"][0], want_long)
+    for img, _ in shorts:
+        np.testing.assert_array_equal(img, want_short)
+    assert st["peak_active"] >= 2
+    assert t_short < done["t_long"]
+
+
+def test_oversized_request_is_rejected():
+    y = data_for("p32", 1)
+    with ReconService(max_slots=2) as probe:
+        need = probe.request_bytes(**kw("p32", y, 2))
+    with ReconService(hbm_budget_bytes=0.5 * need, max_slots=2) as svc:
+        with pytest.raises(ValueError):
+            svc.submit(**kw("p32", y, 2))
