@@ -11,6 +11,9 @@ Prints one JSON line.  Besides the headline rate it carries:
                 launches on the stream they run on; algorithmic bytes per launch = 4 * (voxels + bins)
   cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference projectors) timed on a bounded
                 sample of the same workload on this host's cores
+  serving       reconstructions served per second and p50/p99 latency for a Poisson arrival stream of
+                MLEM requests of mixed panel/volume sizes, through the continuous-batching service and,
+                for comparison, one request at a time
 """
 import argparse
 import json
@@ -36,6 +39,10 @@ def parse():
     ap.add_argument("--mode", default="petl", choices=["petl", "matched"])
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (req/s)")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=8)
     return ap.parse_args()
 
 
@@ -62,6 +69,88 @@ def cpu_baseline(rows, f):
     desc = (f"fp32 oracle forward + PETL-mode back over v1 rows {b0}..{b0 + rows - 1} of both planograms "
             f"({rows}/{nv1} of the views, central rows), {dt:.1f} s measured, scaled by {nv1}/{rows}")
     return 1.0 / per_pair, nt, desc
+
+
+SERVE_SIZES = ("panel48", "panel64", "panel96", "panel128")
+
+
+def serving_datasets(torch, dev, seed):
+    """Two Poisson datasets per panel size (counts drawn log-uniform in [1e5, 1e8]), in page-locked host
+    memory so uploads run at full PCIe rate.  Returns {size: [(counts, [planogram arrays])]}."""
+    from petl_amd.petl_server import PETLserver
+    from petl_amd.workloads import GEOMETRIES, ellipsoid_phantom, setup_server
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    out = {}
+    for k, name in enumerate(SERVE_SIZES):
+        geo = GEOMETRIES[name]
+        s = PETLserver()
+        s.print_warnings = False
+        setup_server(s, name)
+        shapes = [tuple(int(x) for x in s.get_planogramSize(i)) for i in range(s.get_numPlanograms())]
+        f = torch.from_numpy(ellipsoid_phantom(geo["n"], geo["voxel"], 20, seed=seed + k)).to(dev)
+        q = [torch.empty(sh, dtype=torch.float32, device=dev) for sh in shapes]
+        s.project_gpu(q, f)
+        tot = float(sum(float(x.double().sum()) for x in q))
+        sets = []
+        for _ in range(2):
+            counts = float(10.0 ** rng.uniform(5.0, 8.0))
+            ys = []
+            for x in q:
+                host = torch.empty(x.shape, dtype=torch.float32, pin_memory=True)
+                host.copy_(torch.poisson(x * (counts / tot), generator=gen))
+                ys.append(host)
+            sets.append((counts, ys))
+        out[name] = sets
+        del q, f
+    torch.cuda.synchronize()
+    return out
+
+
+def serving_run(datasets, lam, nreq, slots, seed):
+    """Poisson arrivals at rate lam; each request = (panel size, dataset, iterations U{5..20}).  Returns
+    recons/s over the run and latency percentiles (submission to completion)."""
+    import threading
+    from petl_amd.serving import ReconService
+    from petl_amd.workloads import GEOMETRIES
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for _ in range(nreq):
+        t += float(rng.exponential(1.0 / lam))
+        name = SERVE_SIZES[int(rng.integers(len(SERVE_SIZES)))]
+        which = int(rng.integers(2))
+        reqs.append((t, name, which, int(rng.integers(5, 21))))
+    lat = [None] * nreq
+    with ReconService(max_slots=slots) as svc:
+        ids = [None] * nreq
+        ready = [threading.Event() for _ in range(nreq)]
+
+        def waiter():
+            for i in range(nreq):
+                ready[i].wait()
+                _, times = svc.wait(ids[i])
+                lat[i] = times[2]
+
+        th = threading.Thread(target=waiter)
+        t0 = time.perf_counter()
+        th.start()
+        for i, (ta, name, which, iters) in enumerate(reqs):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            geo = GEOMETRIES[name]
+            ys = [y.numpy() for y in datasets[name][which][1]]
+            ids[i] = svc.submit(psis=geo["psis"], D=geo["D"], L=geo["L"], H=geo["H"], T=geo["T"], n=geo["n"],
+                                voxel=geo["voxel"], y=ys, num_iter=iters)
+            ready[i].set()
+        th.join()
+        elapsed = time.perf_counter() - t0
+        st = svc.stats()
+    lat = np.array(lat)
+    return {"recons_per_s": nreq / elapsed, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "peak_on_device": st["peak_active"],
+            "failed": st["failed"]}
 
 
 def main():
@@ -161,6 +250,49 @@ def main():
             "alg_bytes_per_launch": alg_bytes,
         },
     }
+    # the same pair with the volume and planograms stored in bfloat16 / binary16 (fp32 arithmetic, fp32
+    # back-projected volume)
+    storage_forms = {}
+    for name, dt in (("bf16", torch.bfloat16), ("f16", torch.float16)):
+        f16 = f.to(dt)
+        g16 = [torch.empty(sh, dtype=dt, device=dev) for sh in shapes]
+        for _ in range(a.warmup):
+            s.project_gpu(g16, f16, stream=stream, sync=False)
+            s.backproject_gpu(g16, fb, stream=stream, sync=False)
+        ev16 = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for k in range(a.steps):
+            ev16[k][0].record()
+            s.project_gpu(g16, f16, stream=stream, sync=False)
+            ev16[k][1].record()
+            s.backproject_gpu(g16, fb, stream=stream, sync=False)
+            ev16[k][2].record()
+        torch.cuda.synchronize()
+        el16 = time.perf_counter() - t1
+        storage_forms[name] = {
+            "pairs_per_s": a.steps / el16,
+            "forward_call_ms": float(np.mean([e[0].elapsed_time(e[1]) for e in ev16])),
+            "back_call_ms": float(np.mean([e[1].elapsed_time(e[2]) for e in ev16])),
+        }
+        del g16, f16
+    out["storage_forms"] = storage_forms
+
+    if not a.no_serve:
+        del g, f, fb
+        torch.cuda.empty_cache()
+        data = serving_datasets(torch, dev, seed=500)
+        mixed = serving_run(data, a.serve_rate, a.serve_requests, a.serve_slots, seed=501)
+        single = serving_run(data, a.serve_rate, a.serve_requests, 1, seed=501)
+        out["serving"] = {
+            "workload": "Poisson arrivals; panels uniform over 48/64/96/128 crystals (2 orientations, D = 200 mm, "
+                        "matched n^3 volumes at 2 mm); data = Poisson at counts log-uniform in [1e5, 1e8]; "
+                        "MLEM iterations uniform in 5..20; fp32",
+            "arrival_rate_req_s": a.serve_rate,
+            "requests": a.serve_requests,
+            "continuous_batching": dict(mixed, slots=a.serve_slots),
+            "one_at_a_time": dict(single, slots=1),
+        }
     if not a.no_cpu:
         v, cores, desc = cpu_baseline(a.cpu_rows, f_host)
         out["cpu_baseline"] = {"value": v, "unit": "fwd+back projection pairs/s", "cores": cores, "kind": "port",

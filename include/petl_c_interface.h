@@ -138,6 +138,19 @@ PETL_EXTERN int planogram_uses_fast_path(int which);
 PETL_EXTERN bool project_gpu(float** d_g, float* d_f, void* stream, bool sync);
 PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync);
 
+/* Storage formats of device volumes / planograms for the *_ex and *_storage calls: 0 = fp32, 1 = bfloat16,
+ * 2 = IEEE binary16 with a per-tensor scale (value = stored * scale).  Arithmetic is fp32 throughout;
+ * 16-bit formats run on the fast kernels only (psi 0 or 90 degrees, voxel pitch = crystal pitch). */
+#define PETL_STORE_F32 0
+#define PETL_STORE_BF16 1
+#define PETL_STORE_F16 2
+/* project_gpu with the volume d_f in `volume_storage` and the planograms d_g in `planogram_storage` (equal, or
+ * an fp32 volume with 16-bit planograms); f_scale, g_scale: binary16 scales, ignored otherwise. */
+PETL_EXTERN bool project_gpu_ex(void** d_g, const void* d_f, int volume_storage, int planogram_storage,
+                                float f_scale, float g_scale, void* stream, bool sync);
+/* backproject_gpu with the planograms in `storage`; the volume d_f is fp32 (overwritten). */
+PETL_EXTERN bool backproject_gpu_ex(void** d_g, float* d_f, int storage, float g_scale, void* stream, bool sync);
+
 /* Device-resident MLEM / OSEM (the loop of src/petl_server.py:1534-1606 with every step on the GPU).
  * g: host planograms (data); f: host volume, used as the initial estimate (all zeros -> ones, negatives
  * clipped) and overwritten with the result.  prior: 0 none, 1 relative differences (OSL, as the
@@ -148,6 +161,12 @@ PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool syn
  * projected from the estimate in force when that subset is processed). */
 PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
                           int numSubsets, float** mask, float** response, double* loglik);
+/* MLEM_gpu with the planograms (data and ratio) held in `planogram_storage` and the image in
+ * `image_storage` (equal, or an fp32 image) on the device; the sensitivities and back projections stay
+ * fp32.  binary16 scales are chosen per tensor (data: from its largest count; image: from
+ * sum(data) / sum(sensitivity)).  No mask, response or log-likelihood. */
+PETL_EXTERN bool MLEM_gpu_storage(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
+                                  int numSubsets, int planogram_storage, int image_storage);
 
 /* ------------------------------------------------------------------------------------------------
  * Part 3: reconstruction service (no counterpart in the reference).  Continuous batching of MLEM/OSEM

@@ -4,6 +4,7 @@
 #pragma once
 #include <vector>
 #include "petl_model.h"
+#include "storage.h"
 
 namespace petl {
 
@@ -14,6 +15,11 @@ struct MlemParams {
     int nsub = 1;   // OSEM subsets over interleaved v1 rows (clamped to [1, min N_v1])
     bool loglik = false;
     bool mask = false, response = false;
+    // storage of the planograms (data and ratio) and of the image (STORE_*): equal, or an fp32 image with
+    // 16-bit planograms.  16-bit formats need the fast kernels and take no mask, response or
+    // log-likelihood.  Sensitivities and back projections stay fp32.
+    int storage = STORE_F32;
+    int image_storage = STORE_F32;
 };
 
 // Initial estimate rules of src/petl_server.py:1571-1577 on a host volume: all zeros -> ones,
@@ -38,9 +44,12 @@ public:
     const MlemParams& params() const { return prm_; }
 
 private:
+    void setup_storage(float* const* g, const float* f0, hipStream_t s);
+    void iterate_storage(hipStream_t s);
     Model& m_;
     MlemParams prm_;
     int64_t V_ = 0;
+    float s_y_ = 1.0f, s_f_ = 1.0f;  // binary16 scales of the data and of the image
     double* dll_ = nullptr;
     double* ll_scratch_ = nullptr;
 };

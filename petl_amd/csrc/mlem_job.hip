@@ -1,5 +1,6 @@
 // Device-resident MLEM / OSEM (src/petl_server.py:1534-1606, 1257-1273).
 #include <algorithm>
+#include <cmath>
 #include "mlem_job.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
@@ -59,6 +60,12 @@ MlemJob::MlemJob(Model& m, const MlemParams& prm) : m_(m), prm_(prm)
 {
     prm_.nsub = clamp_subsets(m, prm.nsub);
     V_ = m.vol_elems();
+    const bool st16 = prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32;
+    if (st16 && (prm_.mask || prm_.response || prm_.loglik))
+        throw hip_error("16-bit storage takes no mask, response or log-likelihood");
+    if (prm_.storage < STORE_F32 || prm_.storage > STORE_F16 ||
+        !(prm_.image_storage == prm_.storage || prm_.image_storage == STORE_F32))
+        throw hip_error("unsupported storage formats (planograms and image equal, or an fp32 image)");
 }
 
 size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
@@ -80,8 +87,11 @@ size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
         }
     }
     const size_t vol = size_t(m.vol_elems()) * sizeof(float);
+    const size_t es = storage_bytes(prm.storage), ei = storage_bytes(prm.image_storage);
     size_t plan_bufs = 2 + (prm.response ? 1 : 0) + (prm.loglik ? 1 : 0);  // y, aux (+resp) (+q)
-    return G * sizeof(float) * plan_bufs + vol * (4 + size_t(nsub)) + back_ws + (size_t(1) << 20);
+    // image and its update buffer in storage format; back projection and sensitivities fp32
+    return G * es * plan_bufs + size_t(m.vol_elems()) * ei * 2 + vol * (2 + size_t(nsub)) + back_ws +
+           (size_t(1) << 20);
 }
 
 void MlemJob::allocate()
@@ -89,6 +99,12 @@ void MlemJob::allocate()
     Model& m = m_;
     DeviceState& D = m.dev;
     m.prepare();
+    if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32)
+        for (size_t i = 0; i < m.plans.size(); i++)
+            if (!m.uses_fast(int(i)))
+                throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal "
+                                "pitch)");
+    // planogram and image buffers hold storage-format elements; sized for fp32 they fit any format
     ensure_plan_bufs(m, D.y);
     ensure_plan_bufs(m, D.aux);
     if (prm_.response)
@@ -110,6 +126,10 @@ void MlemJob::allocate()
 
 void MlemJob::setup(float* const* g, const float* f0, float* const* mask, float* const* response, hipStream_t s)
 {
+    if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32) {
+        setup_storage(g, f0, s);
+        return;
+    }
     Model& m = m_;
     DeviceState& D = m.dev;
     const size_t nP = m.plans.size();
@@ -141,8 +161,68 @@ void MlemJob::setup(float* const* g, const float* f0, float* const* mask, float*
     }
 }
 
+// 16-bit storage: data converted on the host (binary16: per-tensor power-of-two scale from the largest
+// count), sensitivities as in fp32, then the initial image stored (binary16: with a scale that places
+// 1024 x the expected mean image value sum(y) / sum(S), and the initial values, inside its range).
+void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
+{
+    Model& m = m_;
+    DeviceState& D = m.dev;
+    const int st = prm_.storage;
+    const size_t nP = m.plans.size();
+    std::vector<float*> aux = dev_ptrs(D.aux);
+    float ymax = 0.0f;
+    double ysum = 0.0;
+    for (size_t i = 0; i < nP; i++) {
+        const int64_t n = m.plans[i].elems();
+        for (int64_t k = 0; k < n; k++) {
+            ymax = std::max(ymax, std::fabs(g[i][k]));
+            ysum += g[i][k];
+        }
+    }
+    s_y_ = (st == STORE_F16) ? pow2_scale_for(ymax, 16384.0f) : 1.0f;
+    std::vector<char> tmp;
+    for (size_t i = 0; i < nP; i++) {
+        const int64_t n = m.plans[i].elems();
+        tmp.resize(size_t(n) * storage_bytes(st));
+        to_storage(st, g[i], tmp.data(), n, 1.0f / s_y_);
+        h2d(D.y[i].p, tmp.data(), tmp.size(), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));  // tmp is reused
+    }
+    for (int sub = 0; sub < prm_.nsub; sub++) {
+        for (size_t i = 0; i < nP; i++)
+            launch_fill_st(st, aux[i], 1.0f, 1.0f, m.plans[i].elems(), s);
+        float* Ss = D.S.f() + int64_t(sub) * V_;
+        m.back_st(st, reinterpret_cast<void* const*>(aux.data()), Ss, m.back_mode, s, sub, prm_.nsub, 1.0f);
+        launch_sens_fix(Ss, V_, s);
+    }
+    const int sti = prm_.image_storage;
+    float f0max = 0.0f;
+    for (int64_t k = 0; k < V_; k++)
+        f0max = std::max(f0max, std::fabs(f0[k]));
+    s_f_ = 1.0f;
+    if (sti == STORE_F16) {
+        std::vector<float> S(V_);
+        d2h(S.data(), D.S.f(), size_t(V_) * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        double ssum = 0.0;
+        for (float v : S)
+            ssum += v;
+        const double mean = ssum > 0.0 ? ysum / ssum : 1.0;
+        s_f_ = pow2_scale_for(std::max(float(1024.0 * mean), f0max), 32768.0f);
+    }
+    tmp.resize(size_t(V_) * storage_bytes(sti));
+    to_storage(sti, f0, tmp.data(), V_, 1.0f / s_f_);
+    h2d(D.f.p, tmp.data(), tmp.size(), s);
+    PETL_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void MlemJob::iterate(int it, hipStream_t s)
 {
+    if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32) {
+        iterate_storage(s);
+        return;
+    }
     Model& m = m_;
     DeviceState& D = m.dev;
     const size_t nP = m.plans.size();
@@ -178,8 +258,39 @@ void MlemJob::iterate(int it, hipStream_t s)
     }
 }
 
+void MlemJob::iterate_storage(hipStream_t s)
+{
+    Model& m = m_;
+    DeviceState& D = m.dev;
+    const int st = prm_.storage, sti = prm_.image_storage;
+    const int nsub = prm_.nsub;
+    std::vector<float*> y = dev_ptrs(D.y), aux = dev_ptrs(D.aux);
+    void* const* yv = reinterpret_cast<void* const*>(y.data());
+    void* const* av = reinterpret_cast<void* const*>(aux.data());
+    for (int sub = 0; sub < nsub; sub++) {
+        // ratio y / Af stored with scale 1 (it is near 1 where the model fits the data)
+        m.forward_st(sti, st, av, D.f.p, EPI_RATIO, yv, nullptr, s, sub, nsub, s_f_, s_y_, 1.0f);
+        m.back_st(st, av, D.d.f(), m.back_mode, s, sub, nsub, 1.0f);
+        const float* Ss = D.S.f() + int64_t(sub) * V_;
+        if (prm_.beta <= 0.0f || prm_.prior == 0) {
+            launch_mlem_update_st(sti, D.f.p, D.d.f(), Ss, V_, s_f_, s);
+        } else {
+            launch_mlem_update_prior_st(sti, D.f2.p, D.f.p, D.d.f(), Ss, m.numZ, m.numY, m.numX,
+                                        prm_.prior == 2 ? 1 : 0, prm_.delta, prm_.beta, prm_.p, s_f_, s);
+            launch_copy_st(sti, D.f.p, D.f2.p, V_, s);
+        }
+    }
+}
+
 void MlemJob::fetch(float* f, double* loglik, hipStream_t s)
 {
+    if (prm_.image_storage != STORE_F32) {
+        std::vector<char> tmp(size_t(V_) * storage_bytes(prm_.image_storage));
+        d2h(tmp.data(), m_.dev.f.p, tmp.size(), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        from_storage(prm_.image_storage, tmp.data(), f, V_, s_f_);
+        return;
+    }
     d2h(f, m_.dev.f.p, V_ * sizeof(float), s);
     if (prm_.loglik && loglik && prm_.numIter > 0)
         d2h(loglik, dll_, sizeof(double) * prm_.numIter, s);

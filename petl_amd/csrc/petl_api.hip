@@ -127,7 +127,8 @@ float inner3(const float* x, const float* y, const float* w, int N1, int N2, int
 
 // ---- device-resident MLEM / OSEM (src/petl_server.py:1534-1606, 1257-1273) ----
 bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float beta, int prior, float p, int nsub,
-              float** mask, float** response, double* loglik)
+              float** mask, float** response, double* loglik, int storage = STORE_F32,
+              int image_storage = STORE_F32)
 {
     if (!g || !f || !m.allDefined() || numIter < 0)
         return false;
@@ -144,6 +145,8 @@ bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float bet
     prm.loglik = loglik != nullptr;
     prm.mask = mask != nullptr;
     prm.response = response != nullptr;
+    prm.storage = storage;
+    prm.image_storage = image_storage;
     // initial estimate (src/petl_server.py:1571-1577), in place on the caller's array
     mlem_initial_estimate(f, m.vol_elems());
     hipStream_t s = m.stream();
@@ -633,6 +636,46 @@ bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync)
         if (sync)
             PETL_HIP_CHECK(hipStreamSynchronize(s));
         return true;
+    });
+}
+
+bool project_gpu_ex(void** d_g, const void* d_f, int volume_storage, int planogram_storage, float f_scale,
+                    float g_scale, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || !m->allDefined())
+        return false;
+    return guarded("project_gpu_ex", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        m->forward_st(volume_storage, planogram_storage, d_g, d_f, EPI_NONE, nullptr, nullptr, s, 0, 1, f_scale, 1.0f,
+                      g_scale);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool backproject_gpu_ex(void** d_g, float* d_f, int storage, float g_scale, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || !m->allDefined() || storage < STORE_F32 || storage > STORE_F16)
+        return false;
+    return guarded("backproject_gpu_ex", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        m->back_st(storage, d_g, d_f, m->back_mode, s, 0, 1, g_scale);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool MLEM_gpu_storage(float** g, float* f, int numIter, float delta, float beta, int prior, float p, int numSubsets,
+                      int planogram_storage, int image_storage)
+{
+    Model* m = M();
+    return guarded("MLEM_gpu_storage", [&] {
+        return run_mlem(*m, g, f, numIter, delta, beta, prior, p, numSubsets, nullptr, nullptr, nullptr,
+                        planogram_storage, image_storage);
     });
 }
 

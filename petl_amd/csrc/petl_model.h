@@ -78,6 +78,12 @@ struct Model {
                  int subset = 0, int nsubsets = 1);
     // f = sum_i A_i^T g_i (overwrites f)
     void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
+    // The same with the volume in storage format sf and the planograms in sg (STORE_*; fast kernels only,
+    // the generic kernels are fp32).  For STORE_F16, x = stored * scale with the scales given; back writes
+    // an fp32 volume.
+    void forward_st(int sf, int sg, void* const* g, const void* f, int epilogue, void* const* y, float* const* resp,
+                    hipStream_t s, int subset, int nsubsets, float f_scale, float y_scale, float g_scale);
+    void back_st(int st, void* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets, float g_scale);
 };
 
 Model* current_model();

@@ -223,4 +223,38 @@ void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset,
     }
 }
 
+void Model::forward_st(int sf, int sg, void* const* g, const void* f, int epilogue, void* const* y, float* const* resp,
+                       hipStream_t s, int subset, int nsubsets, float f_scale, float y_scale, float g_scale)
+{
+    prepare();
+    for (size_t i = 0; i < plans.size(); i++) {
+        if (!dev.fast[i] && (sf != STORE_F32 || sg != STORE_F32))
+            throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
+        if (!dev.fast[i]) {
+            launch_fwd_generic(static_cast<float*>(g[i]), static_cast<const float*>(f), plans[i].geom(), vgeom(),
+                               epilogue, y ? static_cast<const float*>(y[i]) : nullptr, resp ? resp[i] : nullptr, s,
+                               subset, nsubsets);
+            continue;
+        }
+        fast_forward_st(*dev.fast[i], sf, sg, g[i], f, epilogue, y ? y[i] : nullptr, resp ? resp[i] : nullptr, s, subset,
+                        nsubsets, f_scale, y_scale, g_scale);
+    }
+}
+
+void Model::back_st(int st, void* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets, float g_scale)
+{
+    prepare();
+    launch_fill(f, 0.0f, vol_elems(), s);
+    for (size_t i = 0; i < plans.size(); i++) {
+        if (!dev.fast[i] && st != STORE_F32)
+            throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
+        if (!dev.fast[i]) {
+            launch_back_generic(static_cast<const float*>(g[i]), f, plans[i].geom(), vgeom(), mode, s, subset,
+                                nsubsets);
+            continue;
+        }
+        fast_back_st(*dev.fast[i], st, g[i], f, mode, dev.h2.f(), s, subset, nsubsets, g_scale);
+    }
+}
+
 }  // namespace petl

@@ -36,9 +36,10 @@ __device__ __forceinline__ float dhuber(float x, const HuberParams& h)
 
 // Gradient at (i,j,k) of an [N1][N2][N3] array.  kind 0: relative differences (with the f < 1e-8 clip of
 // src/relative_differences.cu:107-115 applied to every value read); kind 1: p-Huber TV.
-template <int KIND>
-__device__ __forceinline__ float prior_grad_at(const float* __restrict__ f, int i, int j, int k, int N1, int N2,
-                                               int N3, float delta, float beta, const HuberParams& h)
+// `ld(index)` returns the value at a linear index (lets the image sit in any storage format).
+template <int KIND, typename Load>
+__device__ __forceinline__ float prior_grad_at_ld(const Load& ld, int i, int j, int k, int N1, int N2, int N3,
+                                                  float delta, float beta, const HuberParams& h)
 {
 #pragma clang fp contract(off)
     const int im = max(0, i - 1), ip = min(N1 - 1, i + 1);
@@ -46,7 +47,7 @@ __device__ __forceinline__ float prior_grad_at(const float* __restrict__ f, int 
     const int km = max(0, k - 1), kp = min(N3 - 1, k + 1);
     const float d1 = 1.0f * beta, d2 = 0.7071067811865475f * beta, d3 = 0.5773502691896258f * beta;
     auto F = [&](int a, int b, int c) {
-        const float v = f[(int64_t(a) * N2 + b) * N3 + c];
+        const float v = ld((int64_t(a) * N2 + b) * N3 + c);
         return KIND == 0 ? rd_clip(v) : v;
     };
     const float c0 = F(i, j, k);
@@ -63,6 +64,13 @@ __device__ __forceinline__ float prior_grad_at(const float* __restrict__ f, int 
     const float s3 = D(ip, jp, kp) + D(ip, jp, km) + D(ip, jm, kp) + D(ip, jm, km) + D(im, jp, kp) +
                      D(im, jp, km) + D(im, jm, kp) + D(im, jm, km);
     return s1 * d1 + s2 * d2 + s3 * d3;
+}
+
+template <int KIND>
+__device__ __forceinline__ float prior_grad_at(const float* __restrict__ f, int i, int j, int k, int N1, int N2,
+                                               int N3, float delta, float beta, const HuberParams& h)
+{
+    return prior_grad_at_ld<KIND>([f](int64_t x) { return f[x]; }, i, j, k, N1, N2, N3, delta, beta, h);
 }
 
 // TV_gradient's parameter rules (src/total_variation.cu:574-594): beta <= 0 -> 1, delta >= 1e-8,

@@ -48,10 +48,19 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V);
 // Views are restricted to v1 rows first, first+stride, ... (OSEM subsets); (0,1) = all.
 void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
                   hipStream_t s, int v1_first = 0, int v1_stride = 1);
+// The same with the volume in storage format sf and the planogram and data y in sg (STORE_*; pairs: equal,
+// or an fp32 volume with 16-bit planograms): x = stored * scale for STORE_F16 (f_scale, y_scale for the
+// inputs, g_scale for the output); scales are ignored otherwise.
+void fast_forward_st(const FastPlan& fp, int sf, int sg, void* g, const void* f, int epilogue, const void* y,
+                     const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale, float y_scale,
+                     float g_scale);
 // Bytes of scratch fast_back needs (the intermediate u0-gathered slabs).
 size_t fast_back_workspace_bytes(const FastPlan& fp);
 // f += A^T g (mode BACK_PETL or BACK_MATCHED), one planogram.
 void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, hipStream_t s,
                int v1_first = 0, int v1_stride = 1);
+// The same with the planogram in storage format st (g_scale as above); f stays fp32.
+void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode, float* workspace, hipStream_t s,
+                  int v1_first, int v1_stride, float g_scale);
 
 }  // namespace petl

@@ -31,6 +31,7 @@
 #include "petl_common.h"
 #include "proj_kernels.h"
 #include "proj_fast.h"
+#include "storage.h"
 
 namespace petl {
 
@@ -70,6 +71,7 @@ struct FwdArgs {
     int psi90;
     int v1_first, v1_stride;
     int xrow;  // padded row length of the u0 table
+    float f_scale, y_scale, g_inv_scale;  // storage scales (16-bit formats; 1 for fp32)
 };
 
 // =============================================================================================
@@ -77,14 +79,17 @@ struct FwdArgs {
 // =============================================================================================
 // z interpolation of R u1 rows x SB slices, software-pipelined: the global loads of block b+1 are issued
 // before the block-b accumulation and written to LDS after it.  LDS layout per buffer: [R][SB][FWD_HP].
-template <int MAXE>
+// The loaded volume values stay in their storage format until zstore, so the loads remain in flight.
+template <int MAXE, int ST>
 struct ZLoads {
-    float a[MAXE], b[MAXE], wa[MAXE], wb[MAXE];
+    typename Stor<ST>::T a[MAXE], b[MAXE];
+    float wa[MAXE], wb[MAXE];
     int dst[MAXE];
 };
 
-template <int SB, int R, int MAXE>
-__device__ __forceinline__ void zload(ZLoads<MAXE>& L, const FwdArgs& A, const float* __restrict__ f,
+template <int SB, int R, int MAXE, int ST>
+__device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
+                                      const typename Stor<ST>::T* __restrict__ f,
                                       const int4* __restrict__ ztab, int iv1, int iu1_0, int nrows, int is0,
                                       int tid)
 {
@@ -113,7 +118,7 @@ __device__ __forceinline__ void zload(ZLoads<MAXE>& L, const FwdArgs& A, const f
         const int4 z = ztab[int64_t(iv1) * A.Ns + isc];
         const int iza = iu1_0 + r - z.x - 1;
         const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
-        const float* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
+        const typename Stor<ST>::T* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
         L.a[e] = fp[int64_t(za) * A.fs_z];
         L.b[e] = fp[int64_t(zb) * A.fs_z];
         L.wa[e] = (live && iza >= 0 && iza < A.Nz) ? __int_as_float(z.y) : 0.0f;
@@ -121,25 +126,28 @@ __device__ __forceinline__ void zload(ZLoads<MAXE>& L, const FwdArgs& A, const f
     }
 }
 
-template <int MAXE>
-__device__ __forceinline__ void zstore(const ZLoads<MAXE>& L, float* hb)
+template <int MAXE, int ST>
+__device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, float* hb, float f_scale)
 {
 #pragma unroll
     for (int e = 0; e < MAXE; e++)
         if (L.dst[e] >= 0)
-            hb[L.dst[e]] = L.wa[e] * L.a[e] + L.wb[e] * L.b[e];
+            hb[L.dst[e]] = L.wa[e] * Stor<ST>::ld(L.a[e], f_scale) + L.wb[e] * Stor<ST>::ld(L.b[e], f_scale);
 }
 
-// J views per wave, C output chunks of 64 bins, SB-slice LDS blocks, R u1 rows per workgroup.
-template <int J, int C, int SB, int R>
-__global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict__ g, const float* __restrict__ f,
+// J views per wave, C output chunks of 64 bins, SB-slice LDS blocks, R u1 rows per workgroup, storage SF of
+// the volume and SG of the planogram and the data y.
+template <int J, int C, int SB, int R, int SF, int SG>
+__global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
+                                                               const typename Stor<SF>::T* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ xtab,
                                                                const float* __restrict__ xwp,
                                                                const int4* __restrict__ ztab,
                                                                const float* __restrict__ fscale, int epilogue,
-                                                               const float* __restrict__ yv,
+                                                               const typename Stor<SG>::T* __restrict__ yv,
                                                                const float* __restrict__ resp)
 {
+    using TS = typename Stor<SG>::T;
     extern __shared__ float hbuf[];
     __shared__ int s_lo, s_hi;
     const int tid = threadIdx.x;
@@ -183,9 +191,9 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
-                    float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                    TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
                     for (int p = lane; p < Nu0; p += 64)
-                        gr[p] = 0.0f;
+                        gr[p] = Stor<SG>::st(0.0f, 1.0f);
                 }
             }
         }
@@ -237,9 +245,9 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
         const int is_first = (is_lo / SB) * SB;
         const int nblk = (is_hi - is_first) / SB + 1;
         constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
-        ZLoads<MAXE> L;
+        ZLoads<MAXE, SF> L;
         zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first, tid);
-        zstore(L, hbuf + A.hpad);
+        zstore(L, hbuf + A.hpad, A.f_scale);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
             const bool more = blk + 1 < nblk;
@@ -307,7 +315,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
                 }
             }
             if (more)
-                zstore(L, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad);
+                zstore(L, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad, A.f_scale);
             __syncthreads();
         }
     }
@@ -323,7 +331,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
             if (iv0 < Nv0) {
                 const float sc = rmask[r] ? 0.0f : fscale[int64_t(iv1) * Nv0 + iv0];
                 const float v0 = coordf(iv0, P.Tv0, P.v0_0);
-                float* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     float wn = __shfl_down(W[r][j][c], 1);
@@ -340,11 +348,11 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(float* __restrict
                         if (resp)
                             out = out * resp[gi];
                         if (epilogue == EPI_RATIO) {
-                            out = (out == 0.0f) ? 0.0f : yv[gi] / out;
+                            out = (out == 0.0f) ? 0.0f : Stor<SG>::ld(yv[gi], A.y_scale) / out;
                             if (resp)
                                 out = out * resp[gi];
                         }
-                        gr[p] = out;
+                        gr[p] = Stor<SG>::st(out, A.g_inv_scale);
                     }
                 }
             }
@@ -365,10 +373,12 @@ struct BackArgs {
     int psi90;
     int v1_first, v1_stride;
     int sb;  // slice block of the forward table layout
+    float g_scale;  // storage scale of the planogram (16-bit formats; 1 for fp32)
 };
 
-template <int AG>
-__global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ g, float* __restrict__ h2, BackArgs A,
+template <int AG, int ST>
+__global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
+                                                      float* __restrict__ h2, BackArgs A,
                                                       const int2* __restrict__ xtab, int xrow,
                                                       const float* __restrict__ scale,
                                                       const int2* __restrict__ rows)
@@ -403,7 +413,8 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const float* __restrict__ 
             const int iv0 = blk * BACK_RB + r;
             float val = 0.0f;
             if (iv0 < Nv0) {
-                val = g[((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0 + p] * scale[int64_t(iv1) * Nv0 + iv0];
+                val = Stor<ST>::ld(g[((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0 + p], A.g_scale) *
+                      scale[int64_t(iv1) * Nv0 + iv0];
                 if (A.mode == BACK_MATCHED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
                     val = 0.0f;
             }
@@ -537,35 +548,26 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
 // ---------------------------------------------------------------------------------------------
 // host-side kernel selection
 // ---------------------------------------------------------------------------------------------
-typedef void (*fwd_fn)(float*, const float*, FwdArgs, const int2*, const float*, const int4*, const float*, int,
-                       const float*, const float*);
+template <int SF, int SG>
+using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdArgs, const int2*, const float*,
+                        const int4*, const float*, int, const typename Stor<SG>::T*, const float*);
 
-template <int J, int SB, int R>
-fwd_fn pick_fwd_c(int C)
+// 7 views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
+// two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower)
+template <int SF, int SG>
+fwd_fn<SF, SG> pick_fwd(int C)
 {
     switch (C) {
-    case 1: return fwd_fast_kernel<J, 1, SB, R>;
-    case 2: return fwd_fast_kernel<J, 2, SB, R>;
-    case 3: return fwd_fast_kernel<J, 3, SB, R>;
-    case 4: return fwd_fast_kernel<J, 4, SB, R>;
-    case 5: return fwd_fast_kernel<J, 5, SB, R>;
-    case 6: return fwd_fast_kernel<J, 6, SB, R>;
-    case 7: return fwd_fast_kernel<J, 7, SB, R>;
-    case 8: return fwd_fast_kernel<J, 8, SB, R>;
+    case 1: return fwd_fast_kernel<7, 1, 8, 1, SF, SG>;
+    case 2: return fwd_fast_kernel<7, 2, 8, 1, SF, SG>;
+    case 3: return fwd_fast_kernel<7, 3, 8, 1, SF, SG>;
+    case 4: return fwd_fast_kernel<7, 4, 8, 1, SF, SG>;
+    case 5: return fwd_fast_kernel<7, 5, 8, 1, SF, SG>;
+    case 6: return fwd_fast_kernel<7, 6, 8, 1, SF, SG>;
+    case 7: return fwd_fast_kernel<7, 7, 8, 1, SF, SG>;
+    case 8: return fwd_fast_kernel<7, 8, 8, 1, SF, SG>;
     default: return nullptr;
     }
-}
-
-// (views per wave, u1 rows per workgroup): (7, 1), (4, 2), (2, 2); slice blocks of 8
-fwd_fn pick_fwd(int J, int C, int R)
-{
-    if (R == 1 && J == 7)
-        return pick_fwd_c<7, 8, 1>(C);
-    if (R == 2 && J == 4)
-        return pick_fwd_c<4, 8, 2>(C);
-    if (R == 2 && J == 2)
-        return pick_fwd_c<2, 8, 2>(C);
-    return nullptr;
 }
 
 inline float fbits(int i)
@@ -643,18 +645,6 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     // address; each lane holds 2*R*J*C accumulators (56 at 228 u0 bins)
     fp->R = 1;
     fp->J = 7;
-    // (4, 2) and (2, 2) stay selectable for tuning sweeps; (7, 1) measured fastest at 128^3 / 228 u0 bins
-    if (const char* e = std::getenv("PETL_FWD_R")) {
-        if (std::atoi(e) == 2) {
-            fp->R = 2;
-            fp->J = (fp->C <= 4) ? 4 : 2;
-        }
-    }
-    if (const char* e = std::getenv("PETL_FWD_J")) {
-        const int j = std::atoi(e);
-        if (fp->R == 2 && (j == 2 || j == 4) && 2 * fp->R * j * fp->C <= 64)
-            fp->J = j;
-    }
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
     fp->SB = 8;
     fp->AG = BACK_AG;
@@ -803,9 +793,13 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     return fp;
 }
 
-void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
-                  hipStream_t s, int v1_first, int v1_stride)
+template <int SF, int SG>
+static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
+                           const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
+                           float y_scale, float g_scale)
 {
+    using TG = typename Stor<SG>::T;
+    using TF = typename Stor<SF>::T;
     FwdArgs A;
     A.P = fp.P;
     A.Ns = fp.Ns;
@@ -820,14 +814,43 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
     A.v1_first = v1_first;
     A.v1_stride = v1_stride;
     A.xrow = fp.xrow;
+    A.f_scale = (SF == STORE_F16) ? f_scale : 1.0f;
+    A.y_scale = (SG == STORE_F16) ? y_scale : 1.0f;
+    A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn fn = pick_fwd(fp.J, fp.C, fp.R);
-    if (!fn)
+    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C);
+    if (!fn || fp.J != 7 || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
-    hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s, g, f, A,
-                       fp.dev.xtab, fp.dev.xwp, fp.dev.ztab, fp.dev.fscale, epilogue, y, resp);
+    hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s,
+                       static_cast<TG*>(g), static_cast<const TF*>(f), A, fp.dev.xtab, fp.dev.xwp, fp.dev.ztab,
+                       fp.dev.fscale, epilogue, static_cast<const TG*>(y), resp);
     PETL_LAUNCH_CHECK();
+}
+
+void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
+                  hipStream_t s, int v1_first, int v1_stride)
+{
+    fast_forward_t<STORE_F32, STORE_F32>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, 1.0f, 1.0f, 1.0f);
+}
+
+// (volume, planogram) formats: both fp32, both 16-bit of one kind, or an fp32 volume with 16-bit planograms
+void fast_forward_st(const FastPlan& fp, int sf, int sg, void* g, const void* f, int epilogue, const void* y,
+                     const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale, float y_scale,
+                     float g_scale)
+{
+#define PETL_FWD_CASE(A, B)                                                                                    \
+    if (sf == A && sg == B) {                                                                                 \
+        fast_forward_t<A, B>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, f_scale, y_scale, g_scale); \
+        return;                                                                                               \
+    }
+    PETL_FWD_CASE(STORE_F32, STORE_F32)
+    PETL_FWD_CASE(STORE_BF16, STORE_BF16)
+    PETL_FWD_CASE(STORE_F16, STORE_F16)
+    PETL_FWD_CASE(STORE_F32, STORE_BF16)
+    PETL_FWD_CASE(STORE_F32, STORE_F16)
+#undef PETL_FWD_CASE
+    throw hip_error("fast forward: unsupported pair of storage formats");
 }
 
 size_t fast_back_workspace_bytes(const FastPlan& fp)
@@ -835,9 +858,11 @@ size_t fast_back_workspace_bytes(const FastPlan& fp)
     return size_t(fp.P.Nv1) * fp.P.Nu1 * size_t(fp.Ns) * fp.Na * sizeof(float);
 }
 
-void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, hipStream_t s, int v1_first,
-               int v1_stride)
+template <int ST>
+static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, float* workspace, hipStream_t s,
+                        int v1_first, int v1_stride, float g_scale)
 {
+    using TS = typename Stor<ST>::T;
     BackArgs A;
     A.P = fp.P;
     A.V = fp.V;
@@ -854,17 +879,35 @@ void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* wo
     A.sb = fp.SB;
     A.v1_first = v1_first;
     A.v1_stride = v1_stride;
+    A.g_scale = g_scale;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
     const int2* rows = fp.dev.rows + (mode == BACK_PETL ? fp.P.Nv1 : 0);
     const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
     const int nthr = ((fp.Ns + 63) / 64) * 64;
     const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
-    hipLaunchKernelGGL(back_h2_kernel<BACK_AG>, dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s, g,
-                       workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+    hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
+                       static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     PETL_LAUNCH_CHECK();
     hipLaunchKernelGGL(back_z_kernel<32>, dim3((fp.Na + 63) / 64, fp.Ns, (fp.Nz + 31) / 32), dim3(64), 0, s,
                        workspace, f, A, fp.dev.ztab, rows);
     PETL_LAUNCH_CHECK();
+}
+
+void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, hipStream_t s, int v1_first,
+               int v1_stride)
+{
+    fast_back_t<STORE_F32>(fp, g, f, mode, workspace, s, v1_first, v1_stride, 1.0f);
+}
+
+void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode, float* workspace, hipStream_t s,
+                  int v1_first, int v1_stride, float g_scale)
+{
+    switch (st) {
+    case STORE_F32: fast_back_t<STORE_F32>(fp, g, f, mode, workspace, s, v1_first, v1_stride, 1.0f); break;
+    case STORE_BF16: fast_back_t<STORE_BF16>(fp, g, f, mode, workspace, s, v1_first, v1_stride, 1.0f); break;
+    case STORE_F16: fast_back_t<STORE_F16>(fp, g, f, mode, workspace, s, v1_first, v1_stride, g_scale); break;
+    default: throw hip_error("fast back: unknown storage format");
+    }
 }
 
 }  // namespace petl

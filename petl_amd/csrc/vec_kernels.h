@@ -25,6 +25,13 @@ void launch_ratio(float* r, const float* y, const float* q, const float* resp, i
 // Poisson log-likelihood sum(y log(q) - q) over bins with q > 0, in FP64; q = projection (times response).
 // Accumulates into *out (device double) via a two-pass deterministic reduction using `scratch`
 // (at least ll_scratch_doubles(n) doubles).
+// Storage-format forms (STORE_*; x = stored * scale for STORE_F16): fill, copy, and the image updates with
+// the image f in storage format (d and S stay fp32).
+void launch_fill_st(int st, void* x, float v, float scale, int64_t n, hipStream_t s);
+void launch_copy_st(int st, void* dst, const void* src, int64_t n, hipStream_t s);
+void launch_mlem_update_st(int st, void* f, const float* d, const float* S, int64_t n, float f_scale, hipStream_t s);
+void launch_mlem_update_prior_st(int st, void* f_out, const void* f, const float* d, const float* S, int N1, int N2,
+                                 int N3, int kind, float delta, float beta, float p, float f_scale, hipStream_t s);
 int64_t ll_scratch_doubles(int64_t n);
 void launch_loglik(const float* y, const float* q, int64_t row_elems, int first, int stride, int nrows_total,
                    double* scratch, double* out, bool accumulate, hipStream_t s);

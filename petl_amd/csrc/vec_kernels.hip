@@ -1,7 +1,10 @@
 // Elementwise and reduction kernels of the device-resident reconstruction loop.
+#include <cmath>
+#include <cstring>
 #include "petl_common.h"
 #include "vec_kernels.h"
 #include "priors_dev.h"
+#include "storage.h"
 
 namespace petl {
 
@@ -87,6 +90,59 @@ __global__ void __launch_bounds__(64) mlem_update_prior_kernel(float* __restrict
     fo[v] = f[v] * q;
 }
 
+// ---- storage-format forms (16-bit image / planograms, fp32 arithmetic) ----
+template <int ST>
+__global__ void fill_st_kernel(typename Stor<ST>::T* __restrict__ x, float v, float inv_scale, int64_t n)
+{
+    const typename Stor<ST>::T h = Stor<ST>::st(v, inv_scale);
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = h;
+}
+
+template <int ST>
+__global__ void mlem_update_st_kernel(typename Stor<ST>::T* __restrict__ f, const float* __restrict__ d,
+                                      const float* __restrict__ S, int64_t n, float f_scale)
+{
+    const float inv = 1.0f / f_scale;
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+        const float s = S[i];
+        const float q = (s == 0.0f) ? 0.0f : d[i] / s;
+        f[i] = Stor<ST>::st(Stor<ST>::ld(f[i], f_scale) * q, inv);
+    }
+}
+
+template <int KIND, int ST>
+__global__ void __launch_bounds__(64) mlem_update_prior_st_kernel(typename Stor<ST>::T* __restrict__ fo,
+                                                                  const typename Stor<ST>::T* __restrict__ f,
+                                                                  const float* __restrict__ d,
+                                                                  const float* __restrict__ S, int N1, int N2,
+                                                                  int N3, float delta, float beta, HuberParams h,
+                                                                  float f_scale)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= N3)
+        return;
+    const int64_t v = (int64_t(i) * N2 + j) * N3 + k;
+    const float s = S[v];
+    auto ld = [f, f_scale](int64_t x) { return Stor<ST>::ld(f[x], f_scale); };
+    float G = prior_grad_at_ld<KIND>(ld, i, j, k, N1, N2, N3, delta, beta, h);
+    G = G + s;
+    G = fmaxf(0.1f * s, G);
+    const float q = (G == 0.0f) ? 0.0f : d[v] / G;
+    fo[v] = Stor<ST>::st(ld(v) * q, 1.0f / f_scale);
+}
+
+template <int ST>
+__global__ void copy_st_kernel(typename Stor<ST>::T* __restrict__ x, const typename Stor<ST>::T* __restrict__ y,
+                               int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = y[i];
+}
+
 // pass 1: per-block partial sums in FP64
 __global__ void __launch_bounds__(VT) loglik_partial_kernel(const float* __restrict__ y, const float* __restrict__ q,
                                                             int64_t n, int64_t row_elems, int first, int stride,
@@ -131,6 +187,146 @@ __global__ void __launch_bounds__(VT) loglik_final_kernel(const double* __restri
 }
 
 }  // namespace
+
+void launch_fill_st(int st, void* x, float v, float scale, int64_t n, hipStream_t s)
+{
+    switch (st) {
+    case STORE_F32: launch_fill(static_cast<float*>(x), v, n, s); return;
+    case STORE_BF16:
+        hipLaunchKernelGGL(fill_st_kernel<STORE_BF16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<uint16_t*>(x), v, 1.0f, n);
+        break;
+    case STORE_F16:
+        hipLaunchKernelGGL(fill_st_kernel<STORE_F16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<_Float16*>(x), v, 1.0f / scale, n);
+        break;
+    default: throw hip_error("fill: unknown storage format");
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_copy_st(int st, void* dst, const void* src, int64_t n, hipStream_t s)
+{
+    switch (st) {
+    case STORE_F32: launch_copy(static_cast<float*>(dst), static_cast<const float*>(src), n, s); return;
+    case STORE_BF16:
+        hipLaunchKernelGGL(copy_st_kernel<STORE_BF16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<uint16_t*>(dst), static_cast<const uint16_t*>(src), n);
+        break;
+    case STORE_F16:
+        hipLaunchKernelGGL(copy_st_kernel<STORE_F16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<_Float16*>(dst), static_cast<const _Float16*>(src), n);
+        break;
+    default: throw hip_error("copy: unknown storage format");
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_mlem_update_st(int st, void* f, const float* d, const float* S, int64_t n, float f_scale, hipStream_t s)
+{
+    switch (st) {
+    case STORE_F32: launch_mlem_update(static_cast<float*>(f), d, S, n, s); return;
+    case STORE_BF16:
+        hipLaunchKernelGGL(mlem_update_st_kernel<STORE_BF16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<uint16_t*>(f), d, S, n, 1.0f);
+        break;
+    case STORE_F16:
+        hipLaunchKernelGGL(mlem_update_st_kernel<STORE_F16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<_Float16*>(f), d, S, n, f_scale);
+        break;
+    default: throw hip_error("MLEM update: unknown storage format");
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+template <int ST>
+static void update_prior_st(void* fo, const void* f, const float* d, const float* S, int N1, int N2, int N3, int kind,
+                            float delta, float beta, float p, float f_scale, hipStream_t s)
+{
+    using TS = typename Stor<ST>::T;
+    const dim3 grid(ceil_div(N3, 64), N2, N1);
+    if (kind == 0) {
+        HuberParams h{};
+        hipLaunchKernelGGL((mlem_update_prior_st_kernel<0, ST>), grid, dim3(64), 0, s, static_cast<TS*>(fo),
+                           static_cast<const TS*>(f), d, S, N1, N2, N3, delta, beta, h, f_scale);
+    } else {
+        const HuberParams h = huber_params(delta, beta, p);
+        hipLaunchKernelGGL((mlem_update_prior_st_kernel<1, ST>), grid, dim3(64), 0, s, static_cast<TS*>(fo),
+                           static_cast<const TS*>(f), d, S, N1, N2, N3, delta, beta, h, f_scale);
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_mlem_update_prior_st(int st, void* f_out, const void* f, const float* d, const float* S, int N1, int N2,
+                                 int N3, int kind, float delta, float beta, float p, float f_scale, hipStream_t s)
+{
+    switch (st) {
+    case STORE_F32:
+        launch_mlem_update_prior(static_cast<float*>(f_out), static_cast<const float*>(f), d, S, N1, N2, N3, kind,
+                                 delta, beta, p, s);
+        return;
+    case STORE_BF16: update_prior_st<STORE_BF16>(f_out, f, d, S, N1, N2, N3, kind, delta, beta, p, 1.0f, s); return;
+    case STORE_F16: update_prior_st<STORE_F16>(f_out, f, d, S, N1, N2, N3, kind, delta, beta, p, f_scale, s); return;
+    default: throw hip_error("MLEM prior update: unknown storage format");
+    }
+}
+
+// ---- host conversions ----
+void to_storage(int st, const float* x, void* out, int64_t n, float inv_scale)
+{
+    switch (st) {
+    case STORE_F32: memcpy(out, x, size_t(n) * 4); return;
+    case STORE_BF16: {
+        uint16_t* o = static_cast<uint16_t*>(out);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            o[i] = Stor<STORE_BF16>::st(x[i], 1.0f);
+        return;
+    }
+    case STORE_F16: {
+        _Float16* o = static_cast<_Float16*>(out);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            o[i] = Stor<STORE_F16>::st(x[i], inv_scale);
+        return;
+    }
+    default: throw hip_error("unknown storage format");
+    }
+}
+
+void from_storage(int st, const void* x, float* out, int64_t n, float scale)
+{
+    switch (st) {
+    case STORE_F32: memcpy(out, x, size_t(n) * 4); return;
+    case STORE_BF16: {
+        const uint16_t* a = static_cast<const uint16_t*>(x);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            out[i] = Stor<STORE_BF16>::ld(a[i], 1.0f);
+        return;
+    }
+    case STORE_F16: {
+        const _Float16* a = static_cast<const _Float16*>(x);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            out[i] = Stor<STORE_F16>::ld(a[i], scale);
+        return;
+    }
+    default: throw hip_error("unknown storage format");
+    }
+}
+
+float pow2_scale_for(float max_abs, float limit)
+{
+    if (!(max_abs > 0.0f) || !std::isfinite(max_abs))
+        return 1.0f;
+    float s = 1.0f;
+    while (max_abs / s > limit)
+        s *= 2.0f;
+    while (max_abs / s <= 0.5f * limit)
+        s *= 0.5f;
+    return s;
+}
 
 void launch_fill(float* x, float v, int64_t n, hipStream_t s)
 {

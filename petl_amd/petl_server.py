@@ -70,7 +70,25 @@ _SIGNATURES = {
     "MLEM_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(_c_float_p),
                                  ctypes.POINTER(_c_float_p), ctypes.c_void_p]),
+    "project_gpu_ex": (ctypes.c_bool, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_bool]),
+    "backproject_gpu_ex": (ctypes.c_bool, [ctypes.POINTER(ctypes.c_void_p), _c_float_p, ctypes.c_int, ctypes.c_float,
+                                           ctypes.c_void_p, ctypes.c_bool]),
+    "MLEM_gpu_storage": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]),
 }
+
+# storage formats of device buffers (include/petl_c_interface.h): fp32, bfloat16, binary16 with a scale
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
+
+
+def _storage_of_tensor(t):
+    name = str(t.dtype)
+    for key, code in (("float32", 0), ("bfloat16", 1), ("float16", 2)):
+        if name == "torch." + key:
+            return code
+    raise TypeError(f"unsupported device dtype {t.dtype} (float32, bfloat16 or float16)")
 
 _lib = None
 
@@ -314,20 +332,38 @@ class PETLserver:
         ptrs = [ctypes.cast(ctypes.c_void_p(x.data_ptr()), _c_float_p) for x in g]
         return (_c_float_p * len(ptrs))(*ptrs)
 
-    def project_gpu(self, g, f, stream=None, sync=True):
-        """Forward projection on device tensors (torch, float32, contiguous) without host copies."""
+    def project_gpu(self, g, f, stream=None, sync=True, f_scale=1.0, g_scale=1.0):
+        """Forward projection on device tensors (torch, contiguous) without host copies.  float32 tensors
+        use the reference-compatible entry; bfloat16 / float16 tensors (the volume in the planograms' format
+        or float32) are read and written in that format with fp32 arithmetic; for float16, value =
+        stored * scale (f_scale for the volume, g_scale for the planograms)."""
         self.set_model()
         s = None if stream is None else ctypes.c_void_p(stream)
-        if not self.libprojectors.project_gpu(self._dev_ptrs(g), ctypes.cast(ctypes.c_void_p(f.data_ptr()),
-                                                                                _c_float_p), s, sync):
+        sf, sg = _storage_of_tensor(f), _storage_of_tensor(g[0])
+        if sf == 0 and sg == 0:
+            ok = self.libprojectors.project_gpu(self._dev_ptrs(g), ctypes.cast(ctypes.c_void_p(f.data_ptr()),
+                                                                                _c_float_p), s, sync)
+        else:
+            ptrs = (ctypes.c_void_p * len(g))(*[x.data_ptr() for x in g])
+            ok = self.libprojectors.project_gpu_ex(ptrs, ctypes.c_void_p(f.data_ptr()), sf, sg, float(f_scale),
+                                                   float(g_scale), s, sync)
+        if not ok:
             raise TypeError('project failed')
         return g
 
-    def backproject_gpu(self, g, f, stream=None, sync=True):
+    def backproject_gpu(self, g, f, stream=None, sync=True, g_scale=1.0):
+        """Back projection of device planograms (float32, or bfloat16 / float16 read with fp32 arithmetic)
+        into a float32 device volume."""
         self.set_model()
         s = None if stream is None else ctypes.c_void_p(stream)
-        if not self.libprojectors.backproject_gpu(self._dev_ptrs(g), ctypes.cast(ctypes.c_void_p(f.data_ptr()),
-                                                                                    _c_float_p), s, sync):
+        sg = _storage_of_tensor(g[0])
+        fp = ctypes.cast(ctypes.c_void_p(f.data_ptr()), _c_float_p)
+        if sg == 0:
+            ok = self.libprojectors.backproject_gpu(self._dev_ptrs(g), fp, s, sync)
+        else:
+            ptrs = (ctypes.c_void_p * len(g))(*[x.data_ptr() for x in g])
+            ok = self.libprojectors.backproject_gpu_ex(ptrs, fp, sg, float(g_scale), s, sync)
+        if not ok:
             raise TypeError('backproject failed')
         return f
 
@@ -464,15 +500,34 @@ class PETLserver:
 
     # ------------------------------------------------------------------ reconstruction
     def MLEM(self, g, f, numIter, delta=2.0, beta=0.0, mask=None, prior="rd", p=1.0, num_subsets=1,
-             return_loglik=False):
+             return_loglik=False, storage="f32", image_storage=None):
         r"""Maximum Likelihood-Expectation Maximization, f <- f / (P^T 1 + R'(f)) * P^T[g / Pf].
 
         Same arguments and result as the reference's PETLserver.MLEM (src/petl_server.py:1534-1606):
         f None -> ones, all-zero f -> ones, negatives clipped; beta > 0 adds the relative-difference
         prior gradient (delta) one-step-late; the response, when set, weights the projections.  The
         whole loop runs on the GPU.  Extras: prior='tv' uses the p-Huber TV gradient instead,
-        num_subsets > 1 runs OSEM, return_loglik=True also returns sum(y log Pf - Pf) per iteration.
+        num_subsets > 1 runs OSEM, return_loglik=True also returns sum(y log Pf - Pf) per iteration;
+        storage='bf16' / 'f16' keeps the planograms (data, ratio) on the device in that 16-bit format and
+        image_storage (default: the same) the image, with fp32 arithmetic (no mask, response or
+        log-likelihood then).
         """
+        if image_storage is None:
+            image_storage = storage
+        if storage != "f32" or image_storage != "f32":
+            if mask is not None or return_loglik or self.response is not None:
+                raise ValueError("16-bit storage takes no mask, response or log-likelihood")
+            if f is None:
+                f = self.allocate_volume()
+                self.setToOne(f)
+            self._check_arrays(g, f)
+            kind = 0 if beta <= 0.0 else (2 if prior == "tv" else 1)
+            self.set_model()
+            if not self.libprojectors.MLEM_gpu_storage(self.floatstarstar(g), f, int(numIter), float(delta),
+                                                       float(beta), kind, float(p), int(num_subsets),
+                                                       STORAGE[storage], STORAGE[image_storage]):
+                raise TypeError('MLEM failed')
+            return f
         if mask is not None and (len(mask) != len(g) or any(m.shape != a.shape for m, a in zip(mask, g))):
             print('Error: mask must be the same shape as the projection data!')
             return None

@@ -1,0 +1,40 @@
+"""Reduced-precision storage vs the FP64 oracle (study in tests/precision_common.py).
+
+Bounds after 20 iterations (relative L2 image error against the FP64 oracle's iterate):
+* fp32: 1e-4 at every iteration;
+* fp16 with per-tensor scales (image and planograms), and bf16 or fp16 planograms with an fp32 image:
+  1e-3 at every iteration;
+* bf16 image and planograms: 5e-3.  A bf16 image cannot meet 1e-3: rounding an image to bf16's 8-bit
+  significand alone leaves about 1.6e-3 relative rms error.
+Log-likelihood relative error: 1e-5 for every form.
+"""
+import pytest
+
+import precision_common as pc
+
+pytestmark = pytest.mark.gpu
+
+BOUNDS = {
+    "fp32": 1e-4,
+    "bf16 planograms + bf16 image": 5e-3,
+    "fp16 planograms + fp16 image (per-tensor scale)": 1e-3,
+    "bf16 planograms + fp32 image": 1e-3,
+    "fp16 planograms + fp32 image (per-tensor scale)": 1e-3,
+}
+
+
+@pytest.fixture(scope="module")
+def rows(oracle):
+    return pc.study(oracle)
+
+
+@pytest.mark.parametrize("label", list(BOUNDS))
+def test_image_error_bounded_every_iteration(rows, label):
+    img, _ = rows[label]
+    assert max(img) <= BOUNDS[label], [f"{e:.2e}" for e in img]
+
+
+@pytest.mark.parametrize("label", list(BOUNDS))
+def test_loglik_error_bounded(rows, label):
+    _, ll = rows[label]
+    assert max(ll) <= 1e-5, [f"{e:.2e}" for e in ll]
