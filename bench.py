@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (reference for HBM byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (req/s)")
+    ap.add_argument("--serve-rates", default="4,20",
+                    help="request arrival rates (req/s), comma separated: one below the service's capacity "
+                         "and one above it")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=8)
     return ap.parse_args()
@@ -209,6 +211,9 @@ def main():
 
     V = n ** 3
     G = [int(np.prod(sh)) for sh in shapes]
+    # the forward's arithmetic: two FMAs (4 flop) per bin and slice (the u0 interpolation sums Q and W);
+    # slices = n (the volume axis the view direction crosses)
+    flop_per_launch = 4.0 * n * G[0]
     nplan = len(shapes)
     fwd_launch_ms = fwd_ms / nplan                # the forward call is one fwd kernel launch per planogram
     alg_bytes = 4.0 * (V + G[0])                  # per launch: read the volume, write one planogram
@@ -249,6 +254,16 @@ def main():
             "traffic": None,
             "alg_bytes_per_launch": alg_bytes,
         },
+        # the bound that actually limits this kernel: FP32 vector arithmetic (published peak 157.3 TFLOP/s
+        # counts packed FP32, which this build does not emit; unpacked FP32 peaks at 78.6)
+        "valu_roofline": {
+            "kernel": "fwd_fast_kernel",
+            "achieved": flop_per_launch / (fwd_launch_ms * 1e-3) / 1e12,
+            "peak": 157.3,
+            "unit": "TFLOP/s",
+            "frac": flop_per_launch / (fwd_launch_ms * 1e-3) / 1e12 / 157.3,
+            "flop_per_launch": flop_per_launch,
+        },
     }
     # the same pair with the volume and planograms stored in bfloat16 / binary16 (fp32 arithmetic, fp32
     # back-projected volume)
@@ -282,16 +297,18 @@ def main():
         del g, f, fb
         torch.cuda.empty_cache()
         data = serving_datasets(torch, dev, seed=500)
-        mixed = serving_run(data, a.serve_rate, a.serve_requests, a.serve_slots, seed=501)
-        single = serving_run(data, a.serve_rate, a.serve_requests, 1, seed=501)
+        runs = []
+        for rate in [float(x) for x in a.serve_rates.split(",") if x]:
+            mixed = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501)
+            single = serving_run(data, rate, a.serve_requests, 1, seed=501)
+            runs.append({"arrival_rate_req_s": rate, "continuous_batching": dict(mixed, slots=a.serve_slots),
+                         "one_at_a_time": dict(single, slots=1)})
         out["serving"] = {
             "workload": "Poisson arrivals; panels uniform over 48/64/96/128 crystals (2 orientations, D = 200 mm, "
                         "matched n^3 volumes at 2 mm); data = Poisson at counts log-uniform in [1e5, 1e8]; "
                         "MLEM iterations uniform in 5..20; fp32",
-            "arrival_rate_req_s": a.serve_rate,
             "requests": a.serve_requests,
-            "continuous_batching": dict(mixed, slots=a.serve_slots),
-            "one_at_a_time": dict(single, slots=1),
+            "runs": runs,
         }
     if not a.no_cpu:
         v, cores, desc = cpu_baseline(a.cpu_rows, f_host)

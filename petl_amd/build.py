@@ -30,29 +30,31 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+def _compile(src, build_dir=BUILD, defines=()):
+    obj = os.path.join(build_dir, os.path.basename(src)[:-4] + ".o")
+    cmd = [HIPCC] + CFLAGS + [f"-D{d}" for d in defines] + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(jobs=None, verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
+def build(jobs=None, verbose=True, lib=LIB, build_dir=BUILD, defines=()):
+    """Compiles every source and links `lib`.  `defines` (NAME=VALUE strings) select compile-time kernel
+    variants for tuning builds; the default build uses none."""
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 4, len(srcs))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, build_dir, defines), srcs))
+    tmp = lib + ".tmp"
     r = subprocess.run([HIPCC] + LDFLAGS + objs + ["-o", tmp], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":

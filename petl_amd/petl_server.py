@@ -94,7 +94,8 @@ _lib = None
 
 
 def library_path():
-    return os.path.join(_HERE, "libpetl.so")
+    """petl_amd/libpetl.so, or the library named by the PETL_LIBRARY environment variable."""
+    return os.environ.get("PETL_LIBRARY") or os.path.join(_HERE, "libpetl.so")
 
 
 def load_library(path=None):

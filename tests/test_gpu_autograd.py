@@ -4,9 +4,12 @@ Workload: a batch of 8 seeded 64^3 float32 volumes at 2 mm, two 64 x 64-crystal 
 D = 200 mm) -> planograms (63, 101, 64, 164); Poisson data at 1e7 counts per volume; loss = the negative
 Poisson log-likelihood sum(Af - y log Af).
 
-* central finite differences on 64 seeded voxels.  The step is h = 1e-3 x the mean voxel value, and the
-  loss difference is summed over bins (bins the voxel does not touch cancel exactly), so fp32 rounding of
-  Af sets the error: 1e-3 relative L2 over the 64 derivatives;
+* central finite differences on 64 seeded voxels: 1e-3 relative L2 over the 64 derivatives.  The loss
+  difference is summed over bins (bins the voxel does not touch cancel exactly), so fp32 rounding of Af
+  sets the error.  At 1e7 counts over 1.3e8 bins the counts are 0, 1 or 2 where Af ~ 0.08, so each
+  derivative sums terms of +-12 to a value of order 10-100; with a step of 1e-3 x the mean voxel value
+  that rounding reaches 5 % on such voxels.  The step is 2e-2 x the mean voxel value (truncation error
+  of the central difference ~1e-6 relative);
 * the gradient equals the oracle's matched back projection of the residual 1 - y / Af (oracle forward
   and back in fp32): 1e-4 relative L2;
 * <A x, y> = <x, A^T y> through torch.autograd.grad, to 1e-6.
@@ -63,7 +66,7 @@ def test_gradient_matches_finite_differences(setup):
     # 64 voxels inside the central 60 % of the field of view, spread over the batch
     idx = [(int(rng.integers(B)),) + tuple(int(v) for v in rng.integers(int(0.2 * N), int(0.8 * N), 3))
            for _ in range(64)]
-    h = 1e-3 * float(f0.mean())
+    h = 2e-2 * float(f0.mean())
     fd, ad = [], []
     with torch.no_grad():
         for (b, k, j, i) in idx:
