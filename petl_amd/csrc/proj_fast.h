@@ -19,7 +19,8 @@ namespace petl {
 
 struct FastTables {
     int2* xtab = nullptr;     // [Nv0][Ns + pad]  {k, wq}: u0 interpolation of slice s in view v0
-    float* xwp = nullptr;     // [Nv0][Ns + pad]  1 - wq, formed in float
+    int2* ftab = nullptr;     // forward: [Ns / SB][Nv0][SB] the xtab entries of one slice block and view
+    float* fwp = nullptr;     // forward: [Ns / SB][Nv0][16]  SB x (1 - wq) formed in float, chunk mask, pad
     int4* ztab = nullptr;     // [Nv1][Ns]  {m1, wa, wb, 0}: u1 interpolation of slice s in view v1
     float* fscale = nullptr;  // [Nv1][Nv0] forward / matched scale  l_phi / sqrt(1+v0^2+v1^2)
     float* pscale = nullptr;  // [Nv1][Nv0] PETL back scale  T_x / max(|wy|,|wx|) / sqrt(1+v0^2+v1^2)

@@ -38,6 +38,7 @@ namespace petl {
 namespace {
 
 constexpr int FWD_THREADS = 512;
+constexpr int FWD_WPROW = 16;  // floats per (block, view) row of the forward's wp table: SB wp, mask, pad
 constexpr int FWD_WAVES = FWD_THREADS / 64;
 constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
 constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows
@@ -139,8 +140,8 @@ __device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, float* hb, flo
 template <int J, int C, int SB, int R, int SF, int SG>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
                                                                const typename Stor<SF>::T* __restrict__ f,
-                                                               FwdArgs A, const int2* __restrict__ xtab,
-                                                               const float* __restrict__ xwp,
+                                                               FwdArgs A, const int2* __restrict__ ftab,
+                                                               const float* __restrict__ fwp,
                                                                const int4* __restrict__ ztab,
                                                                const float* __restrict__ fscale, int epilogue,
                                                                const typename Stor<SG>::T* __restrict__ yv,
@@ -243,14 +244,18 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                 zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
             const int is0 = is_first + blk * SB;
             const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
+            // this block's table rows: [block][view] x {SB entries {(is % SB) * FWD_HP + k, wq}} and
+            // {SB x wp, chunk mask, pad}; the wave's views sit at a fixed stride, so every view's scalar
+            // loads are immediate offsets from these two addresses
+            const int bi = is0 / SB;
+            const int2* xt_b = ftab + (int64_t(bi) * Nv0 + iv0_base) * SB;
+            const float* xp_b = fwp + (int64_t(bi) * Nv0 + iv0_base) * FWD_WPROW;
 #pragma unroll
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
-                    // entries {(is % SB) * FWD_HP + k, wq} and wp of view j (scalar loads; requesting them a
-                    // view ahead gained nothing: scalar and LDS loads share one wait counter)
-                    const int2* xt = xtab + int64_t(iv0) * A.xrow + is0;  // rows are padded past Ns
-                    const float* xp = xwp + int64_t(iv0) * A.xrow + is0;
+                    const int2* xt = xt_b + FWD_WAVES * SB * j;
+                    const float* xp = xp_b + FWD_WAVES * FWD_WPROW * j;
                     int off[SB];
                     float wq[SB], wp[SB];
 #pragma unroll
@@ -260,18 +265,15 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                         wq[s] = __int_as_float(e.y);
                         wp[s] = xp[s];
                     }
-                    // k is monotone in the slice index, so the block's extremes are its end values
-                    const int k0 = off[0], k1 = off[SB - 1] - (SB - 1) * FWD_HP;
-                    const int kmin = min(k0, k1), kmax = max(k0, k1);
-                    const int c_lo = max(0, (-kmax) / 64);  // first chunk with 64c + 63 >= -kmax
-                    const int c_hi = min(C - 1, (Na - 1 - kmin) >= 0 ? (Na - 1 - kmin) / 64 : -1);
+                    // bit c: some slice of the block reaches output chunk c
+                    const int cmask = __float_as_int(xp[SB]);
                     const float* hp[SB];
 #pragma unroll
                     for (int s = 0; s < SB; s++)
                         hp[s] = hb + off[s];
 #pragma unroll
                     for (int c = 0; c < C; c++) {
-                        if (c >= c_lo && c <= c_hi) {
+                        if (cmask & (1 << c)) {
 #pragma unroll
                             for (int r = 0; r < R; r++) {
                                 float v[SB];
@@ -725,6 +727,31 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
             pscale[size_t(iv1) * P.Nv0 + iv0] = lp * V.Tx;
         }
     }
+    // forward tables in [slice block][view] order with each block's chunk mask
+    const int SBk = fp->SB;
+    const int nbt = (Ns + SBk - 1) / SBk;
+    std::vector<int2> ftab(size_t(nbt) * P.Nv0 * SBk);
+    std::vector<float> fwp(size_t(nbt) * P.Nv0 * FWD_WPROW, 0.0f);
+    if (SBk + 1 > FWD_WPROW)
+        return nullptr;
+    for (int b = 0; b < nbt; b++)
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const size_t row = size_t(b) * P.Nv0 + iv0;
+            int mask = 0;
+            for (int sl = 0; sl < SBk; sl++) {
+                const int is = b * SBk + sl;  // < xrow: rows are padded past Ns
+                const int2 e = xtab[size_t(iv0) * xrow + is];
+                ftab[row * SBk + sl] = e;
+                fwp[row * FWD_WPROW + sl] = xwp[size_t(iv0) * xrow + is];
+                if (is < Ns) {
+                    const int k = e.x - sl * FWD_HP;  // lane p reads H[s][p + 64 c + k]
+                    for (int c = 0; c < fp->C; c++)
+                        if (64 * c + k <= Na - 1 && 64 * c + 63 + k >= 0)
+                            mask |= 1 << c;
+                }
+            }
+            fwp[row * FWD_WPROW + SBk] = fbits(mask);
+        }
     fp->hpad = 64 + kspread + 1;
     fp->hrow = FWD_HP;
     if (fp->hpad + Na + 63 + kspread >= FWD_HP)  // reads of an active chunk must stay inside the LDS row
@@ -739,18 +766,21 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         return nullptr;
 
     // ---- upload ----
-    const size_t nx = xtab.size() * sizeof(int2), nxw = xwp.size() * sizeof(float), nzb = ztab.size() * sizeof(int4);
+    const size_t nx = xtab.size() * sizeof(int2), nft = ftab.size() * sizeof(int2), nfw = fwp.size() * sizeof(float);
+    const size_t nzb = ztab.size() * sizeof(int4);
     const size_t nf = fscale.size() * sizeof(float), nr = rows.size() * sizeof(int2);
     auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
-    const size_t total = al(nx) + al(nxw) + al(nzb) + 2 * al(nf) + al(nr);
+    const size_t total = al(nx) + al(nft) + al(nfw) + al(nzb) + 2 * al(nf) + al(nr);
     char* blk = nullptr;
     PETL_HIP_CHECK(hipMalloc(&blk, total));
     fp->dev_block = blk;
     size_t off = 0;
     fp->dev.xtab = reinterpret_cast<int2*>(blk + off);
     off += al(nx);
-    fp->dev.xwp = reinterpret_cast<float*>(blk + off);
-    off += al(nxw);
+    fp->dev.ftab = reinterpret_cast<int2*>(blk + off);
+    off += al(nft);
+    fp->dev.fwp = reinterpret_cast<float*>(blk + off);
+    off += al(nfw);
     fp->dev.ztab = reinterpret_cast<int4*>(blk + off);
     off += al(nzb);
     fp->dev.fscale = reinterpret_cast<float*>(blk + off);
@@ -759,7 +789,8 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     off += al(nf);
     fp->dev.rows = reinterpret_cast<int2*>(blk + off);
     PETL_HIP_CHECK(hipMemcpy(fp->dev.xtab, xtab.data(), nx, hipMemcpyHostToDevice));
-    PETL_HIP_CHECK(hipMemcpy(fp->dev.xwp, xwp.data(), nxw, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.ftab, ftab.data(), nft, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(fp->dev.fwp, fwp.data(), nfw, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.ztab, ztab.data(), nzb, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.fscale, fscale.data(), nf, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(fp->dev.pscale, pscale.data(), nf, hipMemcpyHostToDevice));
@@ -797,7 +828,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
     hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s,
-                       static_cast<TG*>(g), static_cast<const TF*>(f), A, fp.dev.xtab, fp.dev.xwp, fp.dev.ztab,
+                       static_cast<TG*>(g), static_cast<const TF*>(f), A, fp.dev.ftab, fp.dev.fwp, fp.dev.ztab,
                        fp.dev.fscale, epilogue, static_cast<const TG*>(y), resp);
     PETL_LAUNCH_CHECK();
 }
