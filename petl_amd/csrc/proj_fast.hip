@@ -37,6 +37,11 @@ namespace petl {
 
 namespace {
 
+// views per wave of the forward (compile-time tuning constant; 2 * J * C accumulators per lane)
+#ifndef PETL_FWD_J
+#define PETL_FWD_J 7
+#endif
+constexpr int FWD_J = PETL_FWD_J;
 constexpr int FWD_THREADS = 512;
 constexpr int FWD_WPROW = 16;  // floats per (block, view) row of the forward's wp table: SB wp, mask, pad
 constexpr int FWD_WAVES = FWD_THREADS / 64;
@@ -528,20 +533,20 @@ template <int SF, int SG>
 using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdArgs, const int2*, const float*,
                         const int4*, const float*, int, const typename Stor<SG>::T*, const float*);
 
-// 7 views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
+// FWD_J views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
 // two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower)
 template <int SF, int SG>
 fwd_fn<SF, SG> pick_fwd(int C)
 {
     switch (C) {
-    case 1: return fwd_fast_kernel<7, 1, 8, 1, SF, SG>;
-    case 2: return fwd_fast_kernel<7, 2, 8, 1, SF, SG>;
-    case 3: return fwd_fast_kernel<7, 3, 8, 1, SF, SG>;
-    case 4: return fwd_fast_kernel<7, 4, 8, 1, SF, SG>;
-    case 5: return fwd_fast_kernel<7, 5, 8, 1, SF, SG>;
-    case 6: return fwd_fast_kernel<7, 6, 8, 1, SF, SG>;
-    case 7: return fwd_fast_kernel<7, 7, 8, 1, SF, SG>;
-    case 8: return fwd_fast_kernel<7, 8, 8, 1, SF, SG>;
+    case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG>;
+    case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG>;
+    case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG>;
+    case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG>;
+    case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG>;
+    case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG>;
+    case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG>;
+    case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG>;
     default: return nullptr;
     }
 }
@@ -620,7 +625,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     // u1 rows per workgroup and views per wave: R rows share each (view, slice) table entry and LDS
     // address; each lane holds 2*R*J*C accumulators (56 at 228 u0 bins)
     fp->R = 1;
-    fp->J = 7;
+    fp->J = FWD_J;
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
     fp->SB = 8;
     fp->AG = BACK_AG;
@@ -824,7 +829,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
     fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C);
-    if (!fn || fp.J != 7 || fp.R != 1)
+    if (!fn || fp.J != FWD_J || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
     hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s,

@@ -109,3 +109,53 @@ def test_sampled_rows_match_oracle(full, oracle):
         oracle.project_rows(p, vol, f, ref, 77, 78)
         got = gi[77].cpu().numpy()
         assert np.linalg.norm(got - ref[77]) / np.linalg.norm(ref[77]) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def full_data(full):
+    """Poisson data at 1e8 counts of the seeded 20-ellipsoid phantom (host arrays)."""
+    s, shapes = full
+    f = torch.from_numpy(ellipsoid_phantom(128, 2.0, 20, seed=2)).to(DEV)
+    q = proj(s, shapes, f)
+    tot = float(sum(float(x.double().sum()) for x in q))
+    gen = torch.Generator(device=DEV).manual_seed(102)
+    y = [torch.poisson(x * (1e8 / tot), generator=gen).cpu().numpy() for x in q]
+    del q
+    torch.cuda.empty_cache()
+    return y
+
+
+def test_mlem_full_size_matched_invariants(full, full_data):
+    """Matched back projector, no prior, 3 iterations: count preservation sum(S f') = sum over bins with
+    Af > 0 of y (to 1e-5), non-negativity, and a non-decreasing Poisson log-likelihood."""
+    s, shapes = full
+    y = full_data
+    s.set_backprojector_mode("matched")
+    try:
+        f0 = np.ones((128, 128, 128), np.float32)
+        f1, ll = s.MLEM(y, f0.copy(), 3, return_loglik=True)
+        S = back(s, [torch.ones(sh, dtype=torch.float32, device=DEV) for sh in shapes]).double()
+        S[S == 0] = 1
+        # count preservation after the first iteration
+        f_one, _ = s.MLEM(y, f0.copy(), 1, return_loglik=True)
+        q0 = proj(s, shapes, torch.from_numpy(f0).to(DEV))
+        counts = sum(float(torch.from_numpy(yy).to(DEV)[a > 0].double().sum()) for a, yy in zip(q0, y))
+        lhs = float((S * torch.from_numpy(f_one).to(DEV).double()).sum())
+    finally:
+        s.set_backprojector_mode("petl")
+    assert lhs == pytest.approx(counts, rel=1e-5)
+    assert np.all(f1 >= 0) and np.all(np.isfinite(f1))
+    assert np.all(np.diff(ll) >= -1e-7 * np.abs(ll[1:]))
+
+
+def test_mlem_full_size_rd_prior(full, full_data):
+    """The full-size run of the benchmark description: 20 iterations with the relative-difference prior
+    (beta = 0.01, gamma = 2), reference (PETL) back projector.  Finite and non-negative; storing the
+    planograms in bf16 changes the image by less than 1e-3 relative L2."""
+    s, _ = full
+    y = full_data
+    f32 = s.MLEM(y, None, 20, delta=2.0, beta=0.01)
+    assert np.all(np.isfinite(f32)) and np.all(f32 >= 0) and float(f32.max()) > 0
+    fbf = s.MLEM(y, None, 20, delta=2.0, beta=0.01, storage="bf16", image_storage="f32")
+    err = float(np.linalg.norm(fbf.astype(np.float64) - f32) / np.linalg.norm(f32.astype(np.float64)))
+    assert err < 1e-3, err
