@@ -92,7 +92,7 @@ PETL_EXTERN bool relativeDifferences_gradient(float* f, float* Df, int N_1, int 
                                               float beta);                                        /* :96 */
 
 /* exports outside the projector / MLEM path: present, return false / 0 with a message */
-PETL_EXTERN bool bin(float** g, char* file_name);                                          /* :55 */
+
 PETL_EXTERN bool simulate_scatter(float** g, float* mu);                                   /* :57 */
 PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
 PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
@@ -137,6 +137,24 @@ PETL_EXTERN int planogram_uses_fast_path(int which);
  * unless sync is true.  backproject_gpu overwrites d_f. */
 PETL_EXTERN bool project_gpu(float** d_g, float* d_f, void* stream, bool sync);
 PETL_EXTERN bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync);
+
+/* List-mode binning (src/petl_c_interface.h:55, src/petl.cpp:65-79, src/binning.cpp).  bin() zeroes the
+ * planograms and bins the coincidences of the file with the reference's defaults: millimetres, list-mode
+ * z origin at the chest wall (u1 -= H), nearest v bins and clamped bilinear u bins.  ".lst" files hold
+ * records of two {float pos[4]; uint64 time} events; any other name is read in the acquisition layout
+ * (int64 count; per end point int16 interactions, float pos[3], float energy, 18 bytes per further
+ * interaction).  Returns false when the file cannot be opened.  The binned values are bit-identical to
+ * the reference's: contributions are added to each bin in event order. */
+PETL_EXTERN bool bin(float** g, char* file_name);                                          /* :55 */
+/* Bins n coincidences given as arrays of x, y, z (mm) of the blue and pink end points, ADDING to the
+ * planograms (no reset): host arrays and planograms / device arrays and planograms on a stream.
+ * nearest: 1 = nearest u bins, 0 = clamped bilinear (the reference default); chest_wall: 1 = z origin at
+ * the chest wall (the reference default), 0 = at the centre of the field of view.  accepted (optional,
+ * host): coincidences each planogram accepted. */
+PETL_EXTERN bool bin_events(const float* blue, const float* pink, long long n, float** g, int nearest,
+                            int chest_wall, long long* accepted);
+PETL_EXTERN bool bin_events_gpu(const float* d_blue, const float* d_pink, long long n, float** d_g, int nearest,
+                                int chest_wall, void* stream, bool sync);
 
 /* Storage formats of device volumes / planograms for the *_ex and *_storage calls: 0 = fp32, 1 = bfloat16,
  * 2 = IEEE binary16 with a per-tensor scale (value = stored * scale).  Arithmetic is fp32 throughout;

@@ -81,6 +81,10 @@ def lib():
         L.orc_project_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         L.orc_backproject_petl_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         L.orc_project_subset_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
+        L.orc_bin_events.argtypes = [P, fp, fp, ctypes.c_int64, c_int, c_int, fp]
+        L.orc_bin_events.restype = ctypes.c_int64
+        L.orc_event_coords.argtypes = [P, fp, fp, fp]
+        L.orc_event_coords.restype = c_int
         L.orc_backproject_petl_subset_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         _lib = L
     return _lib
@@ -270,6 +274,88 @@ def osem(plans, vol, g, num_iter, num_subsets, f0=None, nthreads=None):
 
 
 # --------------------------------------------------------------------------------------------------
+# list-mode binning (src/binning.cpp) and the two list-mode file layouts it reads
+# --------------------------------------------------------------------------------------------------
+def bin_events(plans, blue, pink, nearest=False, chest_wall=True, g=None):
+    """Bins coincidences (blue, pink: (n, 3) float32 end points in mm) into the planograms, adding to g
+    (zeros when None).  Returns (g, accepted counts per planogram)."""
+    blue = np.ascontiguousarray(blue, dtype=np.float32)
+    pink = np.ascontiguousarray(pink, dtype=np.float32)
+    if g is None:
+        g = [np.zeros(p.shape, np.float32) for p in plans]
+    acc = []
+    for p, gi in zip(plans, g):
+        acc.append(int(lib().orc_bin_events(ctypes.byref(p), blue, pink, len(blue), int(bool(nearest)),
+                                            int(bool(chest_wall)), gi)))
+    return g, acc
+
+
+def event_coords(plan, blue, pink):
+    """(accepted, [v1, v0, u1, u0]) of one coincidence (u from the blue end point, no origin shift)."""
+    out = np.zeros(4, np.float32)
+    ok = lib().orc_event_coords(ctypes.byref(plan), np.asarray(blue, np.float32), np.asarray(pink, np.float32), out)
+    return bool(ok), out
+
+
+# .lst: records of two events {float pos[4] (x, y, z, energy); uint64 time} (src/binning.cpp:9-19, 105-114)
+LST_EVENT = np.dtype([("pos", "<f4", (4,)), ("time", "<u8")])
+LST_RECORD = np.dtype([("blue", LST_EVENT), ("pink", LST_EVENT)])
+
+
+def write_lst(path, blue, pink, energy=511.0):
+    rec = np.zeros(len(blue), LST_RECORD)
+    rec["blue"]["pos"][:, :3] = blue
+    rec["pink"]["pos"][:, :3] = pink
+    rec["blue"]["pos"][:, 3] = energy
+    rec["pink"]["pos"][:, 3] = energy
+    rec["blue"]["time"] = np.arange(len(blue))
+    rec["pink"]["time"] = np.arange(len(blue))
+    rec.tofile(path)
+
+
+def read_lst(path):
+    rec = np.fromfile(path, LST_RECORD)
+    return rec["blue"]["pos"][:, :3].copy(), rec["pink"]["pos"][:, :3].copy()
+
+
+def write_acq(path, blue, pink, extra_interactions=None, energy=511.0, seed=0):
+    """Acquisition layout of src/binning.cpp:598-617: int64 N, then per coincidence for blue and pink:
+    int16 number of interactions, float pos[3], float energy, then 18 bytes per further interaction
+    (skipped by the reader).  extra_interactions: (n, 2) counts of further interactions."""
+    rng = np.random.default_rng(seed)
+    n = len(blue)
+    if extra_interactions is None:
+        extra_interactions = np.zeros((n, 2), np.int64)
+    parts = [np.array([n], "<i8").tobytes()]
+    for i in range(n):
+        for side, pos in ((0, blue[i]), (1, pink[i])):
+            k = int(extra_interactions[i, side])
+            parts.append(np.array([k + 1], "<i2").tobytes())
+            parts.append(np.asarray(pos, "<f4").tobytes())
+            parts.append(np.array([energy], "<f4").tobytes())
+            parts.append(rng.integers(0, 255, size=18 * k, dtype=np.uint8).tobytes())
+    with open(path, "wb") as fh:
+        fh.write(b"".join(parts))
+
+
+def read_acq(path, units_cm=False):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:8], "<i8")[0])
+    off = 8
+    blue = np.zeros((n, 3), np.float32)
+    pink = np.zeros((n, 3), np.float32)
+    for i in range(n):
+        for out in (blue, pink):
+            k = int(np.frombuffer(raw[off:off + 2], "<i2")[0])
+            out[i] = np.frombuffer(raw[off + 2:off + 14], "<f4")
+            off += 18 + 18 * max(k - 1, 0)
+    if units_cm:  # bluePos[i] *= 10.0: a float times a double, stored as float
+        blue = (blue.astype(np.float64) * 10.0).astype(np.float32)
+        pink = (pink.astype(np.float64) * 10.0).astype(np.float32)
+    return blue, pink
+
+
+# --------------------------------------------------------------------------------------------------
 # the reference's own geometry / dataCube (only where the reference sources could be compiled)
 # --------------------------------------------------------------------------------------------------
 _ref = None
@@ -291,5 +377,7 @@ def ref_lib():
         R.ref_reciprocal3d.argtypes = [fp, c_int, c_int, c_int, c_float]
         R.ref_inner3d.argtypes = [fp, fp, ctypes.c_void_p, c_int, c_int, c_int]
         R.ref_inner3d.restype = c_float
+        R.ref_event_coords.argtypes = [c_float] * 7 + [fp, fp, fp]
+        R.ref_event_coords.restype = c_int
         _ref = R
     return _ref

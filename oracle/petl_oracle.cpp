@@ -708,4 +708,93 @@ void orc_backproject_petl_subset_f32(const orc_plan* P, const orc_vol* V, const 
     backproject_petl_plan<float>(*P, *V, g, f, nthreads, first, P->N_v1, stride);
 }
 
+// ------------------------------------------------------------------------------------------------
+// List-mode binning.  Per coincidence (blue and pink end points, already in mm):
+//   r = blue - pink; acceptance and (v1, v0) of src/planogram.cpp:113-128; (u1, u0) from the blue end
+//   point, src/planogram.cpp:130-141; chest-wall origin: u1 -= H (src/binning.cpp:636-640).
+// Then the flush of src/binning.cpp:662-750: nearest v bins by int(x + 0.5) (x + 0.5 in double,
+// truncation toward zero), and in u either nearest (binning type 0) or the clamped bilinear split with
+// weights formed in double and added to the float bins one at a time, in event order.
+// Returns the number of accepted coincidences (src/binning.cpp:643 counts them).
+// ------------------------------------------------------------------------------------------------
+// (v1, v0, u1, u0) of one coincidence; false when the planogram does not accept it
+static bool event_coords(const orc_plan& p, const float* b, const float* k, float* out)
+{
+    const float r0 = b[0] - k[0], r1 = b[1] - k[1], r2 = b[2] - k[2];
+    const float c = -p.sin_psi * r0 + p.cos_psi * r1;
+    if (c == 0.0f)
+        return false;
+    const float v0 = -(p.cos_psi * r0 + p.sin_psi * r1) / c;
+    if (std::fabs(v0) > p.v_m0)
+        return false;
+    const float v1 = -r2 / c;
+    if (std::fabs(v1) > p.v_m1)
+        return false;
+    const float t = b[1] * p.cos_psi - b[0] * p.sin_psi;
+    out[0] = v1;
+    out[1] = v0;
+    out[2] = b[2] + v1 * t;
+    out[3] = b[0] * p.cos_psi + b[1] * p.sin_psi + v0 * t;
+    return true;
+}
+
+int orc_event_coords(const orc_plan* P, const float* blue, const float* pink, float* out)
+{
+    return event_coords(*P, blue, pink, out) ? 1 : 0;
+}
+
+int64_t orc_bin_events(const orc_plan* P, const float* blue, const float* pink, int64_t n, int nearest,
+                       int chest_wall, float* g)
+{
+    const orc_plan& p = *P;
+    const int64_t proj = int64_t(p.N_u1) * p.N_u0;
+    const int64_t obl = int64_t(p.N_v0) * proj;
+    int64_t accepted = 0;
+    for (int64_t e = 0; e < n; e++) {
+        float co[4];
+        if (!event_coords(p, blue + 3 * e, pink + 3 * e, co))
+            continue;
+        const float v1 = co[0], v0 = co[1], u0 = co[3];
+        float u1 = co[2];
+        if (chest_wall)
+            u1 -= p.H;
+        accepted++;
+        const float iv1_val = (v1 - p.v1_0) / p.T_v1;
+        const int iv1 = int(double(iv1_val) + 0.5);
+        if (iv1 < 0 || iv1 >= p.N_v1)
+            continue;
+        const int iv0 = int(double((v0 - p.v0_0) / p.T_v0) + 0.5);
+        if (iv0 < 0 || iv0 >= p.N_v0)
+            continue;
+        float* pr = g + int64_t(iv1) * obl + int64_t(iv0) * proj;
+        const float w = 1.0f;
+        if (nearest) {
+            const int a = int(double((u1 - p.u1_0) / p.T_u1) + 0.5);
+            const int q = int(double((u0 - p.u0_0) / p.T_u0) + 0.5);
+            if (a >= 0 && a < p.N_u1 && q >= 0 && q < p.N_u0)
+                pr[int64_t(a) * p.N_u0 + q] += w;
+            continue;
+        }
+        int lo1, hi1, lo0, hi0;
+        float du1, du0;
+        const float iu1 = (u1 - p.u1_0) / p.T_u1;
+        if (iu1 <= 0.0f) { lo1 = hi1 = 0; du1 = 0.0f; }
+        else if (iu1 >= float(p.N_u1 - 1)) { lo1 = hi1 = p.N_u1 - 1; du1 = 0.0f; }
+        else { lo1 = int(iu1); hi1 = lo1 + 1; du1 = iu1 - float(lo1); }
+        const float iu0 = (u0 - p.u0_0) / p.T_u0;
+        if (iu0 <= 0.0f) { lo0 = hi0 = 0; du0 = 0.0f; }
+        else if (iu0 >= float(p.N_u0 - 1)) { lo0 = hi0 = p.N_u0 - 1; du0 = 0.0f; }
+        else { lo0 = int(iu0); hi0 = lo0 + 1; du0 = iu0 - float(lo0); }
+        auto add = [&](int a, int q, double wt) {
+            float& x = pr[int64_t(a) * p.N_u0 + q];
+            x = float(double(x) + wt);
+        };
+        add(lo1, lo0, (1.0 - double(du1)) * (1.0 - double(du0)) * double(w));
+        add(hi1, lo0, double(du1) * (1.0 - double(du0)) * double(w));
+        add(lo1, hi0, (1.0 - double(du1)) * double(du0) * double(w));
+        add(hi1, hi0, double(du1) * double(du0) * double(w));
+    }
+    return accepted;
+}
+
 }  // extern "C"

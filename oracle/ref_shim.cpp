@@ -24,6 +24,25 @@ int ref_plan_init(float psi, float D, float L, float H, float v_m0, float v_m1, 
     return 1;
 }
 
+// The reference's own event geometry (src/planogram.cpp:113-141) for one coincidence: returns 1 when
+// accepted, with out = {v1, v0, u1, u0} (u from the blue end point, no origin shift).
+int ref_event_coords(float psi, float D, float L, float H, float v_m0, float v_m1, float T, const float* blue,
+                     const float* pink, float* out)
+{
+    planogram p;
+    if (!p.init(psi * PI / 180.0, 0.5 * D, 0.5 * L, 0.5 * H, v_m0, v_m1, T))
+        return -1;
+    p.data = nullptr;
+    float r[3] = {blue[0] - pink[0], blue[1] - pink[1], blue[2] - pink[2]};
+    float pos[3] = {blue[0], blue[1], blue[2]};
+    float v1 = 0, v0 = 0, u1 = 0, u0 = 0;
+    if (!p.is_within_acceptable_angle(r, v1, v0))
+        return 0;
+    p.get_u_coords(pos, v1, v0, u1, u0);
+    out[0] = v1; out[1] = v0; out[2] = u1; out[3] = u0;
+    return 1;
+}
+
 // which: 0 multiply, 1 divide, 2 rdivide (y = x / y semantics of rdivide3D), 3 add
 int ref_binary3d(float* x, float* y, int N1, int N2, int N3, int which)
 {

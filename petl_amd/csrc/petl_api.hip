@@ -13,6 +13,7 @@
 #include "petl_common.h"
 #include "petl_model.h"
 #include "mlem_job.h"
+#include "binning.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
 #include "../../include/petl_c_interface.h"
@@ -157,6 +158,91 @@ bool run_mlem(Model& m, float** g, float* f, int numIter, float delta, float bet
         job.iterate(it, s);
     job.fetch(f, loglik, s);
     PETL_HIP_CHECK(hipStreamSynchronize(s));
+    return true;
+}
+
+// ---- list-mode binning (src/petl.cpp:65-79, src/binning.cpp) ----
+// Adds n coincidences staged on the device to the model's device planograms; counts accepted events.
+void bin_staged(Model& m, const float* d_blue, const float* d_pink, int64_t n, float* const* d_g, bool nearest,
+                bool chest_wall, unsigned long long* d_count, hipStream_t s)
+{
+    for (size_t i = 0; i < m.plans.size(); i++)
+        bin_events_device(bin_geom(m.plans[i], nearest, chest_wall), d_blue, d_pink, n, d_g[i], m.dev.bin,
+                          d_count + i, s);
+}
+
+constexpr int64_t BIN_CHUNK = int64_t(1) << 22;  // coincidences staged per upload
+
+// Source of coincidences for run_bin: a list-mode file or host arrays.
+struct EventSource {
+    ListModeReader* file = nullptr;
+    const float* blue = nullptr;
+    const float* pink = nullptr;
+    int64_t n = 0, pos = 0;
+    int64_t next(float* b, float* p, int64_t max)
+    {
+        if (file)
+            return file->next(b, p, max);
+        const int64_t k = std::min(max, n - pos);
+        std::memcpy(b, blue + 3 * pos, size_t(k) * 12);
+        std::memcpy(p, pink + 3 * pos, size_t(k) * 12);
+        pos += k;
+        return k;
+    }
+};
+
+// Bins every coincidence of `src` into the host planograms g (zeroed first when `reset`, as PETL::bin
+// does through planogram::set_constant(0) in src/binning.cpp:58).  accepted: optional host counts.
+bool run_bin(Model& m, float** g, EventSource& src, bool reset, bool nearest, bool chest_wall, int64_t* accepted,
+             int64_t* total_events)
+{
+    hipStream_t s = m.stream();
+    DeviceState& D = m.dev;
+    const size_t nP = m.plans.size();
+    ensure_plan_bufs(m, D.g);
+    std::vector<float*> dg = dev_ptrs(D.g);
+    for (size_t i = 0; i < nP; i++) {
+        const size_t nb = size_t(m.plans[i].elems()) * sizeof(float);
+        if (reset)
+            launch_fill(dg[i], 0.0f, m.plans[i].elems(), s);
+        else
+            h2d(dg[i], g[i], nb, s);
+    }
+    D.events.ensure(size_t(BIN_CHUNK) * 24);
+    D.counter.ensure(nP * sizeof(unsigned long long));
+    unsigned long long* d_count = static_cast<unsigned long long*>(D.counter.p);
+    PETL_HIP_CHECK(hipMemsetAsync(d_count, 0, nP * sizeof(unsigned long long), s));
+    float* hbuf = nullptr;
+    PETL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(BIN_CHUNK) * 24));
+    int64_t total = 0;
+    try {
+        float* d_blue = D.events.f();
+        float* d_pink = d_blue + 3 * BIN_CHUNK;
+        for (;;) {
+            const int64_t n = src.next(hbuf, hbuf + 3 * BIN_CHUNK, BIN_CHUNK);
+            if (n <= 0)
+                break;
+            h2d(d_blue, hbuf, size_t(n) * 12, s);
+            h2d(d_pink, hbuf + 3 * BIN_CHUNK, size_t(n) * 12, s);
+            bin_staged(m, d_blue, d_pink, n, dg.data(), nearest, chest_wall, d_count, s);
+            PETL_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer is refilled next
+            total += n;
+        }
+        std::vector<unsigned long long> cnt(nP);
+        d2h(cnt.data(), d_count, nP * sizeof(unsigned long long), s);
+        for (size_t i = 0; i < nP; i++)
+            d2h(g[i], dg[i], size_t(m.plans[i].elems()) * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        if (accepted)
+            for (size_t i = 0; i < nP; i++)
+                accepted[i] = int64_t(cnt[i]);
+    } catch (...) {
+        (void)hipHostFree(hbuf);
+        throw;
+    }
+    (void)hipHostFree(hbuf);
+    if (total_events)
+        *total_events = total;
     return true;
 }
 
@@ -549,7 +635,7 @@ bool relativeDifferences_gradient(float* f, float* Df, int N_1, int N_2, int N_3
 }
 
 // ---- outside the projector / MLEM path ----
-bool bin(float**, char*) { return unsupported("bin"); }
+
 bool simulate_scatter(float**, float*) { return unsupported("simulate_scatter"); }
 bool FBP(float**, float*) { return unsupported("FBP"); }
 bool PFDR(float**, float**) { return unsupported("PFDR"); }
@@ -663,6 +749,72 @@ bool backproject_gpu_ex(void** d_g, float* d_f, int storage, float g_scale, void
     return guarded("backproject_gpu_ex", [&] {
         hipStream_t s = static_cast<hipStream_t>(stream);
         m->back_st(storage, d_g, d_f, m->back_mode, s, 0, 1, g_scale);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+// src/petl.cpp:65-79: zero the planograms, then bin the file with the reference's defaults (millimetres,
+// chest-wall origin, bilinear u bins since parameters::binningType defaults to PRIMARY)
+bool bin(float** g, char* file_name)
+{
+    Model* m = M();
+    if (g == nullptr || file_name == nullptr || file_name[0] == 0 || !m->geometryDefined())
+        return false;
+    for (size_t i = 0; i < m->plans.size(); i++)
+        if (!g[i])
+            return false;
+    return guarded("bin", [&] {
+        ListModeReader rd(file_name);
+        if (!rd.ok()) {
+            printf("PETL bin: list mode data file %s not found\n", file_name);
+            return false;
+        }
+        EventSource src;
+        src.file = &rd;
+        int64_t total = 0;
+        const bool ok = run_bin(*m, g, src, true, false, true, nullptr, &total);
+        printf("PETL bin: binned %lld coincidences\n", (long long)total);
+        return ok;
+    });
+}
+
+bool bin_events(const float* blue, const float* pink, long long n, float** g, int nearest, int chest_wall,
+                long long* accepted)
+{
+    Model* m = M();
+    if (!blue || !pink || n < 0 || !g || !m->geometryDefined())
+        return false;
+    for (size_t i = 0; i < m->plans.size(); i++)
+        if (!g[i])
+            return false;
+    return guarded("bin_events", [&] {
+        EventSource src;
+        src.blue = blue;
+        src.pink = pink;
+        src.n = n;
+        std::vector<int64_t> acc(m->plans.size());
+        const bool ok = run_bin(*m, g, src, false, nearest != 0, chest_wall != 0, acc.data(), nullptr);
+        if (accepted)
+            for (size_t i = 0; i < acc.size(); i++)
+                accepted[i] = acc[i];
+        return ok;
+    });
+}
+
+bool bin_events_gpu(const float* d_blue, const float* d_pink, long long n, float** d_g, int nearest, int chest_wall,
+                    void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_blue || !d_pink || n < 0 || !d_g || !m->geometryDefined())
+        return false;
+    return guarded("bin_events_gpu", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        DeviceState& D = m->dev;
+        D.counter.ensure(m->plans.size() * sizeof(unsigned long long));
+        bin_staged(*m, d_blue, d_pink, n, d_g, nearest != 0, chest_wall != 0,
+                   static_cast<unsigned long long*>(D.counter.p), s);
         if (sync)
             PETL_HIP_CHECK(hipStreamSynchronize(s));
         return true;

@@ -36,6 +36,11 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// device scratch of the list-mode binning (sort keys and weights, sort workspace), grown on demand
+struct BinScratch {
+    DevBuf keys[2], vals[2], tmp;
+};
+
 struct DeviceState {
     hipStream_t stream = nullptr;
     uint64_t built_version = 0;                   // geometry version the fast plans were made for
@@ -45,6 +50,8 @@ struct DeviceState {
     std::vector<DevBuf> g, y, aux, resp;          // planogram-sized
     DevBuf h2;                                    // fast back projector workspace
     DevBuf red;                                   // reduction scratch
+    BinScratch bin;                               // list-mode binning
+    DevBuf events, counter;                       // staged coincidences, accepted-event counters
     ~DeviceState();
 };
 

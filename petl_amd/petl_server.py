@@ -70,6 +70,12 @@ _SIGNATURES = {
     "MLEM_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(_c_float_p),
                                  ctypes.POINTER(_c_float_p), ctypes.c_void_p]),
+    "bin": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_char_p]),
+    "bin_events": (ctypes.c_bool, [_f32, _f32, ctypes.c_longlong, ctypes.POINTER(_c_float_p), ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]),
+    "bin_events_gpu": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                       ctypes.POINTER(_c_float_p), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_bool]),
     "project_gpu_ex": (ctypes.c_bool, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_bool]),
     "backproject_gpu_ex": (ctypes.c_bool, [ctypes.POINTER(ctypes.c_void_p), _c_float_p, ctypes.c_int, ctypes.c_float,
@@ -98,6 +104,20 @@ def library_path():
     return os.environ.get("PETL_LIBRARY") or os.path.join(_HERE, "libpetl.so")
 
 
+def _share_torch_hip_runtime():
+    """PyTorch's ROCm wheel carries its own HIP runtime (torch/lib/libamdhip64.so, soname libamdhip64.so.7).
+    Loading it before libpetl.so makes libpetl.so bind to that same runtime, so a process that also uses
+    torch holds one HIP/HSA runtime whichever of the two it loads first (two runtimes cannot both
+    initialise the GPU).  Without torch installed, the system ROCm runtime is used."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    hip = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.isfile(hip):
+        ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
 def load_library(path=None):
     """Loads libpetl.so (once per process) and declares every signature."""
     global _lib
@@ -106,6 +126,7 @@ def load_library(path=None):
     path = path or library_path()
     if not os.path.isfile(path):
         raise RuntimeError(f"PETL library not found at {path}; build it with `python -m petl_amd.build`")
+    _share_torch_hip_runtime()
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
@@ -367,6 +388,49 @@ class PETLserver:
         if not ok:
             raise TypeError('backproject failed')
         return f
+
+    # ------------------------------------------------------------------ list-mode binning
+    def bin(self, file_name, g=None):
+        """Bins list mode data (src/petl_server.py:653-677): the planograms are zeroed, then the file's
+        coincidences are binned (".lst" or the acquisition layout; millimetres, chest-wall origin,
+        bilinear u bins).  Returns g."""
+        if not self.all_defined():
+            raise ValueError('Error: volume and PET geometry must be defined')
+        if g is None:
+            g = self.allocate_projections()
+        file_name = bytes(str(file_name), 'ascii')
+        self.set_model()
+        if not self.libprojectors.bin(self.floatstarstar(g), file_name):
+            pass  # the reference ignores a failed bin (src/petl_server.py:674-676)
+        return g
+
+    def bin_events(self, blue, pink, g=None, nearest=False, chest_wall=True):
+        """Adds coincidences (blue, pink: (n, 3) float32 end points, mm) to the planograms g (zeros when
+        None).  Returns (g, coincidences accepted per planogram)."""
+        if not self.geometry_defined():
+            raise ValueError('Error: PET geometry must be defined')
+        blue = np.ascontiguousarray(blue, dtype=np.float32).reshape(-1, 3)
+        pink = np.ascontiguousarray(pink, dtype=np.float32).reshape(-1, 3)
+        if blue.shape != pink.shape:
+            raise ValueError('blue and pink must have the same shape')
+        if g is None:
+            g = self.allocate_projections()
+        acc = (ctypes.c_longlong * self.get_numPlanograms())()
+        self.set_model()
+        if not self.libprojectors.bin_events(blue, pink, len(blue), self.floatstarstar(g), int(bool(nearest)),
+                                             int(bool(chest_wall)), acc):
+            raise TypeError('bin_events failed')
+        return g, [int(x) for x in acc]
+
+    def bin_events_gpu(self, blue, pink, g, nearest=False, chest_wall=True, stream=None, sync=True):
+        """bin_events on device tensors: blue, pink (n, 3) float32, g the device planograms (added to)."""
+        self.set_model()
+        s = None if stream is None else ctypes.c_void_p(stream)
+        if not self.libprojectors.bin_events_gpu(ctypes.c_void_p(blue.data_ptr()), ctypes.c_void_p(pink.data_ptr()),
+                                                 int(blue.shape[0]), self._dev_ptrs(g), int(bool(nearest)),
+                                                 int(bool(chest_wall)), s, sync):
+            raise TypeError('bin_events_gpu failed')
+        return g
 
     def sensitivity(self, Pstar1=None):
         """P*1, the back projection of all-ones planograms (src/petl_server.py:1257-1273)."""

@@ -115,11 +115,11 @@ def test_requests_join_while_others_run():
     """A long request is running when short ones arrive; they are admitted beside it and finish first."""
     y_long = data_for("p64", 7)
     y_short = data_for("p32", 8)
-    want_long = alone("p64", y_long, 30)
+    want_long = alone("p64", y_long, 300)   # ~1 ms per iteration: runs for a few hundred ms
     want_short = alone("p32", y_short, 3)
     with ReconService(max_slots=4) as svc:
         t0 = time.perf_counter()
-        rid_long = svc.submit(**kw("p64", y_long, 30))
+        rid_long = svc.submit(**kw("p64", y_long, 300))
         done = {}
 
         def wait_long():
@@ -128,7 +128,7 @@ def test_requests_join_while_others_run():
 
         th = threading.Thread(target=wait_long)
         th.start()
-        time.sleep(0.05)
+        time.sleep(0.02)
         short_ids = [svc.submit(**kw("p32", y_short, 3)) for _ in range(3)]
         shorts = [svc.wait(i) for i in short_ids]
         t_short = time.perf_counter() - t0
