@@ -7,8 +7,9 @@
 //
 // The reference adds every contribution to its bin in event order.  Float addition is not associative,
 // so to give the same bits this build keeps that order instead of using atomics: each coincidence
-// writes up to four (bin, weight) pairs at position 4 * event + corner, a stable radix sort groups them by
-// bin keeping that order, and one thread per bin adds its run to the bin's value in sequence.
+// writes its (bin, weight) pairs at position 4 * event + corner (bilinear; event for nearest), a stable
+// radix sort groups them by bin keeping that order, and one thread per bin adds its run to the bin's value
+// in sequence.
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
 #include <cstdio>
@@ -60,11 +61,13 @@ __device__ __forceinline__ void split(float x, int n, int& lo, int& hi, float& d
     }
 }
 
+// thread per coincidence; SLOTS = 4 (bilinear: one slot per corner) or 1 (nearest)
+template <int SLOTS>
 __global__ void __launch_bounds__(BT) bin_contrib_kernel(const float* __restrict__ blue,
                                                          const float* __restrict__ pink, int64_t n, BinGeom B,
                                                          uint32_t no_bin, uint32_t* __restrict__ keys,
                                                          double* __restrict__ vals,
-                                                         unsigned long long* __restrict__ accepted)
+                                                         unsigned int* __restrict__ block_accepted)
 {
 #pragma clang fp contract(off)
     const int64_t e = int64_t(blockIdx.x) * BT + threadIdx.x;
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(BT) bin_contrib_kernel(const float* __restrict
             const int iv0 = int(double((v0 - P.v0_0) / P.Tv0) + 0.5);
             if (iv1 >= 0 && iv1 < P.Nv1 && iv0 >= 0 && iv0 < P.Nv0) {
                 const uint32_t base = (uint32_t(iv1) * uint32_t(P.Nv0) + uint32_t(iv0)) * uint32_t(P.Nu1 * P.Nu0);
-                if (B.nearest) {
+                if (SLOTS == 1) {
                     const int a = int(double((u1 - P.u1_0) / P.Tu1) + 0.5);
                     const int q = int(double((u0 - P.u0_0) / P.Tu0) + 0.5);
                     if (a >= 0 && a < P.Nu1 && q >= 0 && q < P.Nu0) {
@@ -107,15 +110,39 @@ __global__ void __launch_bounds__(BT) bin_contrib_kernel(const float* __restrict
                 }
             }
         }
-        for (int c = 0; c < 4; c++) {
-            keys[4 * e + c] = key[c];
-            vals[4 * e + c] = w[c];
+        if (SLOTS == 1) {
+            keys[e] = key[0];
+            vals[e] = w[0];
+        } else {
+            // 16-byte and two 16-byte stores per coincidence (slots 4e .. 4e+3 are contiguous and aligned)
+            reinterpret_cast<uint4*>(keys)[e] = make_uint4(key[0], key[1], key[2], key[3]);
+            reinterpret_cast<double2*>(vals)[2 * e] = make_double2(w[0], w[1]);
+            reinterpret_cast<double2*>(vals)[2 * e + 1] = make_double2(w[2], w[3]);
         }
     }
-    // accepted coincidences (src/binning.cpp:643): one atomic per wave
-    const unsigned long long cnt = __popcll(__ballot(ok));
-    if ((threadIdx.x & 63) == 0 && cnt)
-        atomicAdd(accepted, cnt);
+    // accepted coincidences (src/binning.cpp:643), one count per block
+    const int cnt = __syncthreads_count(ok);
+    if (threadIdx.x == 0)
+        block_accepted[blockIdx.x] = unsigned(cnt);
+}
+
+// adds the per-block counts to *accepted in a fixed order (one block)
+__global__ void __launch_bounds__(BT) bin_count_kernel(const unsigned int* __restrict__ block_accepted, int nblocks,
+                                                       unsigned long long* __restrict__ accepted)
+{
+    __shared__ unsigned long long red[BT];
+    unsigned long long acc = 0;
+    for (int i = threadIdx.x; i < nblocks; i += BT)
+        acc += block_accepted[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = BT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *accepted += red[0];
 }
 
 // one thread per run of equal bins: add the run to the bin in order, each sum rounded to float
@@ -165,7 +192,10 @@ void bin_events_device(const BinGeom& B, const float* blue, const float* pink, i
     if (G >= uint64_t(0xffffffffu))
         throw hip_error("binning: planogram too large for 32-bit bin keys");
     const uint32_t no_bin = uint32_t(G);  // sorts after every real bin
-    const int64_t m = 4 * n;
+    const int slots = B.nearest ? 1 : 4;
+    const int64_t m = slots * n;
+    const int nblocks = ceil_div(n, BT);
+    W.counts.ensure(size_t(nblocks) * sizeof(unsigned int));
     W.keys[0].ensure(size_t(m) * sizeof(uint32_t));
     W.keys[1].ensure(size_t(m) * sizeof(uint32_t));
     W.vals[0].ensure(size_t(m) * sizeof(double));
@@ -174,8 +204,15 @@ void bin_events_device(const BinGeom& B, const float* blue, const float* pink, i
     uint32_t* k1 = static_cast<uint32_t*>(W.keys[1].p);
     double* v0 = static_cast<double*>(W.vals[0].p);
     double* v1 = static_cast<double*>(W.vals[1].p);
-    hipLaunchKernelGGL(bin_contrib_kernel, dim3(ceil_div(n, BT)), dim3(BT), 0, s, blue, pink, n, B, no_bin, k0,
-                       v0, accepted);
+    unsigned int* counts = static_cast<unsigned int*>(W.counts.p);
+    if (slots == 1)
+        hipLaunchKernelGGL(bin_contrib_kernel<1>, dim3(nblocks), dim3(BT), 0, s, blue, pink, n, B, no_bin, k0, v0,
+                           counts);
+    else
+        hipLaunchKernelGGL(bin_contrib_kernel<4>, dim3(nblocks), dim3(BT), 0, s, blue, pink, n, B, no_bin, k0, v0,
+                           counts);
+    PETL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_count_kernel, dim3(1), dim3(BT), 0, s, counts, nblocks, accepted);
     PETL_LAUNCH_CHECK();
     const int bits = key_bits(G);  // keys are <= G
     size_t tmp = 0;

@@ -38,7 +38,7 @@ struct DevBuf {
 
 // device scratch of the list-mode binning (sort keys and weights, sort workspace), grown on demand
 struct BinScratch {
-    DevBuf keys[2], vals[2], tmp;
+    DevBuf keys[2], vals[2], tmp, counts;
 };
 
 struct DeviceState {
