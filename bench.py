@@ -40,10 +40,10 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (reference for HBM byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-rates", default="4,20",
-                    help="request arrival rates (req/s), comma separated: one below the service's capacity "
-                         "and one above it")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rates", default="5,20,50",
+                    help="request arrival rates (req/s), comma separated: below, near and above the service's "
+                         "capacity")
+    ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-slots", type=int, default=8)
     return ap.parse_args()
 
@@ -152,7 +152,9 @@ def serving_run(datasets, lam, nreq, slots, seed):
     lat = np.array(lat)
     return {"recons_per_s": nreq / elapsed, "p50_s": float(np.percentile(lat, 50)),
             "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "peak_on_device": st["peak_active"],
-            "failed": st["failed"]}
+            "failed": st["failed"],
+            "host_s": {k: round(st[k], 4) for k in ("admit_alloc_s", "admit_setup_s", "admit_capture_s", "release_s")},
+            "pool_hits": st["pool_hits"]}
 
 
 def main():

@@ -221,8 +221,11 @@ PETL_EXTERN double petl_service_request_bytes(const petl_recon_request* request)
  * 3 doubles): seconds from submission to admission, from admission to completion, and in total.
  * Returns false for an unknown id or a failed request. */
 PETL_EXTERN bool petl_service_wait(void* service, int id, float* f, double* times);
-/* stats (6 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
- * requests currently queued. */
+/* stats (11 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
+ * requests currently queued, the host seconds spent so far admitting requests (allocation, data upload
+ * with sensitivities, graph capture) and freeing resources, and how many requests reused the pooled
+ * resources (model, buffers, captured iteration) of a finished request with the same geometry and MLEM
+ * parameters.  Pooled memory counts against the budget until a new request needs it. */
 PETL_EXTERN bool petl_service_stats(void* service, double* stats);
 /* Finishes every queued request, then frees the service. */
 PETL_EXTERN void petl_service_destroy(void* service);

@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 #include "mlem_job.h"
@@ -53,7 +54,25 @@ struct Slot {
     float* pinned = nullptr;               // image staging (page-locked)
     int launched = 0, finished = 0;
     bool fetching = false;
+    std::string sig;                       // geometry and MLEM parameters the resources were built for
+    size_t bytes = 0;                      // device bytes the resources hold
 };
+
+// Requests with equal signatures can run on the same model, buffers and captured iteration.
+std::string signature(const petl_recon_request& r)
+{
+    char buf[512];
+    int n = std::snprintf(buf, sizeof(buf), "%d|%a|%a|%a|%a|%a|%a|%d|%d|%d|%a|%a|%d|%d|%a|%a|%a", r.num_planograms,
+                          double(r.D), double(r.L), double(r.H), double(r.T), double(r.v_m0), double(r.v_m1), r.numX,
+                          r.numY, r.numZ, double(r.voxelWidth), double(r.voxelHeight), r.numSubsets, r.prior,
+                          double(r.delta), double(r.beta), double(r.p));
+    std::string sig(buf, size_t(std::max(0, n)));
+    for (int i = 0; i < r.num_planograms && i < PETL_MAX_PLANOGRAMS; i++) {
+        std::snprintf(buf, sizeof(buf), "|%a", double(r.psi[i]));
+        sig += buf;
+    }
+    return sig;
+}
 
 std::unique_ptr<Model> make_model(const petl_recon_request& r)
 {
@@ -190,6 +209,11 @@ public:
         out[3] = double(peak_reserved_);
         out[4] = double(budget_);
         out[5] = double(pending_.size());
+        out[6] = t_alloc_;
+        out[7] = t_setup_;
+        out[8] = t_capture_;
+        out[9] = t_release_;
+        out[10] = double(pool_hits_);
     }
 
 private:
@@ -214,6 +238,12 @@ private:
                     peak_active_ = std::max(peak_active_, active_);
                     peak_reserved_ = std::max(peak_reserved_, reserved_);
                 } else if (stop_ && pending_.empty() && active_ == 0 && retire.empty()) {
+                    std::vector<Slot> pool;
+                    pool.swap(pool_);
+                    pooled_ = 0;
+                    lk.unlock();
+                    for (Slot& p : pool)
+                        destroy_slot(p);
                     return;
                 }
             }
@@ -235,25 +265,106 @@ private:
         return -1;
     }
 
+    // A finished request's resources go to the pool (its memory stays counted in the budget as pooled),
+    // unless they failed or the pool is full; freeing device memory waits for the whole device, so it is
+    // done only when a new request needs the room.
     void release(Slot& s)
     {
         const size_t bytes = s.req->bytes;
-        destroy_slot(s);
+        const bool keep = s.exec != nullptr && !s.req->failed && int(pool_.size()) < max_slots_;
+        const Clock::time_point t0 = Clock::now();
+        if (!keep)
+            destroy_slot(s);
+        const double dt = seconds(t0, Clock::now());
         {
             std::lock_guard<std::mutex> lk(mu_);
             reserved_ -= bytes;
             active_--;
+            t_release_ += dt;
+            if (keep) {
+                s.bytes = bytes;
+                pooled_ += bytes;
+                s.req.reset();
+                pool_.push_back(std::move(s));
+            }
         }
         cv_admit_.notify_all();
+    }
+
+    // frees pooled resources, oldest first, until `need` more bytes fit the budget beside the pool
+    void evict_for(size_t need)
+    {
+        for (;;) {
+            Slot victim;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (pool_.empty() || reserved_ + pooled_ <= budget_)
+                    return;
+                victim = std::move(pool_.front());
+                pool_.erase(pool_.begin());
+                pooled_ -= victim.bytes;
+            }
+            (void)need;
+            const Clock::time_point t0 = Clock::now();
+            destroy_slot(victim);
+            std::lock_guard<std::mutex> lk(mu_);
+            t_release_ += seconds(t0, Clock::now());
+        }
+    }
+
+    // takes a pooled slot built for `sig`, if any
+    bool take_pooled(const std::string& sig, Slot& out)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t i = 0; i < pool_.size(); i++)
+            if (pool_[i].sig == sig) {
+                out = std::move(pool_[i]);
+                pool_.erase(pool_.begin() + long(i));
+                pooled_ -= out.bytes;
+                pool_hits_++;
+                return true;
+            }
+        return false;
     }
 
     void admit(const std::shared_ptr<Request>& req)
     {
         Slot s;
-        s.req = req;
         req->t_admit = Clock::now();
+        const std::string sig = signature(req->r);
+        if (take_pooled(sig, s)) {
+            s.req = req;
+            s.launched = s.finished = 0;
+            s.fetching = false;
+            try {
+                const Clock::time_point t1 = Clock::now();
+                const int64_t V = s.model->vol_elems();
+                std::fill(s.pinned, s.pinned + V, 1.0f);
+                s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
+                PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
+                std::lock_guard<std::mutex> lk(mu_);
+                t_setup_ += seconds(t1, Clock::now());
+            } catch (const std::exception& e) {
+                fprintf(stderr, "PETL service: request %d failed at admission: %s\n", req->id, e.what());
+                (void)hipStreamSynchronize(s.stream);
+                (void)hipGetLastError();
+                finish(req, true);
+                release(s);
+                return;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                ready_.push_back(std::move(s));
+            }
+            cv_launch_.notify_all();
+            return;
+        }
+        evict_for(req->bytes);
+        s.req = req;
+        s.sig = sig;
         try {
             const petl_recon_request& r = req->r;
+            const Clock::time_point t0 = Clock::now();
             s.model = make_model(r);
             if (!s.model)
                 throw std::runtime_error("invalid request");
@@ -265,12 +376,19 @@ private:
             const int64_t V = s.model->vol_elems();
             PETL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s.pinned), size_t(V) * sizeof(float)));
             std::fill(s.pinned, s.pinned + V, 1.0f);  // initial estimate: ones
+            const Clock::time_point t1 = Clock::now();
             s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
             PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
+            const Clock::time_point t2 = Clock::now();
             PETL_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
             s.job->iterate(0, s.stream);
             PETL_HIP_CHECK(hipStreamEndCapture(s.stream, &s.graph));
             PETL_HIP_CHECK(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+            const Clock::time_point t3 = Clock::now();
+            std::lock_guard<std::mutex> lk(mu_);
+            t_alloc_ += seconds(t0, t1);
+            t_setup_ += seconds(t1, t2);
+            t_capture_ += seconds(t2, t3);
         } catch (const std::exception& e) {
             fprintf(stderr, "PETL service: request %d failed at admission: %s\n", req->id, e.what());
             if (s.stream) {
@@ -415,6 +533,10 @@ private:
     size_t reserved_ = 0, peak_reserved_ = 0;
     int active_ = 0, peak_active_ = 0;
     long completed_ = 0, failed_ = 0;
+    double t_alloc_ = 0, t_setup_ = 0, t_capture_ = 0, t_release_ = 0;  // host seconds spent per phase
+    std::vector<Slot> pool_;  // resources of finished requests, oldest first
+    size_t pooled_ = 0;       // device bytes they hold
+    long pool_hits_ = 0;
     std::thread admit_thread_, launch_thread_;
 };
 

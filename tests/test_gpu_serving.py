@@ -148,3 +148,19 @@ def test_oversized_request_is_rejected():
     with ReconService(hbm_budget_bytes=0.5 * need, max_slots=2) as svc:
         with pytest.raises(ValueError):
             svc.submit(**kw("p32", y, 2))
+
+
+def test_pooled_resources_are_reused_bit_exact():
+    """Requests with the same geometry and MLEM parameters reuse a finished request's model, buffers and
+    captured iteration; new data and iteration counts still give the images MLEM gives alone."""
+    cases = [("p48", 21, 3), ("p48", 22, 6), ("p48", 23, 2)]
+    want = [alone(n, data_for(n, sd), it) for (n, sd, it) in cases]
+    with ReconService(max_slots=4) as svc:
+        got = []
+        for (n, sd, it) in cases:
+            rid = svc.submit(**kw(n, data_for(n, sd), it))
+            got.append(svc.wait(rid)[0])
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["pool_hits"] == 2
