@@ -109,7 +109,7 @@ def serving_datasets(torch, dev, seed):
     return out
 
 
-def serving_run(datasets, lam, nreq, slots, seed):
+def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
     """Poisson arrivals at rate lam; each request = (panel size, dataset, iterations U{5..20}).  Returns
     recons/s over the run and latency percentiles (submission to completion)."""
     import threading
@@ -144,7 +144,8 @@ def serving_run(datasets, lam, nreq, slots, seed):
             geo = GEOMETRIES[name]
             ys = [y.numpy() for y in datasets[name][which][1]]
             ids[i] = svc.submit(psis=geo["psis"], D=geo["D"], L=geo["L"], H=geo["H"], T=geo["T"], n=geo["n"],
-                                voxel=geo["voxel"], y=ys, num_iter=iters)
+                                voxel=geo["voxel"], y=ys, num_iter=iters, storage=storage,
+                                image_storage="f32")
             ready[i].set()
         th.join()
         elapsed = time.perf_counter() - t0
@@ -333,13 +334,15 @@ def main():
         runs = []
         for rate in [float(x) for x in a.serve_rates.split(",") if x]:
             mixed = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501)
+            mixed16 = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, storage="bf16")
             single = serving_run(data, rate, a.serve_requests, 1, seed=501)
             runs.append({"arrival_rate_req_s": rate, "continuous_batching": dict(mixed, slots=a.serve_slots),
+                         "continuous_batching_bf16_planograms": dict(mixed16, slots=a.serve_slots),
                          "one_at_a_time": dict(single, slots=1)})
         out["serving"] = {
             "workload": "Poisson arrivals; panels uniform over 48/64/96/128 crystals (2 orientations, D = 200 mm, "
                         "matched n^3 volumes at 2 mm); data = Poisson at counts log-uniform in [1e5, 1e8]; "
-                        "MLEM iterations uniform in 5..20; fp32",
+                        "MLEM iterations uniform in 5..20; fp32 (and bf16 planograms with an fp32 image)",
             "requests": a.serve_requests,
             "runs": runs,
         }

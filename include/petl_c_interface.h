@@ -207,6 +207,8 @@ typedef struct petl_recon_request {
     int numSubsets;                    /* OSEM subsets (1 = MLEM) */
     int prior;                         /* 0 none, 1 relative differences, 2 p-Huber TV (one-step-late) */
     float delta, beta, p;
+    int storage;                       /* planograms on the device: PETL_STORE_F32 / _BF16 / _F16 */
+    int image_storage;                 /* image on the device: equal to storage, or PETL_STORE_F32 */
 } petl_recon_request;
 
 /* hbm_budget_bytes: device memory the service may hold for requests; max_slots: requests on the device

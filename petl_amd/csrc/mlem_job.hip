@@ -171,23 +171,32 @@ void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
     const int st = prm_.storage;
     const size_t nP = m.plans.size();
     std::vector<float*> aux = dev_ptrs(D.aux);
+    const int sti = prm_.image_storage;
+    // the data's largest count (binary16 data scale) and total (binary16 image scale)
     float ymax = 0.0f;
     double ysum = 0.0;
-    for (size_t i = 0; i < nP; i++) {
-        const int64_t n = m.plans[i].elems();
-        for (int64_t k = 0; k < n; k++) {
-            ymax = std::max(ymax, std::fabs(g[i][k]));
-            ysum += g[i][k];
+    if (st == STORE_F16 || sti == STORE_F16)
+        for (size_t i = 0; i < nP; i++) {
+            const float* gi = g[i];
+            const int64_t n = m.plans[i].elems();
+#pragma omp parallel for reduction(max : ymax) reduction(+ : ysum) schedule(static)
+            for (int64_t k = 0; k < n; k++) {
+                ymax = std::max(ymax, std::fabs(gi[k]));
+                ysum += gi[k];
+            }
         }
-    }
     s_y_ = (st == STORE_F16) ? pow2_scale_for(ymax, 16384.0f) : 1.0f;
-    std::vector<char> tmp;
+    // data: fp32 chunks through a device staging buffer, converted on the device
+    const int64_t chunk = int64_t(1) << 24;
+    D.stage.ensure(size_t(chunk) * sizeof(float));
     for (size_t i = 0; i < nP; i++) {
         const int64_t n = m.plans[i].elems();
-        tmp.resize(size_t(n) * storage_bytes(st));
-        to_storage(st, g[i], tmp.data(), n, 1.0f / s_y_);
-        h2d(D.y[i].p, tmp.data(), tmp.size(), s);
-        PETL_HIP_CHECK(hipStreamSynchronize(s));  // tmp is reused
+        for (int64_t o = 0; o < n; o += chunk) {
+            const int64_t k = std::min(chunk, n - o);
+            h2d(D.stage.p, g[i] + o, size_t(k) * sizeof(float), s);
+            launch_convert_st(st, static_cast<char*>(D.y[i].p) + size_t(o) * storage_bytes(st), D.stage.f(), k,
+                              1.0f / s_y_, s);
+        }
     }
     for (int sub = 0; sub < prm_.nsub; sub++) {
         for (size_t i = 0; i < nP; i++)
@@ -196,7 +205,6 @@ void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
         m.back_st(st, reinterpret_cast<void* const*>(aux.data()), Ss, m.back_mode, s, sub, prm_.nsub, 1.0f);
         launch_sens_fix(Ss, V_, s);
     }
-    const int sti = prm_.image_storage;
     float f0max = 0.0f;
     for (int64_t k = 0; k < V_; k++)
         f0max = std::max(f0max, std::fabs(f0[k]));
@@ -211,9 +219,11 @@ void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
         const double mean = ssum > 0.0 ? ysum / ssum : 1.0;
         s_f_ = pow2_scale_for(std::max(float(1024.0 * mean), f0max), 32768.0f);
     }
-    tmp.resize(size_t(V_) * storage_bytes(sti));
-    to_storage(sti, f0, tmp.data(), V_, 1.0f / s_f_);
-    h2d(D.f.p, tmp.data(), tmp.size(), s);
+    // the initial image (volume-sized); the staging buffer may grow, so the data uploads must be done
+    PETL_HIP_CHECK(hipStreamSynchronize(s));
+    D.stage.ensure(size_t(V_) * sizeof(float));
+    h2d(D.stage.p, f0, size_t(V_) * sizeof(float), s);
+    launch_convert_st(sti, D.f.p, D.stage.f(), V_, 1.0f / s_f_, s);
     PETL_HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -52,6 +52,7 @@ struct DeviceState {
     DevBuf red;                                   // reduction scratch
     BinScratch bin;                               // list-mode binning
     DevBuf events, counter;                       // staged coincidences, accepted-event counters
+    DevBuf stage;                                 // fp32 upload staging for storage-format data
     ~DeviceState();
 };
 

@@ -62,10 +62,11 @@ struct Slot {
 std::string signature(const petl_recon_request& r)
 {
     char buf[512];
-    int n = std::snprintf(buf, sizeof(buf), "%d|%a|%a|%a|%a|%a|%a|%d|%d|%d|%a|%a|%d|%d|%a|%a|%a", r.num_planograms,
-                          double(r.D), double(r.L), double(r.H), double(r.T), double(r.v_m0), double(r.v_m1), r.numX,
-                          r.numY, r.numZ, double(r.voxelWidth), double(r.voxelHeight), r.numSubsets, r.prior,
-                          double(r.delta), double(r.beta), double(r.p));
+    int n = std::snprintf(buf, sizeof(buf), "%d|%a|%a|%a|%a|%a|%a|%d|%d|%d|%a|%a|%d|%d|%a|%a|%a|%d|%d",
+                          r.num_planograms, double(r.D), double(r.L), double(r.H), double(r.T), double(r.v_m0),
+                          double(r.v_m1), r.numX, r.numY, r.numZ, double(r.voxelWidth), double(r.voxelHeight),
+                          r.numSubsets, r.prior, double(r.delta), double(r.beta), double(r.p), r.storage,
+                          r.image_storage);
     std::string sig(buf, size_t(std::max(0, n)));
     for (int i = 0; i < r.num_planograms && i < PETL_MAX_PLANOGRAMS; i++) {
         std::snprintf(buf, sizeof(buf), "|%a", double(r.psi[i]));
@@ -104,6 +105,8 @@ MlemParams params_of(const petl_recon_request& r)
     prm.delta = r.delta;
     prm.beta = r.beta;
     prm.p = r.p;
+    prm.storage = r.storage;
+    prm.image_storage = r.image_storage;
     return prm;
 }
 

@@ -29,6 +29,8 @@ void launch_ratio(float* r, const float* y, const float* q, const float* resp, i
 // the image f in storage format (d and S stay fp32).
 void launch_fill_st(int st, void* x, float v, float scale, int64_t n, hipStream_t s);
 void launch_copy_st(int st, void* dst, const void* src, int64_t n, hipStream_t s);
+// dst (storage format st) = src (fp32) / scale, rounded to nearest even
+void launch_convert_st(int st, void* dst, const float* src, int64_t n, float inv_scale, hipStream_t s);
 void launch_mlem_update_st(int st, void* f, const float* d, const float* S, int64_t n, float f_scale, hipStream_t s);
 void launch_mlem_update_prior_st(int st, void* f_out, const void* f, const float* d, const float* S, int N1, int N2,
                                  int N3, int kind, float delta, float beta, float p, float f_scale, hipStream_t s);

@@ -136,6 +136,14 @@ __global__ void __launch_bounds__(64) mlem_update_prior_st_kernel(typename Stor<
 }
 
 template <int ST>
+__global__ void convert_st_kernel(typename Stor<ST>::T* __restrict__ x, const float* __restrict__ y, int64_t n,
+                                  float inv_scale)
+{
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        x[i] = Stor<ST>::st(y[i], inv_scale);
+}
+
+template <int ST>
 __global__ void copy_st_kernel(typename Stor<ST>::T* __restrict__ x, const typename Stor<ST>::T* __restrict__ y,
                                int64_t n)
 {
@@ -218,6 +226,23 @@ void launch_copy_st(int st, void* dst, const void* src, int64_t n, hipStream_t s
                            static_cast<_Float16*>(dst), static_cast<const _Float16*>(src), n);
         break;
     default: throw hip_error("copy: unknown storage format");
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+void launch_convert_st(int st, void* dst, const float* src, int64_t n, float inv_scale, hipStream_t s)
+{
+    switch (st) {
+    case STORE_F32: launch_copy(static_cast<float*>(dst), src, n, s); return;
+    case STORE_BF16:
+        hipLaunchKernelGGL(convert_st_kernel<STORE_BF16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<uint16_t*>(dst), src, n, 1.0f);
+        break;
+    case STORE_F16:
+        hipLaunchKernelGGL(convert_st_kernel<STORE_F16>, dim3(grid_for(n)), dim3(VT), 0, s,
+                           static_cast<_Float16*>(dst), src, n, inv_scale);
+        break;
+    default: throw hip_error("convert: unknown storage format");
     }
     PETL_LAUNCH_CHECK();
 }

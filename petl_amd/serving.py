@@ -28,7 +28,8 @@ class ReconRequest(ctypes.Structure):
                 ("voxelWidth", ctypes.c_float), ("voxelHeight", ctypes.c_float),
                 ("y", ctypes.POINTER(_c_float_p)),
                 ("numIter", ctypes.c_int), ("numSubsets", ctypes.c_int), ("prior", ctypes.c_int),
-                ("delta", ctypes.c_float), ("beta", ctypes.c_float), ("p", ctypes.c_float)]
+                ("delta", ctypes.c_float), ("beta", ctypes.c_float), ("p", ctypes.c_float),
+                ("storage", ctypes.c_int), ("image_storage", ctypes.c_int)]
 
 
 _PRIORS = {None: 0, "none": 0, "rd": 1, "tv": 2}
@@ -50,10 +51,14 @@ def _declare(lib):
     return lib
 
 
+_STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
+
+
 def make_request(psis, D, L, H, T, n, voxel, y, num_iter, v_m0=None, v_m1=None, num_subsets=1, prior=None,
-                 delta=2.0, beta=0.0, p=1.0, shape=None):
+                 delta=2.0, beta=0.0, p=1.0, shape=None, storage="f32", image_storage=None):
     """Builds the C request and the array keeping the data pointers alive.  Defaults of v_m0/v_m1 follow
-    PETLserver.add_planogram (src/petl_server.py:484-507)."""
+    PETLserver.add_planogram (src/petl_server.py:484-507).  storage / image_storage: device formats of the
+    planograms and the image, as PETLserver.MLEM takes them."""
     psis = tuple(float(x) for x in psis)
     if not 1 <= len(psis) <= MAX_PLANOGRAMS:
         raise ValueError("1 to 8 planograms per request")
@@ -78,6 +83,8 @@ def make_request(psis, D, L, H, T, n, voxel, y, num_iter, v_m0=None, v_m1=None, 
     r.numIter, r.numSubsets = int(num_iter), int(num_subsets)
     r.prior = _PRIORS[prior]
     r.delta, r.beta, r.p = delta, beta, p
+    r.storage = _STORAGE[storage]
+    r.image_storage = _STORAGE[image_storage if image_storage is not None else storage]
     return r, (ys, ptrs)
 
 

@@ -164,3 +164,15 @@ def test_pooled_resources_are_reused_bit_exact():
     for g, w in zip(got, want):
         np.testing.assert_array_equal(g, w)
     assert st["pool_hits"] == 2
+
+
+def test_bf16_storage_requests_match_mlem_storage():
+    y = data_for("p48", 31)
+    want = server_for(GEOS["p48"]).MLEM(y, None, 4, storage="bf16", image_storage="f32")
+    want16 = server_for(GEOS["p48"]).MLEM(y, None, 3, storage="f16")
+    with ReconService(max_slots=4) as svc:
+        a = svc.submit(**kw("p48", y, 4, storage="bf16", image_storage="f32"))
+        b = svc.submit(**kw("p48", y, 3, storage="f16"))
+        got, got16 = svc.wait(a)[0], svc.wait(b)[0]
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got16, want16)
