@@ -90,6 +90,14 @@ PETL_EXTERN bool TV_gradient(float* f, float* Df, int N_1, int N_2, int N_3, flo
                              bool data_on_cpu);                                                   /* :92 */
 PETL_EXTERN bool relativeDifferences_gradient(float* f, float* Df, int N_1, int N_2, int N_3, float delta,
                                               float beta);                                        /* :96 */
+/* p-Huber TV cost and quadratic form <d, R''(f) d> (the RWLS step size); -1 for a null input.  With
+ * data_on_cpu false, f and d are device arrays. */
+PETL_EXTERN float TV_cost(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p,
+                          bool data_on_cpu);                                               /* :91 */
+PETL_EXTERN float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, float delta, float beta, float p,
+                              bool data_on_cpu);                                           /* :93 */
+/* Gaussian smoothing of a host array in place: numDims 3 all axes, 2 the last two, 1 the first. */
+PETL_EXTERN bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims); /* :97 */
 
 /* exports outside the projector / MLEM path: present, return false / 0 with a message */
 
@@ -106,15 +114,10 @@ PETL_EXTERN bool ray_trace(float** g, int oversampling);                        
 PETL_EXTERN bool stopping_power(float** g, int oversampling);                              /* :71 */
 PETL_EXTERN bool set_solid_angle_correction(float** g, bool do_inverse);                   /* :72 */
 PETL_EXTERN bool apply_corrections(float** g, float** r, float threshold);                 /* :73 */
-PETL_EXTERN float TV_cost(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p,
-                          bool data_on_cpu);                                               /* :91 */
-PETL_EXTERN float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, float delta, float beta, float p,
-                              bool data_on_cpu);                                           /* :93 */
 PETL_EXTERN bool diffuse(float* f, int N_1, int N_2, int N_3, float delta, float p, int numIter,
                          bool data_on_cpu);                                                /* :94 */
 PETL_EXTERN bool TV_denoise(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p, int numIter,
                             bool doMean, bool data_on_cpu);                                /* :95 */
-PETL_EXTERN bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims); /* :97 */
 
 /* ------------------------------------------------------------------------------------------------
  * Part 2: additions (no counterpart in the reference)
@@ -185,6 +188,19 @@ PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float b
  * sum(data) / sum(sensitivity)).  No mask, response or log-likelihood. */
 PETL_EXTERN bool MLEM_gpu_storage(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
                                   int numSubsets, int planogram_storage, int image_storage);
+
+/* Device-resident RWLS (PETLserver.RWLS, src/petl_server.py:1691-1877): preconditioned conjugate gradients
+ * on 0.5 (Pf - g)^T W (Pf - g) + TV(f) (p-Huber TV with p = 1, delta, beta; none when beta <= 0), every
+ * array on the GPU.  g: host planograms; f: host image, the start (negatives clipped when constrained,
+ * all zeros -> Pf = 0) and overwritten with the result.  weight_mode: 0 = the reference's default
+ * 1 / max(gaussian_filter(g, 3), 7), 1 = the host planogram weights W, 2 = the scalar w_scalar (gradient
+ * unweighted, <Pd, Pd> scaled, as the reference treats a scalar).  preconditioner: 0 none, 1 SQS.
+ * response: optional host planograms the projectors include (the reference's
+ * include_response_with_projectors = True).  cost (optional, numIter doubles): 0.5 <r, r>_W + TV cost after
+ * each iteration run; iterations (optional): iterations run before a step <= 0 ended the loop. */
+PETL_EXTERN bool RWLS_gpu(float** g, float* f, int numIter, float delta, float beta, int weight_mode, float** W,
+                          float w_scalar, int preconditioner, bool nonnegativityConstraint, float** response,
+                          double* cost, int* iterations);
 
 /* ------------------------------------------------------------------------------------------------
  * Part 3: reconstruction service (no counterpart in the reference).  Continuous batching of MLEM/OSEM

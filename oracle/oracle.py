@@ -77,6 +77,9 @@ def lib():
         L.orc_tv_gradient_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_float, c_float]
         L.orc_tv_cost_f64.argtypes = [dp, c_int, c_int, c_int, c_float, c_float, c_float]
         L.orc_tv_cost_f64.restype = c_double
+        L.orc_tv_quadform_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_float, c_float]
+        L.orc_tv_quadform_f64.restype = c_double
+        L.orc_gaussian_filter_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_int]
         L.orc_max_threads.restype = c_int
         L.orc_project_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         L.orc_backproject_petl_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
@@ -194,9 +197,36 @@ def tv_gradient(f, delta, beta, p=1.0, dtype=np.float32):
     return D
 
 
-def tv_cost(f, delta, beta, p=1.0):
+def _as3d(f):
+    """2D arrays are taken as one slice (N_1 = 1), as PETLserver.TVcost does (src/petl_server.py:1906-1913)."""
     f = np.ascontiguousarray(f, dtype=np.float64)
+    return f.reshape((1,) + f.shape) if f.ndim == 2 else f
+
+
+def tv_cost(f, delta, beta, p=1.0):
+    f = _as3d(f)
     return lib().orc_tv_cost_f64(f, *f.shape, delta, beta, p)
+
+
+def tv_quadform(f, d, delta, beta, p=1.0):
+    """<d, R''(f) d> of the p-Huber TV prior with the surrogate curvature h'(t)/t."""
+    f, d = _as3d(f), _as3d(d)
+    return lib().orc_tv_quadform_f64(f, d, *f.shape, delta, beta, p)
+
+
+def gaussian_filter(f, FWHM=2.0, numDims=3):
+    """PETLserver.gaussian_filter (src/petl_server.py:2129-2155): a 4D planogram is smoothed per
+    (v1, v0) row over (u1, u0); a 3D array along numDims axes; a 2D array as one slice."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if f.ndim == 4:
+        shape3, numDims = (f.shape[0] * f.shape[1], f.shape[2], f.shape[3]), 2
+    elif f.ndim == 3:
+        shape3 = f.shape
+    else:
+        shape3 = (1,) + f.shape
+    out = np.zeros(shape3, np.float64)
+    lib().orc_gaussian_filter_f64(f.reshape(shape3), out, *shape3, FWHM, numDims)
+    return out.reshape(f.shape)
 
 
 # --------------------------------------------------------------------------------------------------
@@ -244,6 +274,105 @@ def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0
         if callback is not None:
             callback(n, f, Pd)
     return f
+
+
+def rwls_default_weights(g):
+    """W = 1 / max(gaussian_filter(g, FWHM 3), 7) per planogram (src/petl_server.py:1722-1726)."""
+    return [1.0 / np.maximum(gaussian_filter(gi, 3.0), 7.0) for gi in g]
+
+
+def rwls(plans, vol, g, num_iter, f=None, delta=1.0, beta=0.0, W=None, sqs=False, nonneg=True, mode="petl",
+         response=None, callback=None, nthreads=None):
+    """PETLserver.RWLS (src/petl_server.py:1691-1877) restated in FP64 on the oracle projectors:
+    preconditioned conjugate gradients on 0.5 (Pf - g)^T W (Pf - g) + TV(f) with the p = 1 Huber TV of
+    TVgradient / TVquadForm, Polak-Ribiere directions restarted every 50 iterations or when the direction
+    is not a descent direction, the step <d, grad> / (<Pd, Pd>_W + <d, R'' d>), and the loop ending at a
+    step <= 0.
+
+    W: None (the default weights above), a list of planogram weights, or a scalar.  As in the reference a
+    scalar weight leaves the gradient unweighted and only scales <Pd, Pd> in the step.  sqs: the separable
+    quadratic surrogate preconditioner 1 / P*(W P 1) (0 -> 1).  response: planogram arrays that the
+    projectors include (project -> R * Pf, backproject(x) -> P*(R x)).  Without the nonnegativity
+    constraint the reference updates the residual as r <- W r - step Pd - g (its Pf_minus_g aliases Pf and
+    was weighted in place); that update is restated as it is.  Returns (f, iterations run)."""
+    dt = np.float64
+    nP = len(plans)
+
+    def P(x):
+        q = project(plans, vol, x, dtype=dt, nthreads=nthreads)
+        return [qi * ri for qi, ri in zip(q, response)] if response is not None else q
+
+    def Pt(y):
+        if response is not None:
+            y = [yi * ri for yi, ri in zip(y, response)]
+        return backproject(plans, vol, y, mode=mode, dtype=dt, nthreads=nthreads)
+
+    def inner(x, y, w=None):
+        if isinstance(x, list):
+            if w is None:
+                return sum(float(np.sum(a * b)) for a, b in zip(x, y))
+            if np.isscalar(w):
+                return sum(float(np.sum(a * b)) * w for a, b in zip(x, y))
+            return sum(float(np.sum(a * b * c)) for a, b, c in zip(x, y, w))
+        return float(np.sum(x * y))
+
+    g = [np.asarray(gi, dt) for gi in g]
+    if W is None:
+        W = rwls_default_weights(g)
+    elif not np.isscalar(W):
+        W = [np.asarray(wi, dt) for wi in W]
+    weighted = isinstance(W, list)
+    f = np.zeros(vol.shape, dt) if f is None else np.array(f, dtype=dt)
+    if np.any(f):
+        if nonneg:
+            f[f < 0] = 0
+        r = P(f)
+    else:
+        r = [np.zeros_like(gi) for gi in g]
+    r = [a - b for a, b in zip(r, g)]
+    Q = None
+    if sqs:
+        Pd = P(np.ones(vol.shape, dt))
+        if weighted:
+            Pd = [a * b for a, b in zip(Pd, W)]
+        Q = Pt(Pd)
+        Q = np.where(Q == 0, 1.0, 1.0 / np.where(Q == 0, 1.0, Q))
+    d = np.zeros(vol.shape, dt)
+    grad_old = np.zeros(vol.shape, dt)
+    gg_old = 0.0
+    done = 0
+    for n in range(num_iter):
+        if weighted:
+            r = [a * b for a, b in zip(r, W)]
+        grad = Pt(r)
+        if beta > 0.0:
+            grad = grad + tv_gradient(f, delta, beta, 1.0, dtype=dt)
+        u = Q * grad if Q is not None else grad.copy()
+        if n % 50 == 0:
+            d = u.copy()
+        else:
+            gamma = (inner(u, grad) - inner(u, grad_old)) / gg_old
+            d = u + gamma * d
+            if inner(d, grad) <= 0.0:
+                d = u.copy()
+        gg_old = inner(u, grad)
+        grad_old = grad.copy()
+        Pd = P(d)
+        num = inner(d, grad)
+        den = inner(Pd, Pd, W) + (tv_quadform(f, d, delta, beta, 1.0) if beta > 0.0 else 0.0)
+        step = num / den if abs(den) > 1.0e-16 else 0.0
+        if step <= 0.0:
+            break
+        f = f - step * d
+        if nonneg:
+            f[f < 0] = 0
+            r = [a - b for a, b in zip(P(f), g)]
+        else:
+            r = [a - step * b - c for a, b, c in zip(r, Pd, g)]
+        done = n + 1
+        if callback is not None:
+            callback(n, f)
+    return f, done
 
 
 def osem(plans, vol, g, num_iter, num_subsets, f0=None, nthreads=None):

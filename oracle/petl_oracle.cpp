@@ -22,6 +22,8 @@
 //                                 summed over planograms as in src/petl.cpp:226-277
 //     relative-difference grad    src/relative_differences.cu:21-28, 107-173, 481-532
 //     p-Huber TV gradient         src/total_variation.cu:21-78, 434-660
+//     p-Huber TV cost, quad. form src/total_variation.cu:80-216, 218-417, 662-800
+//     Gaussian smoothing          src/relative_differences.cu:314-420, 753-800
 //
 //  Deliberate differences from the reference:
 //     * texture fetches are replaced by exact array reads: the forward samples texel centres (exact in
@@ -510,6 +512,13 @@ struct huber {
             return 0.5 * x * x;
         return (double)slope * std::pow(std::fabs(x), (double)p) + (double)shift;
     }
+    // curvature weight of the quadratic form (src/total_variation.cu:66-78): h'(t)/t, i.e. 1 inside delta
+    double dd(double x) const
+    {
+        if (std::fabs(x) <= (double)delta)
+            return 1.0;
+        return (double)p * (double)slope * std::pow(std::fabs(x), (double)p - 2.0);
+    }
 };
 
 template <typename R>
@@ -550,6 +559,8 @@ void tv_gradient(const R* f, R* Df, int N1, int N2, int N3, float delta, float b
     }
 }
 
+// p-Huber TV cost: aTV_Huber_costTerm (src/total_variation.cu:80-216) summed over the array by
+// anisotropicTotalVariation_cost (:746-800); h() above is the USE_P_HUBER Huber of :33-46.
 double tv_cost(const double* f, int N1, int N2, int N3, float delta, float beta, float p)
 {
     if (beta <= 0.0f)
@@ -557,6 +568,9 @@ double tv_cost(const double* f, int N1, int N2, int N3, float delta, float beta,
     if (delta < 1.0e-8f)
         delta = 1.0e-8f;
     const huber hb(delta, p);
+    // neighbour weights formed in float as the kernels form them (dist_1..3 = w * beta)
+    const double d1 = (double)(1.0f * beta), d2 = (double)(0.7071067811865475f * beta),
+                 d3 = (double)(0.5773502691896258f * beta);
     double acc = 0.0;
     for (int i = 0; i < N1; i++)
         for (int j = 0; j < N2; j++)
@@ -573,12 +587,86 @@ double tv_cost(const double* f, int N1, int N2, int N3, float delta, float beta,
                             const int a = std::min(N1 - 1, std::max(0, i + di));
                             const int b = std::min(N2 - 1, std::max(0, j + dj));
                             const int c = std::min(N3 - 1, std::max(0, k + dk));
-                            const double w = (m == 1) ? 1.0 : (m == 2 ? (double)0.7071067811865475f
-                                                                      : (double)0.5773502691896258f);
+                            const double w = (m == 1) ? d1 : (m == 2 ? d2 : d3);
                             acc += w * hb.h(c0 - f[(int64_t(a) * N2 + b) * N3 + c]);
                         }
             }
-    return (double)beta * acc;
+    return acc;
+}
+
+// Quadratic form <d, R''(f) d> of the p-Huber TV prior: aTV_Huber_quadFormTerm (src/total_variation.cu:
+// 218-417) summed over the array by anisotropicTotalVariation_quadraticForm (:662-744).  Each neighbour
+// pair term is w * h''(f_i - f_j) * (d_i - d_j)^2 with the surrogate curvature dd() above, over the same
+// 26 (8 in-plane when N1 == 1) border-clamped neighbours as the cost; beta <= 0 -> 1, delta >= 1e-8.
+double tv_quadform(const double* f, const double* d, int N1, int N2, int N3, float delta, float beta, float p)
+{
+    if (beta <= 0.0f)
+        beta = 1.0f;
+    if (delta < 1.0e-8f)
+        delta = 1.0e-8f;
+    const huber hb(delta, p);
+    // neighbour weights formed in float as the kernels form them (dist_1..3 = w * beta)
+    const double d1 = (double)(1.0f * beta), d2 = (double)(0.7071067811865475f * beta),
+                 d3 = (double)(0.5773502691896258f * beta);
+    double acc = 0.0;
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++)
+            for (int k = 0; k < N3; k++) {
+                const int64_t c = (int64_t(i) * N2 + j) * N3 + k;
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -1; dk <= 1; dk++) {
+                            if (N1 == 1 && di != 0)
+                                continue;
+                            const int m = std::abs(di) + std::abs(dj) + std::abs(dk);
+                            if (m == 0)
+                                continue;
+                            const int a = std::min(N1 - 1, std::max(0, i + di));
+                            const int b = std::min(N2 - 1, std::max(0, j + dj));
+                            const int e = std::min(N3 - 1, std::max(0, k + dk));
+                            const int64_t o = (int64_t(a) * N2 + b) * N3 + e;
+                            const double w = (m == 1) ? d1 : (m == 2 ? d2 : d3);
+                            const double dt = d[c] - d[o];
+                            acc += w * hb.dd(f[c] - f[o]) * dt * dt;
+                        }
+            }
+    return acc;
+}
+
+// Gaussian smoothing of GaussianFilter (src/relative_differences.cu:753-800) with the kernels at
+// :314-420.  numDims 3 smooths along all three axes, 2 along the last two, 1 along the first.  Parameters
+// are formed in float as the kernels form them: sigma = FWHM / (2 sqrt(2 ln 2)), radius =
+// ceil(sqrt(2 ln 10) sigma) and the exponent factor 1 / (2 sigma) (sigma, not sigma^2).  A tap at offset t
+// reads the border-clamped sample and is weighted by exp(-factor * s^2), s = distance to that clamped
+// sample (not t); the sum is divided by the sum of the weights.
+void gaussian_filter(const double* f, double* out, int N1, int N2, int N3, float FWHM, int numDims)
+{
+    const float sigma = FWHM / (2.0f * std::sqrt(2.0f * std::log(2.0f)));
+    const int R = int(std::ceil(std::sqrt(2.0f * std::log(10.0f)) * sigma));
+    const double den = (double)(1.0f / (2.0f * sigma));
+    const int Ri = (numDims == 2) ? 0 : R;
+    const int Rj = (numDims == 1) ? 0 : R;
+    const int Rk = (numDims == 1) ? 0 : R;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++)
+            for (int k = 0; k < N3; k++) {
+                double val = 0.0, sum = 0.0;
+                for (int di = -Ri; di <= Ri; di++) {
+                    const int a = std::max(0, std::min(i + di, N1 - 1));
+                    for (int dj = -Rj; dj <= Rj; dj++) {
+                        const int b = std::max(0, std::min(j + dj, N2 - 1));
+                        for (int dk = -Rk; dk <= Rk; dk++) {
+                            const int c = std::max(0, std::min(k + dk, N3 - 1));
+                            const double s2 = double((i - a) * (i - a) + (j - b) * (j - b) + (k - c) * (k - c));
+                            const double w = std::exp(-den * s2);
+                            val += w * f[(int64_t(a) * N2 + b) * N3 + c];
+                            sum += w;
+                        }
+                    }
+                }
+                out[(int64_t(i) * N2 + j) * N3 + k] = val / sum;
+            }
 }
 
 }  // namespace
@@ -678,6 +766,17 @@ void orc_tv_gradient_f64(const double* f, double* Df, int N1, int N2, int N3, fl
 double orc_tv_cost_f64(const double* f, int N1, int N2, int N3, float delta, float beta, float p)
 {
     return tv_cost(f, N1, N2, N3, delta, beta, p);
+}
+
+double orc_tv_quadform_f64(const double* f, const double* d, int N1, int N2, int N3, float delta, float beta,
+                           float p)
+{
+    return tv_quadform(f, d, N1, N2, N3, delta, beta, p);
+}
+
+void orc_gaussian_filter_f64(const double* f, double* out, int N1, int N2, int N3, float FWHM, int numDims)
+{
+    gaussian_filter(f, out, N1, N2, N3, FWHM, numDims);
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }

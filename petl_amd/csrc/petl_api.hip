@@ -13,6 +13,8 @@
 #include "petl_common.h"
 #include "petl_model.h"
 #include "mlem_job.h"
+#include "rwls_job.h"
+#include "lsq_kernels.h"
 #include "binning.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
@@ -243,6 +245,92 @@ bool run_bin(Model& m, float** g, EventSource& src, bool reset, bool nearest, bo
     (void)hipHostFree(hbuf);
     if (total_events)
         *total_events = total;
+    return true;
+}
+
+// src/total_variation.cu:662-800: -1 for a null input; the sum of the per-voxel terms, as float
+float tv_reduce_export(const char* what, bool quad, float* f, float* d, int N_1, int N_2, int N_3, float delta,
+                       float beta, float p, bool data_on_cpu)
+{
+    if (!f || (quad && !d))
+        return -1.0f;
+    if (!dims_ok({N_1, N_2, N_3}))
+        return -1.0f;
+    float out = -1.0f;
+    guarded(what, [&] {
+        Model* m = M();
+        hipStream_t s = m->stream();
+        const int64_t n = int64_t(N_1) * N_2 * N_3;
+        DevBuf a, b, red;
+        const float* df = f;
+        const float* dd = d;
+        if (data_on_cpu) {
+            a.ensure(n * sizeof(float));
+            h2d(a.p, f, n * sizeof(float), s);
+            df = a.f();
+            if (d) {
+                b.ensure(n * sizeof(float));
+                h2d(b.p, d, n * sizeof(float), s);
+                dd = b.f();
+            }
+        }
+        red.ensure(sizeof(double) * size_t(1 + red_scratch_doubles(n)));
+        double* slot = static_cast<double*>(red.p);
+        launch_tv_reduce(df, dd, N_1, N_2, N_3, delta, beta, p, slot + 1, slot, s);
+        double h = 0.0;
+        d2h(&h, slot, sizeof(double), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        out = float(h);
+        return true;
+    });
+    return out;
+}
+
+// RWLS on the GPU (src/petl_server.py:1691-1877); f is the caller's host image, updated in place.
+bool run_rwls(Model& m, float** g, float* f, int numIter, float delta, float beta, int weight_mode, float** W,
+              float w_scalar, int preconditioner, bool nonneg, float** response, double* cost, int* iterations)
+{
+    if (!g || !f || !m.allDefined() || numIter < 0)
+        return false;
+    if (weight_mode < RWLS_W_DEFAULT || weight_mode > RWLS_W_SCALAR || (weight_mode == RWLS_W_ARRAYS && !W))
+        return false;
+    for (size_t i = 0; i < m.plans.size(); i++)
+        if (!g[i] || (weight_mode == RWLS_W_ARRAYS && !W[i]) || (response && !response[i]))
+            return false;
+    RwlsParams prm;
+    prm.numIter = numIter;
+    prm.delta = delta;
+    prm.beta = beta;
+    prm.weights = weight_mode;
+    prm.w_scalar = w_scalar;
+    prm.sqs = preconditioner == 1;
+    prm.nonneg = nonneg;
+    prm.response = response != nullptr;
+    // start rules (src/petl_server.py:1743-1759): a nonzero image has its negatives clipped (in place) when
+    // constrained and is projected; an all-zero image starts from Pf = 0
+    const int64_t V = m.vol_elems();
+    bool nonzero = false;
+    for (int64_t i = 0; i < V && !nonzero; i++)
+        nonzero = f[i] != 0.0f;
+    if (nonzero && nonneg)
+        for (int64_t i = 0; i < V; i++)
+            if (f[i] < 0.0f)
+                f[i] = 0.0f;
+    hipStream_t s = m.stream();
+    RwlsJob job(m, prm);
+    job.setup(g, f, nonzero, W, response, s);
+    int done = 0;
+    for (int n = 0; n < numIter; n++) {
+        if (!job.iterate(n, s))
+            break;
+        done = n + 1;
+        if (cost)
+            cost[n] = job.cost(s);
+    }
+    job.fetch(f, s);
+    PETL_HIP_CHECK(hipStreamSynchronize(s));
+    if (iterations)
+        *iterations = done;
     return true;
 }
 
@@ -634,6 +722,37 @@ bool relativeDifferences_gradient(float* f, float* Df, int N_1, int N_2, int N_3
     });
 }
 
+// ---- TV cost / quadratic form and Gaussian smoothing (the RWLS step size and default weights) ----
+float TV_cost(float* f, int N_1, int N_2, int N_3, float delta, float beta, float p, bool data_on_cpu)
+{
+    return tv_reduce_export("TV_cost", false, f, nullptr, N_1, N_2, N_3, delta, beta, p, data_on_cpu);
+}
+
+float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, float delta, float beta, float p, bool data_on_cpu)
+{
+    return tv_reduce_export("TV_quadForm", true, f, d, N_1, N_2, N_3, delta, beta, p, data_on_cpu);
+}
+
+// src/relative_differences.cu:753-800 (host array, smoothed in place)
+bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims)
+{
+    if (f == nullptr || N_1 <= 0 || N_2 <= 0 || N_3 <= 0)
+        return false;
+    return guarded("gaussian_filter", [&] {
+        Model* m = M();
+        hipStream_t s = m->stream();
+        const int64_t n = int64_t(N_1) * N_2 * N_3;
+        DevBuf a, b;
+        a.ensure(n * sizeof(float));
+        b.ensure(n * sizeof(float));
+        h2d(a.p, f, n * sizeof(float), s);
+        gaussian_filter_device(a.f(), b.f(), N_1, N_2, N_3, FWHM, numDims, false, s);
+        d2h(f, a.p, n * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
 // ---- outside the projector / MLEM path ----
 
 bool simulate_scatter(float**, float*) { return unsupported("simulate_scatter"); }
@@ -649,19 +768,8 @@ bool ray_trace(float**, int) { return unsupported("ray_trace"); }
 bool stopping_power(float**, int) { return unsupported("stopping_power"); }
 bool set_solid_angle_correction(float**, bool) { return unsupported("set_solid_angle_correction"); }
 bool apply_corrections(float**, float**, float) { return unsupported("apply_corrections"); }
-float TV_cost(float*, int, int, int, float, float, float, bool)
-{
-    unsupported("TV_cost");
-    return 0.0f;
-}
-float TV_quadForm(float*, float*, int, int, int, float, float, float, bool)
-{
-    unsupported("TV_quadForm");
-    return 0.0f;
-}
 bool diffuse(float*, int, int, int, float, float, int, bool) { return unsupported("diffuse"); }
 bool TV_denoise(float*, int, int, int, float, float, float, int, bool, bool) { return unsupported("TV_denoise"); }
-bool gaussian_filter(float*, int, int, int, float, int) { return unsupported("gaussian_filter"); }
 
 // =================================================================================================
 // Part 2: additions
@@ -837,6 +945,16 @@ bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int pri
     Model* m = M();
     return guarded("MLEM_gpu",
                    [&] { return run_mlem(*m, g, f, numIter, delta, beta, prior, p, numSubsets, mask, response, loglik); });
+}
+
+bool RWLS_gpu(float** g, float* f, int numIter, float delta, float beta, int weight_mode, float** W, float w_scalar,
+              int preconditioner, bool nonnegativityConstraint, float** response, double* cost, int* iterations)
+{
+    Model* m = M();
+    return guarded("RWLS_gpu", [&] {
+        return run_rwls(*m, g, f, numIter, delta, beta, weight_mode, W, w_scalar, preconditioner,
+                        nonnegativityConstraint, response, cost, iterations);
+    });
 }
 
 }  // extern "C"

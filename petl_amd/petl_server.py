@@ -8,7 +8,8 @@ that path run unchanged.  The compute runs in libpetl.so (HIP kernels for gfx950
 required and loading fails loudly without it.
 
 Additions beyond the reference: ``set_backprojector_mode``, ``project_gpu`` / ``backproject_gpu`` on
-torch tensors, and a device-resident ``MLEM`` (with optional OSEM subsets and log-likelihood trace).
+torch tensors, a device-resident ``MLEM`` (with optional OSEM subsets and log-likelihood trace) and a
+device-resident ``RWLS`` (and LS / WLS / RLS).
 """
 import ctypes
 import glob
@@ -60,6 +61,13 @@ _SIGNATURES = {
     "TV_gradient": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 +
                     [ctypes.c_float] * 3 + [ctypes.c_bool]),
     "relativeDifferences_gradient": (ctypes.c_bool, [_f32, _f32] + [ctypes.c_int] * 3 + [ctypes.c_float] * 2),
+    "TV_cost": (ctypes.c_float, [ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_float] * 3 + [ctypes.c_bool]),
+    "TV_quadForm": (ctypes.c_float, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_float] * 3 +
+                    [ctypes.c_bool]),
+    "gaussian_filter": (ctypes.c_bool, [_f32] + [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int]),
+    "RWLS_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_int, ctypes.POINTER(_c_float_p), ctypes.c_float, ctypes.c_int, ctypes.c_bool,
+                                 ctypes.POINTER(_c_float_p), ctypes.c_void_p, ctypes.c_void_p]),
     "set_backprojector_mode": (ctypes.c_bool, [ctypes.c_int]),
     "get_backprojector_mode": (ctypes.c_int, []),
     "set_projector_path": (ctypes.c_bool, [ctypes.c_int]),
@@ -619,6 +627,81 @@ class PETLserver:
     def OSEM(self, g, f, numIter, numSubsets, delta=2.0, beta=0.0, mask=None):
         return self.MLEM(g, f, numIter, delta=delta, beta=beta, mask=mask, num_subsets=numSubsets)
 
+    def LS(self, g, f, numIter, preconditioner=None, nonnegativityConstraint=True):
+        """Least squares, 0.5 ||Pf - g||^2, by preconditioned CG (src/petl_server.py:1608-1633)."""
+        return self.RWLS(g, f, numIter, 1.0, 0.0, 1.0, preconditioner, nonnegativityConstraint)
+
+    def WLS(self, g, f, numIter, W=None, preconditioner=None, nonnegativityConstraint=True):
+        """Weighted least squares, 0.5 (Pf - g)^T W (Pf - g) (src/petl_server.py:1635-1661)."""
+        return self.RWLS(g, f, numIter, 1.0, 0.0, W, preconditioner, nonnegativityConstraint)
+
+    def RLS(self, g, f, numIter, delta=1.0, beta=0.0, preconditioner=None, nonnegativityConstraint=True):
+        """Regularized least squares, 0.5 ||Pf - g||^2 + TV(f) (src/petl_server.py:1663-1689)."""
+        return self.RWLS(g, f, numIter, delta, beta, 1.0, preconditioner, nonnegativityConstraint)
+
+    def RWLS(self, g, f, numIter, delta=1.0, beta=0.0, W=None, preconditioner=None, nonnegativityConstraint=True,
+             include_response_with_projectors=False):
+        r"""Regularized weighted least squares, 0.5 (Pf - g)^T W (Pf - g) + R(f), by preconditioned conjugate
+        gradients (src/petl_server.py:1691-1845), the whole loop on the GPU.
+
+        R is the anisotropic p-Huber TV (p = 1, delta, beta; none when beta <= 0).  W: None -> 1 /
+        max(gaussian_filter(g, 3.0), 7.0); a list of planogram weights; or a scalar, which (as in the
+        reference) leaves the gradient unweighted and scales <Pd, Pd> in the step.  preconditioner: 'SQS' or
+        True for the separable quadratic surrogate (P* W P 1)^-1, anything else none.  f: the start (all zeros
+        -> Pf = 0; negatives clipped when constrained), updated in place and returned.  With a response set,
+        include_response_with_projectors=True makes the projectors include it; the default (False) needs the
+        reference's apply_corrections (bad-pixel correction), which this library does not provide.
+        """
+        if self.response is not None and not include_response_with_projectors:
+            raise ValueError("RWLS with a response needs include_response_with_projectors=True here "
+                             "(apply_corrections is not available)")
+        if f is None:
+            f = self.allocate_volume()
+        self._check_arrays(g, f)
+        if W is None:
+            mode, Wl, ws = 0, None, 1.0
+        elif isinstance(W, list):
+            if len(W) != len(g) or any(np.shape(a) != np.shape(b) for a, b in zip(W, g)):
+                raise ValueError("W must be a list of arrays shaped like the projection data")
+            mode, Wl, ws = 1, [np.ascontiguousarray(a, dtype=np.float32) for a in W], 1.0
+        elif isinstance(W, (int, float)):
+            mode, Wl, ws = 2, None, float(W)
+        else:
+            raise ValueError("W must be None, a list of planogram weights or a scalar")
+        sqs = preconditioner is True or preconditioner == 'SQS'
+        resp = self.floatstarstar(self.response) if (self.response is not None and include_response_with_projectors) \
+            else None
+        cost = np.zeros(max(int(numIter), 1), dtype=np.float64)
+        done = ctypes.c_int(0)
+        self.set_model()
+        ok = self.libprojectors.RWLS_gpu(self.floatstarstar(g), f, int(numIter), float(delta), float(beta), mode,
+                                         self.floatstarstar(Wl) if Wl is not None else None, ws, 1 if sqs else 0,
+                                         bool(nonnegativityConstraint), resp,
+                                         cost.ctypes.data_as(ctypes.c_void_p), ctypes.byref(done))
+        if not ok:
+            raise TypeError('RWLS failed')
+        if self.print_warnings:
+            for n in range(done.value):
+                print('RWLS iteration ' + str(n + 1) + ' of ' + str(numIter))
+            if done.value < numIter:
+                print('invalid step size; quitting!')
+        if self.print_cost:
+            for n in range(done.value):
+                print('\tcost = ' + str(cost[n]))
+        return f
+
+    def RWLSstepSize(self, f, grad, d, Pd, W, delta, beta, num=None):
+        """<d, grad> / (<Pd, Pd>_W + <d, R''(f) d>), 0 when the denominator is ~0 (src/petl_server.py:1847-1877)."""
+        if num is None:
+            num = self.innerProd(d, grad)
+        denomA = self.innerProd(Pd, Pd, W) if W is not None else self.innerProd(Pd, Pd)
+        denomB = self.TVquadForm(f, d, delta, beta) if beta > 0.0 else 0.0
+        denom = denomA + denomB
+        stepSize = num / denom if np.abs(denom) > 1.0e-16 else 0.0
+        if self.print_warnings:
+            print('\tlambda = ' + str(stepSize))
+        return stepSize
+
     # ------------------------------------------------------------------ priors
     @staticmethod
     def _dims3(f):
@@ -640,6 +723,42 @@ class PETLserver:
         fc = np.ascontiguousarray(f, dtype=np.float32)
         self.libprojectors.TV_gradient(fc.ctypes.data, Df.ctypes.data, N_1, N_2, N_3, delta, beta, p, True)
         return Df
+
+    def TVcost(self, f, delta, beta=0.0, p=1.0):
+        """Anisotropic p-Huber TV functional of a 2D/3D array (src/petl_server.py:1879-1922)."""
+        N_1, N_2, N_3 = self._dims3(f)
+        self.set_model()
+        if hasattr(f, "data_ptr"):
+            return self.libprojectors.TV_cost(f.data_ptr(), N_1, N_2, N_3, delta, beta, p, not f.is_cuda)
+        fc = np.ascontiguousarray(f, dtype=np.float32)
+        return self.libprojectors.TV_cost(fc.ctypes.data, N_1, N_2, N_3, delta, beta, p, True)
+
+    def TVquadForm(self, f, d, delta, beta=0.0, p=1.0):
+        """<d, R''(f) d> of the p-Huber TV functional with h'(t)/t for h''(t) (src/petl_server.py:1976-2026)."""
+        N_1, N_2, N_3 = self._dims3(f)
+        self.set_model()
+        if hasattr(f, "data_ptr"):
+            return self.libprojectors.TV_quadForm(f.data_ptr(), d.data_ptr(), N_1, N_2, N_3, delta, beta, p,
+                                                  not f.is_cuda)
+        fc = np.ascontiguousarray(f, dtype=np.float32)
+        dc = np.ascontiguousarray(d, dtype=np.float32)
+        return self.libprojectors.TV_quadForm(fc.ctypes.data, dc.ctypes.data, N_1, N_2, N_3, delta, beta, p, True)
+
+    def gaussian_filter(self, f, FWHM=2.0, numDims=3):
+        """Gaussian smoothing in place (src/petl_server.py:2129-2155): lists elementwise, a 4D planogram over
+        (u1, u0) within each (v1, v0) row, a 3D array along numDims axes, a 2D array as one slice."""
+        if isinstance(f, list):
+            for a in f:
+                self.gaussian_filter(a, FWHM, numDims)
+            return
+        if f.ndim == 4:
+            N_1, N_2, N_3, numDims = f.shape[0] * f.shape[1], f.shape[2], f.shape[3], 2
+        elif f.ndim == 3:
+            N_1, N_2, N_3 = f.shape
+        else:
+            N_1, N_2, N_3 = 1, f.shape[0], f.shape[1]
+        self.set_model()
+        self.libprojectors.gaussian_filter(f, N_1, N_2, N_3, FWHM, numDims)
 
     def relative_differences_grad(self, f, delta, beta):
         """Gradient of the relative-difference penalty (src/petl_server.py:2112-2127)."""
