@@ -640,7 +640,7 @@ class PETLserver:
         return self.RWLS(g, f, numIter, delta, beta, 1.0, preconditioner, nonnegativityConstraint)
 
     def RWLS(self, g, f, numIter, delta=1.0, beta=0.0, W=None, preconditioner=None, nonnegativityConstraint=True,
-             include_response_with_projectors=False):
+             include_response_with_projectors=False, return_cost=False):
         r"""Regularized weighted least squares, 0.5 (Pf - g)^T W (Pf - g) + R(f), by preconditioned conjugate
         gradients (src/petl_server.py:1691-1845), the whole loop on the GPU.
 
@@ -651,6 +651,8 @@ class PETLserver:
         -> Pf = 0; negatives clipped when constrained), updated in place and returned.  With a response set,
         include_response_with_projectors=True makes the projectors include it; the default (False) needs the
         reference's apply_corrections (bad-pixel correction), which this library does not provide.
+        Extra: return_cost=True also returns the cost 0.5 (Pf - g)^T W (Pf - g) + TV(f) after each iteration
+        run (the values the reference prints with print_cost).
         """
         if self.response is not None and not include_response_with_projectors:
             raise ValueError("RWLS with a response needs include_response_with_projectors=True here "
@@ -688,6 +690,8 @@ class PETLserver:
         if self.print_cost:
             for n in range(done.value):
                 print('\tcost = ' + str(cost[n]))
+        if return_cost:
+            return f, cost[:done.value].copy()
         return f
 
     def RWLSstepSize(self, f, grad, d, Pd, W, delta, beta, num=None):

@@ -5,7 +5,9 @@
 * closed form of a uniform volume: bins whose footprint stays inside the volume in every slice equal
   (128 slices x 2 mm) / sqrt(1 + v0^2 + v1^2);
 * linearity A(2a - 3b) = 2Aa - 3Ab;
-* the oracle agrees with the device result on a sampled v1 row (the full oracle projection takes minutes).
+* the oracle agrees with the device result on a sampled v1 row (the full oracle projection takes minutes);
+* MLEM: count preservation, non-negativity, non-decreasing log-likelihood, bf16 storage consistency;
+* RWLS: the first conjugate-gradient step in closed form and its exact line search; a decreasing cost.
 """
 import numpy as np
 import pytest
@@ -159,3 +161,36 @@ def test_mlem_full_size_rd_prior(full, full_data):
     fbf = s.MLEM(y, None, 20, delta=2.0, beta=0.01, storage="bf16", image_storage="f32")
     err = float(np.linalg.norm(fbf.astype(np.float64) - f32) / np.linalg.norm(f32.astype(np.float64)))
     assert err < 1e-3, err
+
+
+def test_ls_full_size_first_step_closed_form(full, full_data):
+    """From f = 0 the first LS step (unit weights, no preconditioner, matched back projector) is
+    f1 = t A^T y with t = |A^T y|^2 / |A A^T y|^2 (no clipping: A^T y >= 0), and the exact line search
+    leaves the new gradient A^T (A f1 - y) orthogonal to the direction."""
+    s, shapes = full
+    y = full_data
+    s.set_backprojector_mode("matched")
+    try:
+        f1 = s.LS(y, None, 1)
+        yd = [torch.from_numpy(a).to(DEV) for a in y]
+        ATy = back(s, yd)
+        AATy = proj(s, shapes, ATy)
+        t = dot(ATy, ATy) / sum(dot(a, a) for a in AATy)
+        want = (t * ATy.double()).cpu().numpy()
+        r = [a - b for a, b in zip(proj(s, shapes, torch.from_numpy(f1).to(DEV)), yd)]
+        grad1 = back(s, r)
+    finally:
+        s.set_backprojector_mode("petl")
+    assert np.linalg.norm(f1 - want) / np.linalg.norm(want) < 1e-5
+    cos = dot(grad1, ATy) / np.sqrt(dot(grad1, grad1) * dot(ATy, ATy))
+    assert abs(cos) < 1e-4, cos
+
+
+def test_rwls_full_size_cost_decreases(full, full_data):
+    """RWLS as the reference runs it (default weights, SQS, TV with beta = 0.02, PETL back projector),
+    5 iterations: finite and non-negative, and the printed cost ends below where it started."""
+    s, _ = full
+    f, cost = s.RWLS(full_data, None, 5, delta=0.5, beta=0.02, preconditioner="SQS", return_cost=True)
+    assert len(cost) == 5 and np.all(np.isfinite(cost))
+    assert np.all(np.isfinite(f)) and np.all(f >= 0) and float(f.max()) > 0
+    assert cost[-1] < cost[0]
