@@ -29,7 +29,8 @@ void launch_dot(const float* x, const float* y, const float* w, int64_t n, doubl
 void launch_tv_reduce(const float* f, const float* d, int N1, int N2, int N3, float delta, float beta, float p,
                       double* scratch, double* out, hipStream_t s);
 
-// ---- conjugate-gradient vector updates (fp32, contraction off: every product rounds as numpy's) ----
+// ---- conjugate-gradient vector updates (fp32, contraction off: every product rounds as numpy's; float4
+//      loads, several per thread in flight, where the pointers are 16-byte aligned) ----
 void launch_sub(float* x, const float* y, int64_t n, hipStream_t s);                  // x -= y
 void launch_add(float* x, const float* y, int64_t n, hipStream_t s);                  // x += y
 void launch_mul_to(float* dst, const float* a, const float* b, int64_t n, hipStream_t s);  // dst = a * b
@@ -41,5 +42,8 @@ void launch_resid_update(float* r, const float* Pd, const float* g, float c, int
 // x = (x == 0 ? dz : 1 / x), the quotient formed in double as dataCube's RECIPROCAL does
 void launch_reciprocal(float* x, float dz, int64_t n, hipStream_t s);
 void launch_clip_neg(float* x, int64_t n, hipStream_t s);  // x < 0 -> 0
+void launch_mul_inplace(float* x, const float* y, int64_t n, hipStream_t s);  // x *= y
+void launch_copy_vec(float* dst, const float* src, int64_t n, hipStream_t s);  // dst = src
+void launch_fill_vec(float* x, float v, int64_t n, hipStream_t s);             // x = v
 
 }  // namespace petl

@@ -21,6 +21,8 @@ constexpr int VT = 256;
 
 inline int grid_for(int64_t n) { return int(std::min<int64_t>((n + VT - 1) / VT, 65536)); }
 
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 // ---------------------------------------------------------------------------------------------
 // Gaussian smoothing along one axis of an [outer][n][inner] array
 // ---------------------------------------------------------------------------------------------
@@ -65,12 +67,6 @@ void gauss_pass(const float* in, float* out, int64_t outer, int n, int64_t inner
     PETL_LAUNCH_CHECK();
 }
 
-__global__ void __launch_bounds__(VT) copy_kernel(float* __restrict__ x, const float* __restrict__ y, int64_t n)
-{
-    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
-        x[i] = y[i];
-}
-
 // ---------------------------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------------------------
@@ -88,14 +84,56 @@ __device__ __forceinline__ void block_sum_store(double acc, double* __restrict__
         *dst = red[0];
 }
 
+// per-block partial of sum double(x_i * y_i [* w_i]): float4 loads, DOT_U of them per array in flight per
+// thread (n4 = 0 when a pointer is not 16-byte aligned), then the scalar tail from 4 * n4
+constexpr int DOT_U = 4;
+
 template <bool HAS_W>
 __global__ void __launch_bounds__(VT) dot_partial_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                         const float* __restrict__ w, int64_t n,
+                                                         const float* __restrict__ w, int64_t n, int64_t n4,
                                                          double* __restrict__ part)
 {
 #pragma clang fp contract(off)
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    const float4* w4 = reinterpret_cast<const float4*>(w);
+    const int64_t stride = int64_t(gridDim.x) * VT;
     double acc = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
+    for (int64_t k = int64_t(blockIdx.x) * VT + threadIdx.x; k < n4; k += DOT_U * stride) {
+        float4 vx[DOT_U], vy[DOT_U], vw[DOT_U];
+#pragma unroll
+        for (int u = 0; u < DOT_U; u++) {
+            const int64_t kk = k + u * stride;
+            if (kk < n4) {
+                vx[u] = x4[kk];
+                vy[u] = y4[kk];
+                if (HAS_W)
+                    vw[u] = w4[kk];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < DOT_U; u++) {
+            const int64_t kk = k + u * stride;
+            if (kk < n4) {
+                float4 p;
+                p.x = vx[u].x * vy[u].x;
+                p.y = vx[u].y * vy[u].y;
+                p.z = vx[u].z * vy[u].z;
+                p.w = vx[u].w * vy[u].w;
+                if (HAS_W) {
+                    p.x = p.x * vw[u].x;
+                    p.y = p.y * vw[u].y;
+                    p.z = p.z * vw[u].z;
+                    p.w = p.w * vw[u].w;
+                }
+                acc += double(p.x);
+                acc += double(p.y);
+                acc += double(p.z);
+                acc += double(p.w);
+            }
+        }
+    }
+    for (int64_t i = 4 * n4 + int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += stride) {
         float p = x[i] * y[i];
         if (HAS_W)
             p = p * w[i];
@@ -188,45 +226,144 @@ __global__ void __launch_bounds__(VT) tv_partial_kernel(const float* __restrict_
 
 // ---------------------------------------------------------------------------------------------
 // vector updates
+//
+// o[i] = op(o[i], a[i], b[i]) over n elements.  HBM needs many bytes in flight per wave to reach its
+// bandwidth (a scalar-load grid-stride loop peaks near 1.1 TB/s here), so each thread issues EW_U float4
+// loads per array before it computes and stores.  The scalar kernel covers unaligned pointers and the
+// tail of fewer than four elements.
 // ---------------------------------------------------------------------------------------------
-#define PETL_EW_KERNEL(name, params, body)                                                               \
-    __global__ void __launch_bounds__(VT) name params                                                   \
-    {                                                                                                    \
-        _Pragma("clang fp contract(off)") for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; \
-                                               i += int64_t(gridDim.x) * VT) body                        \
-    }
+constexpr int EW_U = 4;
 
-PETL_EW_KERNEL(sub_kernel, (float* __restrict__ x, const float* __restrict__ y, int64_t n), { x[i] = x[i] - y[i]; })
-PETL_EW_KERNEL(add_kernel, (float* __restrict__ x, const float* __restrict__ y, int64_t n), { x[i] = x[i] + y[i]; })
-PETL_EW_KERNEL(mul_to_kernel,
-               (float* __restrict__ dst, const float* __restrict__ a, const float* __restrict__ b, int64_t n),
-               { dst[i] = a[i] * b[i]; })
-PETL_EW_KERNEL(xpay_kernel, (float* __restrict__ d, const float* __restrict__ u, float gamma, int64_t n), {
-    const float t = gamma * d[i];
-    d[i] = u[i] + t;
-})
-PETL_EW_KERNEL(step_kernel, (float* __restrict__ f, const float* __restrict__ d, float step, int clip, int64_t n), {
-    const float t = step * d[i];
-    float v = f[i] - t;
-    if (clip && v < 0.0f)
-        v = 0.0f;
-    f[i] = v;
-})
-PETL_EW_KERNEL(resid_kernel,
-               (float* __restrict__ r, const float* __restrict__ Pd, const float* __restrict__ g, float c, int64_t n),
-               {
-                   const float t = c * Pd[i];
-                   const float v = r[i] + t;
-                   r[i] = v - g[i];
-               })
-PETL_EW_KERNEL(recip_kernel, (float* __restrict__ x, float dz, int64_t n), {
-    const float v = x[i];
-    x[i] = (v == 0.0f) ? dz : float(1.0 / double(v));
-})
-PETL_EW_KERNEL(clip_neg_kernel, (float* __restrict__ x, int64_t n), {
-    if (x[i] < 0.0f)
-        x[i] = 0.0f;
-})
+template <bool READ_O, int NA, typename Op>
+__global__ void __launch_bounds__(VT) ew_vec_kernel(float* __restrict__ o, const float* __restrict__ a,
+                                                    const float* __restrict__ b, int64_t n4, Op op)
+{
+    float4* o4 = reinterpret_cast<float4*>(o);
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    const int64_t stride = int64_t(gridDim.x) * VT;
+    for (int64_t k = int64_t(blockIdx.x) * VT + threadIdx.x; k < n4; k += EW_U * stride) {
+        float4 vo[EW_U], va[EW_U], vb[EW_U];
+#pragma unroll
+        for (int u = 0; u < EW_U; u++) {
+            const int64_t kk = k + u * stride;
+            if (kk < n4) {
+                if (READ_O)
+                    vo[u] = o4[kk];
+                if (NA >= 1)
+                    va[u] = a4[kk];
+                if (NA >= 2)
+                    vb[u] = b4[kk];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < EW_U; u++) {
+            const int64_t kk = k + u * stride;
+            if (kk < n4) {
+                float4 r;
+                r.x = op(READ_O ? vo[u].x : 0.0f, NA >= 1 ? va[u].x : 0.0f, NA >= 2 ? vb[u].x : 0.0f);
+                r.y = op(READ_O ? vo[u].y : 0.0f, NA >= 1 ? va[u].y : 0.0f, NA >= 2 ? vb[u].y : 0.0f);
+                r.z = op(READ_O ? vo[u].z : 0.0f, NA >= 1 ? va[u].z : 0.0f, NA >= 2 ? vb[u].z : 0.0f);
+                r.w = op(READ_O ? vo[u].w : 0.0f, NA >= 1 ? va[u].w : 0.0f, NA >= 2 ? vb[u].w : 0.0f);
+                o4[kk] = r;
+            }
+        }
+    }
+}
+
+template <bool READ_O, int NA, typename Op>
+__global__ void __launch_bounds__(VT) ew_scalar_kernel(float* __restrict__ o, const float* __restrict__ a,
+                                                       const float* __restrict__ b, int64_t first, int64_t n,
+                                                       Op op)
+{
+    for (int64_t i = first + int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        o[i] = op(READ_O ? o[i] : 0.0f, NA >= 1 ? a[i] : 0.0f, NA >= 2 ? b[i] : 0.0f);
+}
+
+template <bool READ_O, int NA, typename Op>
+void ew_launch(float* o, const float* a, const float* b, int64_t n, Op op, hipStream_t s)
+{
+    if (n <= 0)
+        return;
+    const bool vec = aligned16(o) && (NA < 1 || aligned16(a)) && (NA < 2 || aligned16(b));
+    const int64_t n4 = vec ? n / 4 : 0;
+    if (n4 > 0) {
+        const int blocks = int(std::min<int64_t>((n4 + int64_t(VT) * EW_U - 1) / (int64_t(VT) * EW_U), 65536));
+        hipLaunchKernelGGL((ew_vec_kernel<READ_O, NA, Op>), dim3(blocks), dim3(VT), 0, s, o, a, b, n4, op);
+        PETL_LAUNCH_CHECK();
+    }
+    if (4 * n4 < n) {
+        hipLaunchKernelGGL((ew_scalar_kernel<READ_O, NA, Op>), dim3(grid_for(n - 4 * n4)), dim3(VT), 0, s, o, a, b,
+                           4 * n4, n, op);
+        PETL_LAUNCH_CHECK();
+    }
+}
+
+// the operations (contraction off: every product rounds as numpy's float32 arithmetic does)
+struct OpSub {
+    __device__ float operator()(float o, float a, float) const { return o - a; }
+};
+struct OpAdd {
+    __device__ float operator()(float o, float a, float) const { return o + a; }
+};
+struct OpMulTo {
+    __device__ float operator()(float, float a, float b) const
+    {
+#pragma clang fp contract(off)
+        return a * b;
+    }
+};
+struct OpXpay {  // d = u + gamma d   (o = d, a = u)
+    float gamma;
+    __device__ float operator()(float o, float a, float) const
+    {
+#pragma clang fp contract(off)
+        const float t = gamma * o;
+        return a + t;
+    }
+};
+struct OpStep {  // f = f - step d, clipped at 0   (o = f, a = d)
+    float step;
+    int clip;
+    __device__ float operator()(float o, float a, float) const
+    {
+#pragma clang fp contract(off)
+        const float t = step * a;
+        const float v = o - t;
+        return (clip && v < 0.0f) ? 0.0f : v;
+    }
+};
+struct OpResid {  // r = (r + c Pd) - g   (o = r, a = Pd, b = g)
+    float c;
+    __device__ float operator()(float o, float a, float b) const
+    {
+#pragma clang fp contract(off)
+        const float t = c * a;
+        const float v = o + t;
+        return v - b;
+    }
+};
+struct OpRecip {
+    float dz;
+    __device__ float operator()(float o, float, float) const { return (o == 0.0f) ? dz : float(1.0 / double(o)); }
+};
+struct OpClipNeg {
+    __device__ float operator()(float o, float, float) const { return o < 0.0f ? 0.0f : o; }
+};
+struct OpCopy {
+    __device__ float operator()(float, float a, float) const { return a; }
+};
+struct OpFill {
+    float v;
+    __device__ float operator()(float, float, float) const { return v; }
+};
+struct OpMul {
+    __device__ float operator()(float o, float a, float) const
+    {
+#pragma clang fp contract(off)
+        return o * a;
+    }
+};
 
 }  // namespace
 
@@ -266,8 +403,7 @@ void gaussian_filter_device(float* f, float* tmp, int N1, int N2, int N3, float 
     }
     if (src != f) {
         const int64_t n = int64_t(N1) * N2 * N3;
-        hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(VT), 0, s, f, src, n);
-        PETL_LAUNCH_CHECK();
+        launch_copy_vec(f, src, n, s);
     }
 }
 
@@ -277,10 +413,12 @@ void launch_dot(const float* x, const float* y, const float* w, int64_t n, doubl
                 hipStream_t s)
 {
     const int nparts = int(red_scratch_doubles(n));
+    const bool vec = aligned16(x) && aligned16(y) && (!w || aligned16(w));
+    const int64_t n4 = vec ? n / 4 : 0;
     if (w)
-        hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(nparts), dim3(VT), 0, s, x, y, w, n, scratch);
+        hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(nparts), dim3(VT), 0, s, x, y, w, n, n4, scratch);
     else
-        hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(nparts), dim3(VT), 0, s, x, y, w, n, scratch);
+        hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(nparts), dim3(VT), 0, s, x, y, w, n, n4, scratch);
     PETL_LAUNCH_CHECK();
     hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(VT), 0, s, scratch, nparts, out);
     PETL_LAUNCH_CHECK();
@@ -312,31 +450,40 @@ void launch_tv_reduce(const float* f, const float* d, int N1, int N2, int N3, fl
     PETL_LAUNCH_CHECK();
 }
 
-#define PETL_EW_LAUNCH(kernel, n, ...)                                                      \
-    do {                                                                                    \
-        hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(VT), 0, s, __VA_ARGS__, n);      \
-        PETL_LAUNCH_CHECK();                                                                \
-    } while (0)
-
-void launch_sub(float* x, const float* y, int64_t n, hipStream_t s) { PETL_EW_LAUNCH(sub_kernel, n, x, y); }
-void launch_add(float* x, const float* y, int64_t n, hipStream_t s) { PETL_EW_LAUNCH(add_kernel, n, x, y); }
+void launch_sub(float* x, const float* y, int64_t n, hipStream_t s) { ew_launch<true, 1>(x, y, nullptr, n, OpSub{}, s); }
+void launch_add(float* x, const float* y, int64_t n, hipStream_t s) { ew_launch<true, 1>(x, y, nullptr, n, OpAdd{}, s); }
 void launch_mul_to(float* dst, const float* a, const float* b, int64_t n, hipStream_t s)
 {
-    PETL_EW_LAUNCH(mul_to_kernel, n, dst, a, b);
+    ew_launch<false, 2>(dst, a, b, n, OpMulTo{}, s);
 }
 void launch_xpay(float* d, const float* u, float gamma, int64_t n, hipStream_t s)
 {
-    PETL_EW_LAUNCH(xpay_kernel, n, d, u, gamma);
+    ew_launch<true, 1>(d, u, nullptr, n, OpXpay{gamma}, s);
 }
 void launch_step(float* f, const float* d, float step, bool clip, int64_t n, hipStream_t s)
 {
-    PETL_EW_LAUNCH(step_kernel, n, f, d, step, clip ? 1 : 0);
+    ew_launch<true, 1>(f, d, nullptr, n, OpStep{step, clip ? 1 : 0}, s);
 }
 void launch_resid_update(float* r, const float* Pd, const float* g, float c, int64_t n, hipStream_t s)
 {
-    PETL_EW_LAUNCH(resid_kernel, n, r, Pd, g, c);
+    ew_launch<true, 2>(r, Pd, g, n, OpResid{c}, s);
 }
-void launch_reciprocal(float* x, float dz, int64_t n, hipStream_t s) { PETL_EW_LAUNCH(recip_kernel, n, x, dz); }
-void launch_clip_neg(float* x, int64_t n, hipStream_t s) { PETL_EW_LAUNCH(clip_neg_kernel, n, x); }
+void launch_reciprocal(float* x, float dz, int64_t n, hipStream_t s)
+{
+    ew_launch<true, 0>(x, nullptr, nullptr, n, OpRecip{dz}, s);
+}
+void launch_clip_neg(float* x, int64_t n, hipStream_t s) { ew_launch<true, 0>(x, nullptr, nullptr, n, OpClipNeg{}, s); }
+void launch_mul_inplace(float* x, const float* y, int64_t n, hipStream_t s)
+{
+    ew_launch<true, 1>(x, y, nullptr, n, OpMul{}, s);
+}
+void launch_copy_vec(float* dst, const float* src, int64_t n, hipStream_t s)
+{
+    ew_launch<false, 1>(dst, src, nullptr, n, OpCopy{}, s);
+}
+void launch_fill_vec(float* x, float v, int64_t n, hipStream_t s)
+{
+    ew_launch<false, 0>(x, nullptr, nullptr, n, OpFill{v}, s);
+}
 
 }  // namespace petl

@@ -306,6 +306,7 @@ bool run_rwls(Model& m, float** g, float* f, int numIter, float delta, float bet
     prm.sqs = preconditioner == 1;
     prm.nonneg = nonneg;
     prm.response = response != nullptr;
+    prm.cost = cost != nullptr;
     // start rules (src/petl_server.py:1743-1759): a nonzero image has its negatives clipped (in place) when
     // constrained and is projected; an all-zero image starts from Pf = 0
     const int64_t V = m.vol_elems();

@@ -80,10 +80,11 @@ struct Model {
     hipStream_t stream();
     void prepare();  // (re)build per-planogram kernel plans when geometry changed
     bool uses_fast(int i);
-    // g_i = A_i f for all planograms (device pointers), with optional ratio epilogue (y, resp lists may
-    // be null).  v1 subset (first, stride) restricts the views; (0,1) = all.
+    // g_i = A_i f for all planograms (device pointers), with an optional epilogue (ratio, or residual with
+    // the weights wts; the y, resp, wts lists may be null).  v1 subset (first, stride) restricts the views;
+    // (0,1) = all.
     void forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
-                 int subset = 0, int nsubsets = 1);
+                 int subset = 0, int nsubsets = 1, float* const* wts = nullptr);
     // f = sum_i A_i^T g_i (overwrites f)
     void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
     // The same with the volume in storage format sf and the planograms in sg (STORE_*; fast kernels only,

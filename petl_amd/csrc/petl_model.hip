@@ -196,17 +196,18 @@ bool Model::uses_fast(int i)
 }
 
 void Model::forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
-                    int subset, int nsubsets)
+                    int subset, int nsubsets, float* const* wts)
 {
     prepare();
     const VolGeom V = vgeom();
     for (size_t i = 0; i < plans.size(); i++) {
         const float* yi = y ? y[i] : nullptr;
         const float* ri = resp ? resp[i] : nullptr;
+        const float* wi = wts ? wts[i] : nullptr;
         if (dev.fast[i])
-            fast_forward(*dev.fast[i], g[i], f, epilogue, yi, ri, s, subset, nsubsets);
+            fast_forward(*dev.fast[i], g[i], f, epilogue, yi, ri, s, subset, nsubsets, wi);
         else
-            launch_fwd_generic(g[i], f, plans[i].geom(), V, epilogue, yi, ri, s, subset, nsubsets);
+            launch_fwd_generic(g[i], f, plans[i].geom(), V, epilogue, yi, ri, s, subset, nsubsets, wi);
     }
 }
 

@@ -47,8 +47,9 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V);
 
 // g = A f (+ epilogue), one planogram.
 // Views are restricted to v1 rows first, first+stride, ... (OSEM subsets); (0,1) = all.
+// wts: weights of the EPI_RESID epilogue (nullable).
 void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
-                  hipStream_t s, int v1_first = 0, int v1_stride = 1);
+                  hipStream_t s, int v1_first = 0, int v1_stride = 1, const float* wts = nullptr);
 // The same with the volume in storage format sf and the planogram and data y in sg (STORE_*; pairs: equal,
 // or an fp32 volume with 16-bit planograms): x = stored * scale for STORE_F16 (f_scale, y_scale for the
 // inputs, g_scale for the output); scales are ignored otherwise.

@@ -77,6 +77,7 @@ struct FwdArgs {
     int v1_first, v1_stride;
     int xrow;  // padded row length of the u0 table
     float f_scale, y_scale, g_inv_scale;  // storage scales (16-bit formats; 1 for fp32)
+    const float* wts;                     // EPI_RESID weights (nullable)
 };
 
 // =============================================================================================
@@ -186,6 +187,21 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
                     TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                    if (epilogue == EPI_RESID) {
+                        // residual of a zero projection: (resp * 0 - y) * w
+                        for (int p = lane; p < Nu0; p += 64) {
+#pragma clang fp contract(off)
+                            const int64_t gi = int64_t(gr - g) + p;
+                            float out = 0.0f;
+                            if (resp)
+                                out = out * resp[gi];
+                            out = out - Stor<SG>::ld(yv[gi], A.y_scale);
+                            if (A.wts)
+                                out = out * A.wts[gi];
+                            gr[p] = Stor<SG>::st(out, A.g_inv_scale);
+                        }
+                        continue;
+                    }
                     for (int p = lane; p < Nu0; p += 64)
                         gr[p] = Stor<SG>::st(0.0f, 1.0f);
                 }
@@ -322,6 +338,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                         wn = w0;
                     const int p = 64 * c + lane;
                     if (p < Nu0) {
+#pragma clang fp contract(off)
                         float out = (Q[r][j][c] + wn) * sc;
                         if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
                             out = 0.0f;
@@ -332,6 +349,10 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                             out = (out == 0.0f) ? 0.0f : Stor<SG>::ld(yv[gi], A.y_scale) / out;
                             if (resp)
                                 out = out * resp[gi];
+                        } else if (epilogue == EPI_RESID) {
+                            out = out - Stor<SG>::ld(yv[gi], A.y_scale);
+                            if (A.wts)
+                                out = out * A.wts[gi];
                         }
                         gr[p] = Stor<SG>::st(out, A.g_inv_scale);
                     }
@@ -806,7 +827,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
 template <int SF, int SG>
 static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
                            const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
-                           float y_scale, float g_scale)
+                           float y_scale, float g_scale, const float* wts = nullptr)
 {
     using TG = typename Stor<SG>::T;
     using TF = typename Stor<SF>::T;
@@ -827,6 +848,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.f_scale = (SF == STORE_F16) ? f_scale : 1.0f;
     A.y_scale = (SG == STORE_F16) ? y_scale : 1.0f;
     A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
+    A.wts = wts;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
     fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C);
     if (!fn || fp.J != FWD_J || fp.R != 1)
@@ -839,9 +861,10 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
 }
 
 void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, const float* y, const float* resp,
-                  hipStream_t s, int v1_first, int v1_stride)
+                  hipStream_t s, int v1_first, int v1_stride, const float* wts)
 {
-    fast_forward_t<STORE_F32, STORE_F32>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, 1.0f, 1.0f, 1.0f);
+    fast_forward_t<STORE_F32, STORE_F32>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, 1.0f, 1.0f, 1.0f,
+                                         wts);
 }
 
 // (volume, planogram) formats: both fp32, both 16-bit of one kind, or an fp32 volume with 16-bit planograms

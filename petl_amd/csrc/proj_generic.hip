@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(256) fwd_generic_kernel(float* __restrict__ g,
                                                           PlanGeom P, VolGeom V, int v1_first, int v1_stride,
                                                           int v1_chunk, int epilogue,
                                                           const float* __restrict__ y,
-                                                          const float* __restrict__ resp)
+                                                          const float* __restrict__ resp,
+                                                          const float* __restrict__ wts)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,6 +166,11 @@ __global__ void __launch_bounds__(256) fwd_generic_kernel(float* __restrict__ g,
         out = (out == 0.0f) ? 0.0f : y[gi] / out;
         if (resp)
             out = out * resp[gi];
+    } else if (epilogue == EPI_RESID) {
+        // Pf - g, then the in-place weighting of src/petl_server.py:1790-1792
+        out = out - y[gi];
+        if (wts)
+            out = out * wts[gi];
     }
     g[gi] = out;
 }
@@ -379,14 +385,15 @@ __global__ void __launch_bounds__(256) back_matched_generic_kernel(const float* 
 }  // namespace
 
 void launch_fwd_generic(float* g, const float* f, const PlanGeom& P, const VolGeom& V, int epilogue,
-                        const float* y, const float* resp, hipStream_t s, int v1_first, int v1_stride)
+                        const float* y, const float* resp, hipStream_t s, int v1_first, int v1_stride,
+                        const float* wts)
 {
     const int nb = ceil_div(int64_t(P.Nu1) * P.Nu0, 256);
     const int nv1 = (P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
     for (int c = 0; c < nv1; c += 65535) {
         const int n = min(65535, nv1 - c);
         hipLaunchKernelGGL(fwd_generic_kernel, dim3(nb, P.Nv0, n), dim3(256), 0, s, g, f, P, V, v1_first, v1_stride, c,
-                           epilogue, y, resp);
+                           epilogue, y, resp, wts);
         PETL_LAUNCH_CHECK();
     }
 }

@@ -21,6 +21,7 @@ struct RwlsParams {
     bool sqs = false;     // separable quadratic surrogate preconditioner
     bool nonneg = true;   // nonnegativity constraint
     bool response = false;  // projectors include the response (project -> R Pf, backproject(x) -> P*(R x))
+    bool cost = false;      // the cost is read after every iteration (keeps the residual unweighted)
 };
 
 class RwlsJob {
@@ -38,6 +39,8 @@ public:
 
 private:
     void project(const float* x, std::vector<float*>& out, hipStream_t s);
+    // r = R P x - g, times W when weigh (one forward with the residual epilogue)
+    void project_residual(const float* x, bool weigh, hipStream_t s);
     void backproject(std::vector<float*>& y, bool y_is_scratch, float* out, hipStream_t s);
     float dot_vol(const float* x, const float* y, int slot, hipStream_t s);
     Model& m_;
@@ -51,6 +54,7 @@ private:
     double* slots_ = nullptr;    // device result slots
     double* scratch_ = nullptr;  // reduction partials
     double gg_old_ = 0.0;
+    bool r_weighted_ = false;  // r already holds W r (weighting fused into the residual projection)
 };
 
 }  // namespace petl

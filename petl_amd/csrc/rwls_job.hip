@@ -39,6 +39,13 @@ void RwlsJob::project(const float* x, std::vector<float*>& out, hipStream_t s)
     m_.forward(out.data(), x, EPI_NONE, nullptr, prm_.response ? respp_.data() : nullptr, s);
 }
 
+void RwlsJob::project_residual(const float* x, bool weigh, hipStream_t s)
+{
+    m_.forward(rp_.data(), x, EPI_RESID, gp_.data(), prm_.response ? respp_.data() : nullptr, s, 0, 1,
+               weigh ? Wp_.data() : nullptr);
+    r_weighted_ = weigh;
+}
+
 // out = sum_i P_i^T (R_i y_i); the response product goes to Pd (or into y itself when y is scratch), as the
 // reference multiplies a copy of the data (src/petl_server.py:743-746)
 void RwlsJob::backproject(std::vector<float*>& y, bool y_is_scratch, float* out, hipStream_t s)
@@ -102,31 +109,33 @@ void RwlsJob::setup(float* const* g, const float* f0, bool f0_nonzero, float* co
     if (prm_.weights == RWLS_W_DEFAULT)
         for (size_t i = 0; i < nP_; i++) {
             const Planogram& pl = m.plans[i];
-            launch_copy(Wp_[i], gp_[i], G_[i], s);
+            launch_copy_vec(Wp_[i], gp_[i], G_[i], s);
             gaussian_filter_device(Wp_[i], rp_[i], pl.N_v1 * pl.N_v0, pl.N_u1, pl.N_u0, 3.0f, 2, true, s);
         }
 
     // Pf - g, with Pf = 0 for an all-zero start
-    if (f0_nonzero)
-        project(f_.f(), rp_, s);
-    else
-        for (size_t i = 0; i < nP_; i++)
-            launch_fill(rp_[i], 0.0f, G_[i], s);
-    for (size_t i = 0; i < nP_; i++)
-        launch_sub(rp_[i], gp_[i], G_[i], s);
+    if (f0_nonzero) {
+        project_residual(f_.f(), false, s);
+    } else {
+        for (size_t i = 0; i < nP_; i++) {
+            launch_fill_vec(rp_[i], 0.0f, G_[i], s);
+            launch_sub(rp_[i], gp_[i], G_[i], s);
+        }
+        r_weighted_ = false;
+    }
 
     // SQS: Q = 1 / P*(W P 1), 0 -> 1
     if (prm_.sqs) {
-        launch_fill(Q_.f(), 1.0f, V_, s);
+        launch_fill_vec(Q_.f(), 1.0f, V_, s);
         project(Q_.f(), Pdp_, s);
         if (weighted)
             for (size_t i = 0; i < nP_; i++)
-                launch_mul(Pdp_[i], Wp_[i], G_[i], s);
+                launch_mul_inplace(Pdp_[i], Wp_[i], G_[i], s);
         backproject(Pdp_, true, Q_.f(), s);
         launch_reciprocal(Q_.f(), 1.0f, V_, s);
     }
-    launch_fill(d_.f(), 0.0f, V_, s);
-    launch_fill(grad_old_.f(), 0.0f, V_, s);
+    launch_fill_vec(d_.f(), 0.0f, V_, s);
+    launch_fill_vec(grad_old_.f(), 0.0f, V_, s);
     gg_old_ = 0.0;
 }
 
@@ -143,10 +152,12 @@ float RwlsJob::dot_vol(const float* x, const float* y, int slot, hipStream_t s)
 bool RwlsJob::iterate(int n, hipStream_t s)
 {
     const bool weighted = prm_.weights != RWLS_W_SCALAR;
-    // WPf_minus_g aliases Pf_minus_g: the weighting is in place (src/petl_server.py:1790-1792)
-    if (weighted)
+    // WPf_minus_g aliases Pf_minus_g: the weighting is in place (src/petl_server.py:1790-1792); the
+    // residual projection of the previous step may have applied it already
+    if (weighted && !r_weighted_)
         for (size_t i = 0; i < nP_; i++)
-            launch_mul(rp_[i], Wp_[i], G_[i], s);
+            launch_mul_inplace(rp_[i], Wp_[i], G_[i], s);
+    r_weighted_ = weighted;
     backproject(rp_, false, grad_.f(), s);
     if (prm_.beta > 0.0f) {
         launch_tv_gradient(f_.f(), u_.f(), m_.numZ, m_.numY, m_.numX, prm_.delta, prm_.beta, 1.0f, s);
@@ -155,11 +166,11 @@ bool RwlsJob::iterate(int n, hipStream_t s)
     if (prm_.sqs)
         launch_mul_to(u_.f(), Q_.f(), grad_.f(), V_, s);
     else
-        launch_copy(u_.f(), grad_.f(), V_, s);
+        launch_copy_vec(u_.f(), grad_.f(), V_, s);
 
     float u_grad;
     if (n % 50 == 0) {
-        launch_copy(d_.f(), u_.f(), V_, s);
+        launch_copy_vec(d_.f(), u_.f(), V_, s);
         u_grad = dot_vol(u_.f(), grad_.f(), 0, s);
     } else {
         launch_dot(u_.f(), grad_.f(), nullptr, V_, scratch_, slots_ + 0, s);
@@ -173,10 +184,10 @@ bool RwlsJob::iterate(int n, hipStream_t s)
         const double gamma = (double(float(h[0])) - double(float(h[1]))) / gg_old_;
         launch_xpay(d_.f(), u_.f(), float(gamma), V_, s);
         if (dot_vol(d_.f(), grad_.f(), 2, s) <= 0.0f)
-            launch_copy(d_.f(), u_.f(), V_, s);
+            launch_copy_vec(d_.f(), u_.f(), V_, s);
     }
     gg_old_ = double(u_grad);
-    launch_copy(grad_old_.f(), grad_.f(), V_, s);
+    launch_copy_vec(grad_old_.f(), grad_.f(), V_, s);
 
     project(d_.f(), Pdp_, s);
     // num = <d, grad>; denominator <Pd, Pd>_W (+ <d, R''(f) d>), all read back at once
@@ -201,20 +212,22 @@ bool RwlsJob::iterate(int n, hipStream_t s)
 
     launch_step(f_.f(), d_.f(), float(step), prm_.nonneg, V_, s);
     if (prm_.nonneg) {
-        project(f_.f(), rp_, s);
-        for (size_t i = 0; i < nP_; i++)
-            launch_sub(rp_[i], gp_[i], G_[i], s);
+        // Pf - g and the next iteration's weighting in one forward (unless the cost needs r unweighted)
+        project_residual(f_.f(), weighted && !prm_.cost, s);
     } else {
         // scalarAdd(Pf, -step, Pd) then subtract(Pf_minus_g, g) on the same (already weighted) array
         // (src/petl_server.py:1831-1834)
         for (size_t i = 0; i < nP_; i++)
             launch_resid_update(rp_[i], Pdp_[i], gp_[i], float(-step), G_[i], s);
+        r_weighted_ = false;
     }
     return true;
 }
 
 double RwlsJob::cost(hipStream_t s)
 {
+    if (r_weighted_)
+        throw hip_error("RWLS: the cost needs the unweighted residual (set RwlsParams::cost)");
     const bool weighted = prm_.weights != RWLS_W_SCALAR;
     for (size_t i = 0; i < nP_; i++)
         launch_dot(rp_[i], rp_[i], weighted ? Wp_[i] : nullptr, G_[i], scratch_, slots_ + 2 + i, s);

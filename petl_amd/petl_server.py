@@ -673,13 +673,15 @@ class PETLserver:
         sqs = preconditioner is True or preconditioner == 'SQS'
         resp = self.floatstarstar(self.response) if (self.response is not None and include_response_with_projectors) \
             else None
+        want_cost = return_cost or self.print_cost
         cost = np.zeros(max(int(numIter), 1), dtype=np.float64)
         done = ctypes.c_int(0)
         self.set_model()
         ok = self.libprojectors.RWLS_gpu(self.floatstarstar(g), f, int(numIter), float(delta), float(beta), mode,
                                          self.floatstarstar(Wl) if Wl is not None else None, ws, 1 if sqs else 0,
                                          bool(nonnegativityConstraint), resp,
-                                         cost.ctypes.data_as(ctypes.c_void_p), ctypes.byref(done))
+                                         cost.ctypes.data_as(ctypes.c_void_p) if want_cost else None,
+                                         ctypes.byref(done))
         if not ok:
             raise TypeError('RWLS failed')
         if self.print_warnings:
