@@ -27,6 +27,13 @@ class Plan(ctypes.Structure):
         return (self.N_v1, self.N_v0, self.N_u1, self.N_u0)
 
 
+class Solid(ctypes.Structure):
+    """A geometric solid as geometricObject::init builds it (src/phantom.cpp:535-640)."""
+    _fields_ = [("type", ctypes.c_int), ("c", ctypes.c_float * 3), ("r", ctypes.c_float * 3),
+                ("val", ctypes.c_float), ("A", ctypes.c_float * 9), ("clip", (ctypes.c_float * 4) * 6),
+                ("rotated", ctypes.c_int), ("nclip", ctypes.c_int), ("cone", ctypes.c_float * 2)]
+
+
 class Vol(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("numX", "numY", "numZ")] + \
                [(n, ctypes.c_float) for n in ("voxelWidth", "voxelHeight", "offsetX", "offsetY", "offsetZ",
@@ -80,6 +87,14 @@ def lib():
         L.orc_tv_quadform_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_float, c_float]
         L.orc_tv_quadform_f64.restype = c_double
         L.orc_gaussian_filter_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_float, c_int]
+        S = ctypes.POINTER(Solid)
+        L.orc_solid_init.argtypes = [c_int, fp, fp, c_float, ctypes.c_void_p, ctypes.c_void_p, S]
+        L.orc_solid_init.restype = c_int
+        L.orc_intersect.argtypes = [S, fp, fp, fp]
+        L.orc_intersect.restype = c_int
+        L.orc_ray_value.argtypes = [S, c_int, fp, fp, c_int]
+        L.orc_ray_value.restype = c_float
+        L.orc_ray_trace.argtypes = [P, S, c_int, c_int, c_int, fp, c_int]
         L.orc_max_threads.restype = c_int
         L.orc_project_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
         L.orc_backproject_petl_rows_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
@@ -375,6 +390,66 @@ def rwls(plans, vol, g, num_iter, f=None, delta=1.0, beta=0.0, W=None, sqs=False
     return f, done
 
 
+# --------------------------------------------------------------------------------------------------
+# response model: geometric solids, ray tracing, stopping power, solid angle
+# --------------------------------------------------------------------------------------------------
+def solid(type, c, r, val, A=None, clip=None):
+    """type 0 ellipsoid, 1 parallelepiped, 2/3/4 cylinder x/y/z, 5/6/7 cone x/y/z; c, r: 3 floats; A: 3x3
+    rotation (row-major); clip: 3 floats (non-zero entries clip the solid to coordinate <= value)."""
+    out = Solid()
+    cc = np.ascontiguousarray(c, np.float32)
+    rr = np.ascontiguousarray(r, np.float32)
+    Aa = None if A is None else np.ascontiguousarray(A, np.float32).reshape(9)
+    cl = None if clip is None else np.ascontiguousarray(clip, np.float32)
+    ok = lib().orc_solid_init(int(type), cc, rr, float(val), None if Aa is None else Aa.ctypes.data,
+                              None if cl is None else cl.ctypes.data, ctypes.byref(out))
+    if not ok:
+        raise ValueError("invalid solid")
+    return out
+
+
+def _solid_array(solids):
+    arr = (Solid * max(len(solids), 1))()
+    for i, s in enumerate(solids):
+        arr[i] = s
+    return arr
+
+
+def intersect(s, p, r):
+    ts = np.zeros(2, np.float32)
+    hit = lib().orc_intersect(ctypes.byref(s), np.asarray(p, np.float32), np.asarray(r, np.float32), ts)
+    return bool(hit), float(ts[0]), float(ts[1])
+
+
+def ray_value(solids, p, r, stopping=False):
+    arr = _solid_array(solids)
+    return float(lib().orc_ray_value(arr, len(solids), np.asarray(p, np.float32), np.asarray(r, np.float32),
+                                     int(bool(stopping))))
+
+
+def ray_trace(plan, solids, oversampling=1, stopping=False, nthreads=None):
+    """Line integrals (stopping=False) or stopping powers (True) of every bin of one planogram."""
+    g = np.zeros(plan.shape, np.float32)
+    arr = _solid_array(solids)
+    lib().orc_ray_trace(ctypes.byref(plan), arr, len(solids), int(oversampling), int(bool(stopping)), g,
+                        _nt(nthreads))
+    return g
+
+
+def solid_angle_factor(plan, do_inverse=False):
+    """planogram::solid_angle_correction_factor per (v1, v0) (src/planogram.cpp:262-265): 2 pi (1 + v0^2 +
+    v1^2)^1.5 / (T_v1 T_v0), or its reciprocal.  Returns float32 (N_v1, N_v0)."""
+    v1 = (np.arange(plan.N_v1, dtype=np.float32) * np.float32(plan.T_v1) + np.float32(plan.v1_0)).astype(np.float32)
+    v0 = (np.arange(plan.N_v0, dtype=np.float32) * np.float32(plan.T_v0) + np.float32(plan.v0_0)).astype(np.float32)
+    V1, V0 = np.meshgrid(v1, v0, indexing="ij")
+    base = 1.0 + (V0 * V0).astype(np.float64) + (V1 * V1).astype(np.float64)
+    c = (2.0 * 3.1415926535897932385 * base ** 1.5 / float(np.float32(plan.T_v1) * np.float32(plan.T_v0)))
+    c = c.astype(np.float32)
+    if do_inverse:
+        c = (1.0 / c.astype(np.float64)).astype(np.float32)
+    return c
+
+
 def osem(plans, vol, g, num_iter, num_subsets, f0=None, nthreads=None):
     """OSEM with interleaved v1-row subsets (rows s, s+n, ...), fp32, PETL-mode back projection; one
     iteration = one pass over the subsets.  f0: start image (default ones).  With one subset and the
@@ -508,5 +583,7 @@ def ref_lib():
         R.ref_inner3d.restype = c_float
         R.ref_event_coords.argtypes = [c_float] * 7 + [fp, fp, fp]
         R.ref_event_coords.restype = c_int
+        R.ref_solid_angle.argtypes = [c_float] * 7 + [c_int, fp]
+        R.ref_solid_angle.restype = c_int
         _ref = R
     return _ref

@@ -24,6 +24,8 @@
 //     p-Huber TV gradient         src/total_variation.cu:21-78, 434-660
 //     p-Huber TV cost, quad. form src/total_variation.cu:80-216, 218-417, 662-800
 //     Gaussian smoothing          src/relative_differences.cu:314-420, 753-800
+//     geometric solids            src/phantom.cpp:502-640 (init), 1120-1141 (scale)
+//     ray tracing, stopping power src/analytic_ray_tracing_gpu.cu:48-862
 //
 //  Deliberate differences from the reference:
 //     * texture fetches are replaced by exact array reads: the forward samples texel centres (exact in
@@ -55,6 +57,18 @@ typedef struct orc_vol {
     float offsetX, offsetY, offsetZ;
     float x0, y0, z0;
 } orc_vol;
+
+// one geometric solid of a phantom or of the scintillator module model (src/phantom.h:17-50)
+typedef struct orc_solid {
+    int type;  // 0 ellipsoid, 1 parallelepiped, 2-4 cylinder x/y/z, 5-7 cone x/y/z
+    float c[3], r[3];
+    float val;
+    float A[9];
+    float clip[6][4];
+    int rotated;
+    int nclip;
+    float cone[2];
+} orc_solid;
 
 }  // extern "C"
 
@@ -669,6 +683,413 @@ void gaussian_filter(const double* f, double* out, int N1, int N2, int N3, float
             }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Analytic ray tracing through geometric solids: the line integrals of ray_trace and the stopping power of
+// the scintillator modules (src/analytic_ray_tracing_gpu.cu:48-862, the GPU path the reference runs), with
+// the solids built as geometricObject::init builds them (src/phantom.cpp:502-640).  The float / double mix
+// of the kernels is kept: every quantity is formed in the type the reference forms it in.
+// ------------------------------------------------------------------------------------------------
+const float OOB = 1.0e12f;
+
+bool solid_init(int type, const float* c, const float* r, float val, const float* A, const float* clip, orc_solid* o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->type = -1;
+    if (c == nullptr || r == nullptr || type < 0 || type > 7)
+        return false;
+    o->type = type;
+    for (int i = 0; i < 3; i++) {
+        o->c[i] = c[i];
+        o->r[i] = r[i];
+    }
+    if (clip != nullptr) {
+        o->nclip = 0;
+        for (int i = 0; i < 3; i++)
+            if (clip[i] != 0.0f) {
+                o->clip[o->nclip][i] = -1.0f;
+                o->clip[o->nclip][3] = -clip[i];
+                o->nclip += 1;
+            }
+    }
+    o->val = val;
+    o->rotated = 0;
+    if (A != nullptr)
+        for (int i = 0; i < 9; i++) {
+            o->A[i] = A[i];
+            if (i == 0 || i == 4 || i == 8) {
+                if (std::fabs(double(A[i]) - 1.0) > 1.0e-8)
+                    o->rotated = 1;
+            } else if (std::fabs(double(A[i])) > 1.0e-8) {
+                o->rotated = 1;
+            }
+        }
+    // cones: axis `ax` with end radii r_1, r_2 and half length l become the unit cone about the apex
+    auto cone = [&](int ax, int i1, int i2, int il) {
+        const double l = o->r[il], r_1 = o->r[i1], r_2 = o->r[i2];
+        const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+        o->r[o1] = float(std::fabs(r_2 - r_1) / (2.0 * l));
+        o->r[o2] = o->r[o1];
+        o->r[ax] = 1.0f;
+        o->c[ax] = float(o->c[ax] - l * (r_2 + r_1) / (r_2 - r_1));
+        o->cone[0] = float(-l + l * (r_2 + r_1) / (r_2 - r_1));
+        o->cone[1] = float(l + l * (r_2 + r_1) / (r_2 - r_1));
+    };
+    if (type == 5)
+        cone(0, 1, 2, 0);  // CONE_X: radii = (l, r_1, r_2)
+    else if (type == 6)
+        cone(1, 0, 2, 1);  // CONE_Y: radii = (r_1, l, r_2)
+    else if (type == 7)
+        cone(2, 0, 1, 2);  // CONE_Z: radii = (r_1, r_2, l)
+    return true;
+}
+
+struct f2 {
+    float x, y;
+};
+struct d3 {
+    double x, y, z;
+};
+
+bool p1d(f2& ts, float p, float r)
+{
+    if (std::fabs(r) < 1e-12f) {
+        if (std::fabs(p) < 1.0f) {
+            ts.x = -OOB;
+            ts.y = OOB;
+            return true;
+        }
+        return false;
+    }
+    if (r > 0.0f) {
+        ts.x = (-1.0f - p) / r;
+        ts.y = (1.0f - p) / r;
+    } else {
+        ts.y = (-1.0f - p) / r;
+        ts.x = (1.0f - p) / r;
+    }
+    return true;
+}
+
+bool clip_planes(f2& ts, const float* p, const float* r, const orc_solid& s)
+{
+    for (int i = 0; i < s.nclip; i++) {
+        const float p_dot_n = s.clip[i][0] * p[0] + s.clip[i][1] * p[1] + s.clip[i][2] * p[2];
+        const float r_dot_n = s.clip[i][0] * r[0] + s.clip[i][1] * r[1] + s.clip[i][2] * r[2];
+        if (std::fabs(r_dot_n) < 1.0e-12f) {
+            if (p_dot_n < s.clip[i][3])
+                return false;
+        } else if (r_dot_n > 0.0f) {
+            const float temp = (s.clip[i][3] - p_dot_n) / r_dot_n;
+            if (ts.y < temp)
+                return false;
+            if (temp > ts.x)
+                ts.x = temp;
+        } else {
+            const float temp = (s.clip[i][3] - p_dot_n) / r_dot_n;
+            if (ts.x > temp)
+                return false;
+            if (ts.y > temp)
+                ts.y = temp;
+        }
+    }
+    return true;
+}
+
+// intersectionEndPoints_centeredAndNormalized (src/analytic_ray_tracing_gpu.cu:150-463)
+bool centered(const d3& p, const d3& r, f2& ts, const orc_solid& s)
+{
+    ts.x = -OOB;
+    ts.y = ts.x;
+    const double r_dot_r = r.x * r.x + r.y * r.y + r.z * r.z;
+    const double p_dot_r = r.x * p.x + r.y * p.y + r.z * p.z;
+    if (s.type == 0) {
+        double disc = p_dot_r * p_dot_r + r_dot_r * (1.0 - (p.x * p.x + p.y * p.y + p.z * p.z));
+        if (!(disc > 0.0))
+            return false;
+        disc = std::sqrt(disc);
+        ts.x = float((-p_dot_r - disc) / r_dot_r);
+        ts.y = float((-p_dot_r + disc) / r_dot_r);
+    } else if (s.type == 1) {
+        f2 tx, ty, tz;
+        if (!p1d(tx, float(p.x), float(r.x)) || !p1d(ty, float(p.y), float(r.y)) || !p1d(tz, float(p.z), float(r.z)))
+            return false;
+        ts.x = std::max(std::max(tx.x, ty.x), tz.x);
+        ts.y = std::min(std::min(tx.y, ty.y), tz.y);
+    } else if (s.type >= 2 && s.type <= 4) {
+        // cylinders: ax = axis; the cross section is the unit disc in the other two coordinates
+        const int ax = s.type - 2;  // 2: x, 3: y, 4: z
+        const double pa = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+        const double ra = ax == 0 ? r.x : (ax == 1 ? r.y : r.z);
+        double rr2, pr2, pp2, rz0, rz1;
+        if (ax == 2) {
+            rr2 = r_dot_r - r.z * r.z;
+            pr2 = p_dot_r - p.z * r.z;
+            pp2 = p.x * p.x + p.y * p.y;
+            rz0 = r.x;
+            rz1 = r.y;
+        } else if (ax == 0) {
+            rr2 = r_dot_r - r.x * r.x;
+            pr2 = p.z * r.z + p.y * r.y;
+            pp2 = p.z * p.z + p.y * p.y;
+            rz0 = r.y;
+            rz1 = r.z;
+        } else {
+            rr2 = r_dot_r - r.y * r.y;
+            pr2 = p.x * r.x + p.z * r.z;
+            pp2 = p.x * p.x + p.z * p.z;
+            rz0 = r.x;
+            rz1 = r.z;
+        }
+        double disc = pr2 * pr2 - rr2 * (pp2 - 1.0);
+        if (disc > 0.0) {
+            disc = std::sqrt(disc);
+            const float tmin = float((-pr2 - disc) / rr2);
+            const float tmax = float((-pr2 + disc) / rr2);
+            f2 tz;
+            if (!p1d(tz, float(pa), float(ra)))
+                return false;
+            ts.x = std::max(tmin, tz.x);
+            ts.y = std::min(tmax, tz.y);
+        } else if (rz0 == 0.0 && rz1 == 0.0 && pp2 <= 1.0) {
+            return p1d(ts, float(pa), float(ra));  // returns without the final ordering test, as the reference
+        } else {
+            return false;
+        }
+    } else if (s.type == 5 || s.type == 7) {
+        // CONE_X, CONE_Z: x'^2 + y'^2 = a'^2 about the axis, clipped along it by cone[]
+        const bool zc = s.type == 7;
+        const double pa = zc ? p.z : p.x, ra = zc ? r.z : r.x;
+        const double pb = zc ? p.x : p.z, rb = zc ? r.x : r.z;
+        double a = rb * rb + r.y * r.y - ra * ra;
+        const double b_half = pb * rb + p.y * r.y - pa * ra;
+        const double c = pb * pb + p.y * p.y - pa * pa;
+        double disc = b_half * b_half - a * c;
+        if (!(disc > 0.0))
+            return false;
+        disc = std::sqrt(disc);
+        float tmin = float((-b_half - disc) / a);
+        float tmax = float((-b_half + disc) / a);
+        if (tmin > tmax) {
+            a = tmin;
+            tmin = tmax;
+            tmax = float(a);
+        }
+        const double shift = 0.5 * (double(s.cone[1]) + double(s.cone[0]));
+        const double scale = 0.5 * (double(s.cone[1]) - double(s.cone[0]));
+        f2 tz;
+        if (!p1d(tz, float((pa - shift) / scale), float(ra / scale)))
+            return false;
+        ts.x = std::max(tmin, tz.x);
+        ts.y = std::min(tmax, tz.y);
+    } else if (s.type == 6) {
+        // CONE_Y, with the end-cap handling of src/analytic_ray_tracing_gpu.cu:353-451
+        double a = r.x * r.x + r.z * r.z - r.y * r.y;
+        const double b_half = p.x * r.x + p.z * r.z - p.y * r.y;
+        const double c = p.x * p.x + p.z * p.z - p.y * p.y;
+        double disc = b_half * b_half - a * c;
+        if (disc > 0.0) {
+            disc = std::sqrt(disc);
+            double tmin = (-b_half - disc) / a;
+            double tmax = (-b_half + disc) / a;
+            if (tmin > tmax) {
+                a = tmin;
+                tmin = tmax;
+                tmax = a;
+            }
+            const double shift = 0.5 * (double(s.cone[1]) + double(s.cone[0]));
+            const double scale = 0.5 * (double(s.cone[1]) - double(s.cone[0]));
+            f2 tz;
+            if (p1d(tz, float((p.y - shift) / scale), float(r.y / scale))) {
+                if (std::fabs(r.y) > 1.0e-12f) {
+                    auto inside = [&](float t) {
+                        double xv = p.x + t * r.x, yv = p.y + t * r.y, zv = p.z + t * r.z;
+                        return xv * xv + zv * zv <= yv * yv;
+                    };
+                    const bool in0 = inside(tz.x), in1 = inside(tz.y);
+                    if (in0) {
+                        if (in1) {
+                            ts.x = tz.x;
+                            ts.y = tz.y;
+                        } else if (tz.x <= tmin && tmin <= tz.y) {
+                            ts.x = tz.x;
+                            ts.y = float(tmin);
+                        } else {
+                            ts.x = tz.x;
+                            ts.y = float(tmax);
+                        }
+                    } else {
+                        if (in1) {
+                            if (tz.x <= tmin && tmin <= tz.y) {
+                                ts.x = float(tmin);
+                                ts.y = tz.y;
+                            } else {
+                                ts.x = float(tmax);
+                                ts.y = tz.y;
+                            }
+                        } else if (tz.x <= tmin && tmax <= tz.y) {
+                            ts.x = float(tmin);
+                            ts.y = float(tmax);
+                        } else {
+                            ts.x = -OOB;
+                            ts.y = ts.x;
+                            return false;
+                        }
+                    }
+                } else {
+                    ts.x = float(tmin);
+                    ts.y = float(tmax);
+                }
+            }
+        }
+    } else {
+        ts.x = -OOB;
+        ts.y = -OOB;
+        return false;
+    }
+    if (ts.x >= ts.y) {
+        ts.x = -OOB;
+        ts.y = ts.x;
+        return false;
+    }
+    return true;
+}
+
+// intersectionEndPoints (src/analytic_ray_tracing_gpu.cu:465-518)
+bool intersect(const float* p, const float* r, f2& ts, const orc_solid& s)
+{
+    d3 q, mr;
+    if (!s.rotated) {
+        q.x = (p[0] - s.c[0]) / s.r[0];
+        q.y = (p[1] - s.c[1]) / s.r[1];
+        q.z = (p[2] - s.c[2]) / s.r[2];
+        mr.x = r[0] / s.r[0];
+        mr.y = r[1] / s.r[1];
+        mr.z = r[2] / s.r[2];
+    } else {
+        q.x = p[0] - s.c[0];
+        q.y = p[1] - s.c[1];
+        q.z = p[2] - s.c[2];
+        const double tx = (q.x * s.A[0] + q.y * s.A[1] + q.z * s.A[2]) / s.r[0];
+        const double ty = (q.x * s.A[3] + q.y * s.A[4] + q.z * s.A[5]) / s.r[1];
+        const double tz = (q.x * s.A[6] + q.y * s.A[7] + q.z * s.A[8]) / s.r[2];
+        q.x = tx;
+        q.y = ty;
+        q.z = tz;
+        mr.x = (r[0] * s.A[0] + r[1] * s.A[1] + r[2] * s.A[2]) / s.r[0];
+        mr.y = (r[0] * s.A[3] + r[1] * s.A[4] + r[2] * s.A[5]) / s.r[1];
+        mr.z = (r[0] * s.A[6] + r[1] * s.A[7] + r[2] * s.A[8]) / s.r[2];
+    }
+    if (!centered(q, mr, ts, s))
+        return false;
+    if (!clip_planes(ts, p, r, s)) {
+        ts.x = -OOB;
+        ts.y = ts.x;
+        return false;
+    }
+    return ts.y > ts.x;
+}
+
+// stoppingPower_geometricSolids / lineIntegral_geometricSolids (src/analytic_ray_tracing_gpu.cu:536-691):
+// the union of the solids' chords cut at every end point; each piece belongs to the last-listed solid
+// that contains its mid point
+float ray_value(const float* p, const float* r, const orc_solid* s, int n, bool stopping)
+{
+    std::vector<float> ends;
+    std::vector<float> t0(n), t1(n);
+    std::vector<int> idx;
+    for (int i = 0; i < n; i++) {
+        f2 ts;
+        if (intersect(p, r, ts, s[i])) {
+            ends.push_back(ts.x);
+            ends.push_back(ts.y);
+            t0[i] = ts.x;
+            t1[i] = ts.y;
+            idx.push_back(i);
+        } else {
+            t0[i] = OOB;
+            t1[i] = OOB;
+        }
+    }
+    float fwd = 0.0f, bwd = 0.0f, all = 0.0f;
+    const int count = int(idx.size());
+    if (count > 0) {
+        std::sort(ends.begin(), ends.end());
+        for (int i = 0; i < 2 * count - 1; i++) {
+            const float mid = (ends[i + 1] + ends[i]) * 0.5f;
+            for (int ind = count - 1; ind >= 0; ind--) {
+                const int j = idx[ind];
+                if (t0[j] <= mid && mid <= t1[j]) {
+                    const float piece = s[j].val * (ends[i + 1] - ends[i]);
+                    all += piece;
+                    if (mid >= 0.0f)
+                        fwd += piece;
+                    else
+                        bwd += piece;
+                    break;
+                }
+            }
+        }
+    }
+    if (!stopping)
+        return all;
+    return (1.0f - std::exp(-fwd)) * (1.0f - std::exp(-bwd));
+}
+
+// rayTracingKernel / stoppingPowerKernel (src/analytic_ray_tracing_gpu.cu:693-862) for one planogram
+void ray_trace_plan(const orc_plan& P, const orc_solid* s, int n, int oversampling, bool stopping, float* g,
+                    int nthreads)
+{
+    oversampling = std::max(1, std::min(oversampling, 11));
+    if (oversampling % 2 == 0)
+        oversampling += 1;
+    oversampling = std::max(1, std::min(oversampling, 11));
+    const float cs = float(std::cos(double(P.psi))), sn = float(std::sin(double(P.psi)));
+    const int64_t plane = int64_t(P.N_u1) * P.N_u0;
+#pragma omp parallel for collapse(2) schedule(dynamic) num_threads(nthreads)
+    for (int iv1 = 0; iv1 < P.N_v1; iv1++)
+        for (int iv0 = 0; iv0 < P.N_v0; iv0++) {
+            const float v1 = float(iv1) * P.T_v1 + P.v1_0;
+            const float v0 = float(iv0) * P.T_v0 + P.v0_0;
+            float* out = g + (int64_t(iv1) * P.N_v0 + iv0) * plane;
+            // trajectory: r / rsqrt(r.r), i.e. r scaled by |r| (not normalised), as the kernel computes it
+            float rv[3] = {-(v0 * cs + sn), -(v0 * sn - cs), -v1};
+            const float mag = 1.0f / std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+            rv[0] = rv[0] / mag;
+            rv[1] = rv[1] / mag;
+            rv[2] = rv[2] / mag;
+            for (int iu1 = 0; iu1 < P.N_u1; iu1++)
+                for (int iu0 = 0; iu0 < P.N_u0; iu0++) {
+                    const float u1 = float(iu1) * P.T_u1 + P.u1_0;
+                    const float u0 = float(iu0) * P.T_u0 + P.u0_0;
+                    float& o = out[int64_t(iu1) * P.N_u0 + iu0];
+                    // the u0 test uses T_v0 for the bin half width (the kernel reads T.x)
+                    if (std::fabs(u1) - 0.5f * P.T_u1 > P.H - P.R * std::fabs(v1) ||
+                        std::fabs(u0) - 0.5f * P.T_v0 > P.L + P.R * std::fabs(v0)) {
+                        o = 0.0f;
+                        continue;
+                    }
+                    if (oversampling <= 1) {
+                        const float pos[3] = {u0 * cs, u0 * sn, u1};
+                        o = ray_value(pos, rv, s, n, stopping);
+                        continue;
+                    }
+                    const float T_v_os = P.T_u1 / float(oversampling + 1);
+                    const float T_u_os = P.T_u0 / float(oversampling + 1);
+                    const int rad = (oversampling - 1) / 2;
+                    float acc = 0.0f;
+                    for (int j = -rad; j <= rad; j++) {
+                        const float du1 = j * T_v_os;
+                        for (int k = -rad; k <= rad; k++) {
+                            const float du0 = k * T_u_os;
+                            const float pos[3] = {(u0 + du0) * cs, (u0 + du0) * sn, u1 + du1};
+                            acc += ray_value(pos, rv, s, n, stopping);
+                        }
+                    }
+                    o = acc / float(oversampling * oversampling);
+                }
+        }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -777,6 +1198,33 @@ double orc_tv_quadform_f64(const double* f, const double* d, int N1, int N2, int
 void orc_gaussian_filter_f64(const double* f, double* out, int N1, int N2, int N3, float FWHM, int numDims)
 {
     gaussian_filter(f, out, N1, N2, N3, FWHM, numDims);
+}
+
+int orc_solid_init(int type, const float* c, const float* r, float val, const float* A, const float* clip,
+                   orc_solid* out)
+{
+    return solid_init(type, c, r, val, A, clip, out) ? 1 : 0;
+}
+
+// chord end points of one ray through one solid; returns 1 when it intersects
+int orc_intersect(const orc_solid* s, const float* p, const float* r, float* ts)
+{
+    f2 t;
+    const bool hit = intersect(p, r, t, *s);
+    ts[0] = t.x;
+    ts[1] = t.y;
+    return hit ? 1 : 0;
+}
+
+float orc_ray_value(const orc_solid* s, int n, const float* p, const float* r, int stopping)
+{
+    return ray_value(p, r, s, n, stopping != 0);
+}
+
+void orc_ray_trace(const orc_plan* P, const orc_solid* s, int n, int oversampling, int stopping, float* g,
+                   int nthreads)
+{
+    ray_trace_plan(*P, s, n, oversampling, stopping != 0, g, nthreads);
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }

@@ -3,6 +3,7 @@
 // sources where they lie.  It lets the tests pin the oracle's geometry and zero-division conventions
 // against the reference itself.  The degree->radian conversion and half-size scaling restate the
 // one line of src/petl_c_interface.cpp:218, which cannot be compiled here (it needs the CUDA build).
+#include <vector>
 #include "petl_defines.h"
 #include "planogram.h"
 #include "data_cube.h"
@@ -72,6 +73,25 @@ float ref_inner3d(float* x, float* y, float* w, int N1, int N2, int N3)
     else r = X.innerProduct(&Y);
     X.data = nullptr; Y.data = nullptr;
     return r;
+}
+
+// The reference's own solid-angle factor (src/planogram.cpp:262-265) applied by
+// planogram::apply_solid_angle_correction (:272-301) to a planogram of ones: out[iv1][iv0] = c or 1/c.
+int ref_solid_angle(float psi, float D, float L, float H, float v_m0, float v_m1, float T, int do_inverse,
+                    float* out)
+{
+    planogram p;
+    if (!p.init(psi * PI / 180.0, 0.5 * D, 0.5 * L, 0.5 * H, v_m0, v_m1, T))
+        return 0;
+    const uint64 n = uint64(p.N_v1) * p.N_v0 * p.N_u1 * p.N_u0;
+    std::vector<float> g(n, 1.0f);
+    p.data = g.data();
+    p.apply_solid_angle_correction(do_inverse != 0);
+    for (int i = 0; i < p.N_v1; i++)
+        for (int j = 0; j < p.N_v0; j++)
+            out[i * p.N_v0 + j] = g[(uint64(i) * p.N_v0 + j) * p.N_u1 * p.N_u0];
+    p.data = nullptr;
+    return 1;
 }
 
 }

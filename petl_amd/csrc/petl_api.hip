@@ -335,6 +335,35 @@ bool run_rwls(Model& m, float** g, float* f, int numIter, float delta, float bet
     return true;
 }
 
+// ray_trace / stopping_power on host planograms (src/petl.cpp:81-152): each planogram traced on the
+// device; stopping_power with no modules sets the planograms to 1
+bool run_ray_trace(Model& m, float** g, int oversampling, bool stopping)
+{
+    for (size_t i = 0; i < m.plans.size(); i++)
+        if (g[i] == nullptr)
+            return false;
+    const std::vector<Solid>& solids = stopping ? m.modules : m.phantom;
+    if (stopping && solids.empty()) {
+        for (size_t i = 0; i < m.plans.size(); i++)
+            std::fill(g[i], g[i] + m.plans[i].elems(), 1.0f);
+        return true;
+    }
+    hipStream_t s = m.stream();
+    DevBuf d_solids, d_g;
+    d_solids.ensure(std::max<size_t>(1, solids.size()) * sizeof(Solid));
+    if (!solids.empty())
+        h2d(d_solids.p, solids.data(), solids.size() * sizeof(Solid), s);
+    for (size_t i = 0; i < m.plans.size(); i++) {
+        const size_t bytes = size_t(m.plans[i].elems()) * sizeof(float);
+        d_g.ensure(bytes);
+        ray_trace_device(m.plans[i].geom(), static_cast<const Solid*>(d_solids.p), int(solids.size()), oversampling,
+                         stopping, d_g.f(), s);
+        d2h(g[i], d_g.p, bytes, s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return true;
+}
+
 }  // namespace
 }  // namespace petl
 
@@ -754,20 +783,85 @@ bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDim
     });
 }
 
+// ---- response model: geometric solids, ray tracing, stopping power, solid angle ----
+// src/petl_c_interface.cpp:271-310, src/petl.cpp:81-162
+bool add_object(int type, float* c, float* r, float val, float* A, float* clip)
+{
+    Solid o;
+    if (!solid_init(type, c, r, val, A, clip, o))
+        return false;
+    M()->phantom.push_back(o);
+    return true;
+}
+bool add_module(int type, float* c, float* r, float val, float* A, float* clip)
+{
+    Solid o;
+    if (!solid_init(type, c, r, val, A, clip, o))
+        return false;
+    M()->modules.push_back(o);
+    return true;
+}
+bool clear_phantom()
+{
+    M()->phantom.clear();
+    return true;
+}
+bool clear_modules()
+{
+    M()->modules.clear();
+    return true;
+}
+bool scale_phantom(float scale_x, float scale_y, float scale_z)
+{
+    solids_scale(M()->phantom, scale_x, scale_y, scale_z);
+    return true;
+}
+bool ray_trace(float** g, int oversampling)
+{
+    if (g == nullptr || !M()->geometryDefined())
+        return false;
+    return guarded("ray_trace", [&] { return run_ray_trace(*M(), g, oversampling, false); });
+}
+bool stopping_power(float** g, int oversampling)
+{
+    if (g == nullptr || !M()->geometryDefined())
+        return false;
+    return guarded("stopping_power", [&] { return run_ray_trace(*M(), g, oversampling, true); });
+}
+// planogram::apply_solid_angle_correction (src/planogram.cpp:257-301): each (v1, v0) view times
+// c = 2 pi (1 + v0^2 + v1^2)^1.5 / (T_v1 T_v0), or times 1 / c
+bool set_solid_angle_correction(float** g, bool do_inverse)
+{
+    if (g == nullptr)
+        return false;
+    Model* m = M();
+    for (size_t i = 0; i < m->plans.size(); i++) {
+        if (g[i] == nullptr)
+            return false;
+        const Planogram& pl = m->plans[i];
+        const int64_t view = int64_t(pl.N_u1) * pl.N_u0;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int a = 0; a < pl.N_v1; a++)
+            for (int b = 0; b < pl.N_v0; b++) {
+                const float v1 = float(a) * pl.T_v1 + pl.v1_0;
+                const float v0 = float(b) * pl.T_v0 + pl.v0_0;
+                float c = float(2.0 * PETL_PI * std::pow(1.0 + v0 * v0 + v1 * v1, 1.5) / (pl.T_v1 * pl.T_v0));
+                if (do_inverse)
+                    c = float(1.0 / c);
+                float* x = g[i] + (int64_t(a) * pl.N_v0 + b) * view;
+                for (int64_t k = 0; k < view; k++)
+                    x[k] *= c;
+            }
+    }
+    return true;
+}
+
 // ---- outside the projector / MLEM path ----
 
 bool simulate_scatter(float**, float*) { return unsupported("simulate_scatter"); }
 bool FBP(float**, float*) { return unsupported("FBP"); }
 bool PFDR(float**, float**) { return unsupported("PFDR"); }
-bool add_object(int, float*, float*, float, float*, float*) { return unsupported("add_object"); }
-bool add_module(int, float*, float*, float, float*, float*) { return unsupported("add_module"); }
-bool clear_phantom() { return unsupported("clear_phantom"); }
-bool clear_modules() { return unsupported("clear_modules"); }
-bool scale_phantom(float, float, float) { return unsupported("scale_phantom"); }
 bool voxelize(float*, int) { return unsupported("voxelize"); }
-bool ray_trace(float**, int) { return unsupported("ray_trace"); }
-bool stopping_power(float**, int) { return unsupported("stopping_power"); }
-bool set_solid_angle_correction(float**, bool) { return unsupported("set_solid_angle_correction"); }
 bool apply_corrections(float**, float**, float) { return unsupported("apply_corrections"); }
 bool diffuse(float*, int, int, int, float, float, int, bool) { return unsupported("diffuse"); }
 bool TV_denoise(float*, int, int, int, float, float, float, int, bool, bool) { return unsupported("TV_denoise"); }

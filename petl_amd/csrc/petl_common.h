@@ -56,4 +56,6 @@ struct hip_error : std::runtime_error {
 
 inline int ceil_div(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 
+constexpr double PETL_PI = 3.1415926535897932385;  // src/petl_defines.h:22
+
 }  // namespace petl

@@ -5,6 +5,7 @@
 #include <memory>
 #include <vector>
 #include "petl_common.h"
+#include "ray_trace.h"
 #include "proj_fast.h"
 
 namespace petl {
@@ -58,6 +59,8 @@ struct DeviceState {
 
 struct Model {
     std::vector<Planogram> plans;
+    // geometric phantom (add_object) and scintillator modules (add_module) of the response model
+    std::vector<Solid> phantom, modules;
     int numX = 0, numY = 0, numZ = 0;
     float voxelWidth = 0, voxelHeight = 0, offsetX = 0, offsetY = 0, offsetZ = 0;
     int back_mode = BACK_PETL;

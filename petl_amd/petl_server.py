@@ -65,6 +65,14 @@ _SIGNATURES = {
     "TV_quadForm": (ctypes.c_float, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_float] * 3 +
                     [ctypes.c_bool]),
     "gaussian_filter": (ctypes.c_bool, [_f32] + [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int]),
+    "add_object": (ctypes.c_bool, [ctypes.c_int, _f32, _f32, ctypes.c_float, _f32, _f32]),
+    "add_module": (ctypes.c_bool, [ctypes.c_int, _f32, _f32, ctypes.c_float, _f32, _f32]),
+    "clear_phantom": (ctypes.c_bool, []),
+    "clear_modules": (ctypes.c_bool, []),
+    "scale_phantom": (ctypes.c_bool, [ctypes.c_float] * 3),
+    "ray_trace": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_int]),
+    "stopping_power": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_int]),
+    "set_solid_angle_correction": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_bool]),
     "RWLS_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_int, ctypes.POINTER(_c_float_p), ctypes.c_float, ctypes.c_int, ctypes.c_bool,
                                  ctypes.POINTER(_c_float_p), ctypes.c_void_p, ctypes.c_void_p]),
@@ -439,6 +447,110 @@ class PETLserver:
                                                  int(bool(chest_wall)), s, sync):
             raise TypeError('bin_events_gpu failed')
         return g
+
+    # ------------------------------------------------------------------ response model
+    _OBJECT_TYPES = {"ball": 0, "sphere": 0, "ellipsoid": 0, "box": 1, "parallelepiped": 1, "can_x": 2,
+                     "can_y": 3, "can_z": 4, "can": 4, "cone_x": 5, "cone_y": 6, "cone_z": 7, "cone": 7}
+
+    @staticmethod
+    def _solid_args(c, r, A, clip):
+        if A is None:
+            A = np.eye(3, dtype=np.float32)
+        if clip is None:
+            clip = np.zeros(3, dtype=np.float32)
+        if isinstance(c, (int, float)):
+            c = [c, c, c]
+        if isinstance(r, (int, float)):
+            r = [r, r, r]
+        return (np.ascontiguousarray(c, dtype=np.float32), np.ascontiguousarray(r, dtype=np.float32),
+                np.ascontiguousarray(A, dtype=np.float32), np.ascontiguousarray(clip, dtype=np.float32))
+
+    def add_object(self, typeOfObject, c, r, val, A=None, clip=None):
+        """Adds a geometric object to the ray-tracing phantom (src/petl_server.py:813-875): type 0-7 or a name
+        ('sphere', 'box', 'can_x', ..., 'cone'); later objects override earlier ones where they overlap."""
+        if isinstance(typeOfObject, str):
+            t = self._OBJECT_TYPES.get(typeOfObject.lower())
+            if t is None:
+                print('Error: unknown object type')
+                return False
+            typeOfObject = t
+        c, r, A, clip = self._solid_args(c, r, A, clip)
+        self.set_model()
+        return self.libprojectors.add_object(int(typeOfObject), c, r, float(val), A, clip)
+
+    def add_module(self, c, r, val=0.0815, A=None):
+        """Adds a scintillator module (a box; val = attenuation coefficient at 511 keV, 1/mm) to the model the
+        stopping power is computed from (src/petl_server.py:877-910)."""
+        c, r, A, clip = self._solid_args(c, r, A, None)
+        self.set_model()
+        return self.libprojectors.add_module(1, c, r, float(val), A, clip)
+
+    def clear_phantom(self):
+        self.set_model()
+        self.libprojectors.clear_phantom()
+
+    def clear_modules(self):
+        self.set_model()
+        self.libprojectors.clear_modules()
+
+    def scale_phantom(self, c):
+        """Scales the ray-tracing phantom (src/petl_server.py:922-944)."""
+        if isinstance(c, (int, float)):
+            cx = cy = cz = c
+        elif isinstance(c, np.ndarray) and c.size == 3:
+            cx, cy, cz = (float(v) for v in c.ravel())
+        else:
+            return False
+        self.set_model()
+        return self.libprojectors.scale_phantom(cx, cy, cz)
+
+    def ray_trace(self, g=None, oversampling=1):
+        """Line integrals of the phantom along every bin's ray (src/petl_server.py:967-993)."""
+        if not self.geometry_defined():
+            raise ValueError("Error: PET scanner not specified")
+        if g is None:
+            g = self.allocate_projections()
+        self.set_model()
+        return g if self.libprojectors.ray_trace(self.floatstarstar(g), int(oversampling)) else None
+
+    def stopping_power(self, g=None, oversampling=1):
+        """Scintillator stopping power (1 - e^-a)(1 - e^-b) of every bin's two half rays through the modules
+        (1 everywhere without modules), then smoothed with a FWHM 0.6571 Gaussian (src/petl_server.py:995-1022)."""
+        if not self.geometry_defined():
+            raise ValueError("Error: PET scanner not specified")
+        if g is None:
+            g = self.allocate_projections()
+        self.set_model()
+        if self.libprojectors.stopping_power(self.floatstarstar(g), int(oversampling)):
+            self.gaussian_filter(g, 0.6571)
+            return g
+        return None
+
+    def apply_solid_angle_correction(self, g=None, do_inverse=False):
+        """Each view times 2 pi (1 + v0^2 + v1^2)^1.5 / (T_v1 T_v0), or its reciprocal (src/petl_server.py:1024-1043)."""
+        if g is None:
+            g = self.allocate_projections()
+            self.setToOne(g)
+        self.set_model()
+        return g if self.libprojectors.set_solid_angle_correction(self.floatstarstar(g), bool(do_inverse)) else None
+
+    def apply_solid_angle_model(self, g=None):
+        return self.apply_solid_angle_correction(g, True)
+
+    def calc_response(self, oversampling=1, mu=None):
+        """response = stopping power (oversampling > 0) x solid-angle model / exp(P mu) (src/petl_server.py:578-605)."""
+        if oversampling > 0:
+            self.response = self.stopping_power(self.response, oversampling=oversampling)
+        else:
+            self.response = None
+        self.response = self.apply_solid_angle_model(self.response)
+        if mu is not None:
+            Pmu = self.allocate_projections()
+            self.project(Pmu, mu)
+            for a in Pmu:
+                np.exp(a, out=a)
+            self.divide(self.response, Pmu)
+        return self.response
 
     def sensitivity(self, Pstar1=None):
         """P*1, the back projection of all-ones planograms (src/petl_server.py:1257-1273)."""
