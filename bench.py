@@ -353,6 +353,37 @@ def main():
                                "ms_per_batch": 1e3 * ag_s, "volumes_per_s": 8 / ag_s}
     del fbatch
 
+    # response model at the benchmark geometry: scintillator stopping power through 4 x 64 box modules
+    # (30.32 mm blocks, 1.28 mm gaps, 20 mm deep, behind each of the four panels), device planograms
+    modules = []
+    blk, gap, depth = 30.32, 1.28, 20.0
+    npan = int(geo["L"] // (blk + gap))
+    for m_ in range(npan):
+        cz = (m_ + 0.5) * (blk + gap)
+        for n_ in range(npan):
+            cx = (n_ - (npan - 1) / 2) * (blk + gap)
+            for c_, r_ in (([cx, geo["D"] / 2 + depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
+                           ([cx, -geo["D"] / 2 - depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
+                           ([geo["D"] / 2 + depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2]),
+                           ([-geo["D"] / 2 - depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2])):
+                modules.append((c_, r_))
+    for c_, r_ in modules:
+        s.add_module(c_, r_)
+    resp = {}
+    for os_ in (1, 3):
+        s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=True)  # warm-up
+        er = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        er[0].record()
+        s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=False)
+        er[1].record()
+        torch.cuda.synchronize()
+        ms = er[0].elapsed_time(er[1])
+        resp[f"oversampling_{os_}"] = {"ms_both_planograms": ms,
+                                       "rays_per_s": os_ * os_ * sum(int(np.prod(sh)) for sh in shapes) / (ms * 1e-3)}
+    s.clear_modules()
+    out["response"] = dict(resp, workload=f"stopping power of every bin of both planograms through {len(modules)} "
+                                          "box modules (two half rays each, oversampling^2 sub-rays per bin)")
+
     # reconstruction loops at the benchmark geometry, host data in and image out as PETLserver.MLEM /
     # PETLserver.RWLS take them: per-iteration cost from the difference of a short and a long run (the
     # upload, weights, sensitivity and preconditioner set-up cancel)

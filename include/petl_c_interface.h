@@ -3,14 +3,15 @@
  *
  * Part 1 keeps every export of the reference's src/petl_c_interface.h:14-97 with the same name, C
  * signature and bool conventions, so the reference's ctypes bindings (src/petl_server.py) load this
- * library unchanged.  Each declaration cites the reference line it replaces.  Exports that belong to
- * subsystems outside the projector / MLEM path (list-mode binning, phantoms, ray tracing, FBP/PFDR, TV
- * denoising, Gaussian filtering) are present so symbol lookup succeeds, but they print an error and
- * return false / 0.
+ * library unchanged.  Each declaration cites the reference line it replaces.  Exports of subsystems
+ * this library does not provide (voxelisation, FBP/PFDR, scatter simulation, bad-pixel correction, TV
+ * denoising and diffusion) are present so symbol lookup succeeds, but they print an error and return
+ * false / 0.
  *
  * Part 2 adds entry points the reference does not have: device-resident projection on caller-owned
- * device buffers (torch tensors), back projector mode selection, and a device-resident MLEM/OSEM
- * reconstruction.  Host-pointer calls are synchronous, like the reference's.
+ * device buffers (torch tensors), back projector mode selection, list-mode binning of event arrays, and
+ * device-resident MLEM/OSEM and RWLS reconstructions.  Host-pointer calls are synchronous, like the
+ * reference's.
  *
  * Threading: like the reference, calls act on the model selected by set_model / create_new_model,
  * which is process-global state; do not call concurrently from several threads.
@@ -99,20 +100,24 @@ PETL_EXTERN float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, flo
 /* Gaussian smoothing of a host array in place: numDims 3 all axes, 2 the last two, 1 the first. */
 PETL_EXTERN bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims); /* :97 */
 
-/* exports outside the projector / MLEM path: present, return false / 0 with a message */
-
-PETL_EXTERN bool simulate_scatter(float** g, float* mu);                                   /* :57 */
-PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
-PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
+/* response model: geometric solids (type 0 ellipsoid, 1 parallelepiped, 2-4 cylinder x/y/z, 5-7 cone x/y/z;
+ * c, r: 3 floats; A: 3x3 row-major rotation or NULL; clip: 3 floats or NULL), a ray-tracing phantom and the
+ * scintillator modules; ray_trace / stopping_power fill host planograms (oversampling: odd, 1..11). */
 PETL_EXTERN bool add_object(int type, float* c, float* r, float val, float* A, float* clip); /* :64 */
 PETL_EXTERN bool add_module(int type, float* c, float* r, float val, float* A, float* clip); /* :65 */
 PETL_EXTERN bool clear_phantom();                                                          /* :66 */
 PETL_EXTERN bool clear_modules();                                                          /* :67 */
 PETL_EXTERN bool scale_phantom(float, float, float);                                       /* :68 */
-PETL_EXTERN bool voxelize(float* f, int oversampling);                                     /* :69 */
 PETL_EXTERN bool ray_trace(float** g, int oversampling);                                   /* :70 */
 PETL_EXTERN bool stopping_power(float** g, int oversampling);                              /* :71 */
 PETL_EXTERN bool set_solid_angle_correction(float** g, bool do_inverse);                   /* :72 */
+
+/* exports outside the projector / MLEM path: present, return false / 0 with a message */
+
+PETL_EXTERN bool simulate_scatter(float** g, float* mu);                                   /* :57 */
+PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
+PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
+PETL_EXTERN bool voxelize(float* f, int oversampling);                                     /* :69 */
 PETL_EXTERN bool apply_corrections(float** g, float** r, float threshold);                 /* :73 */
 PETL_EXTERN bool diffuse(float* f, int N_1, int N_2, int N_3, float delta, float p, int numIter,
                          bool data_on_cpu);                                                /* :94 */
@@ -188,6 +193,10 @@ PETL_EXTERN bool MLEM_gpu(float** g, float* f, int numIter, float delta, float b
  * sum(data) / sum(sensitivity)).  No mask, response or log-likelihood. */
 PETL_EXTERN bool MLEM_gpu_storage(float** g, float* f, int numIter, float delta, float beta, int prior, float p,
                                   int numSubsets, int planogram_storage, int image_storage);
+
+/* ray_trace (stopping = 0) or stopping_power (stopping = 1) into device planograms d_g on a stream
+ * (NULL = the default stream); without modules, stopping_power writes 1. */
+PETL_EXTERN bool ray_trace_gpu(float** d_g, int oversampling, int stopping, void* stream, bool sync);
 
 /* Device-resident RWLS (PETLserver.RWLS, src/petl_server.py:1691-1877): preconditioned conjugate gradients
  * on 0.5 (Pf - g)^T W (Pf - g) + TV(f) (p-Huber TV with p = 1, delta, beta; none when beta <= 0), every

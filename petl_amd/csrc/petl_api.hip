@@ -337,27 +337,36 @@ bool run_rwls(Model& m, float** g, float* f, int numIter, float delta, float bet
 
 // ray_trace / stopping_power on host planograms (src/petl.cpp:81-152): each planogram traced on the
 // device; stopping_power with no modules sets the planograms to 1
+// uploads the phantom's or the modules' solids to the model's device copy; returns their count
+int upload_solids(Model& m, bool stopping, hipStream_t s)
+{
+    const std::vector<Solid>& solids = stopping ? m.modules : m.phantom;
+    m.dev.solids.ensure(std::max<size_t>(1, solids.size()) * sizeof(Solid));
+    if (!solids.empty()) {
+        h2d(m.dev.solids.p, solids.data(), solids.size() * sizeof(Solid), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));  // the host vector may change after this call
+    }
+    return int(solids.size());
+}
+
 bool run_ray_trace(Model& m, float** g, int oversampling, bool stopping)
 {
     for (size_t i = 0; i < m.plans.size(); i++)
         if (g[i] == nullptr)
             return false;
-    const std::vector<Solid>& solids = stopping ? m.modules : m.phantom;
-    if (stopping && solids.empty()) {
+    if (stopping && m.modules.empty()) {
         for (size_t i = 0; i < m.plans.size(); i++)
             std::fill(g[i], g[i] + m.plans[i].elems(), 1.0f);
         return true;
     }
     hipStream_t s = m.stream();
-    DevBuf d_solids, d_g;
-    d_solids.ensure(std::max<size_t>(1, solids.size()) * sizeof(Solid));
-    if (!solids.empty())
-        h2d(d_solids.p, solids.data(), solids.size() * sizeof(Solid), s);
+    const int n = upload_solids(m, stopping, s);
+    DevBuf d_g;
     for (size_t i = 0; i < m.plans.size(); i++) {
         const size_t bytes = size_t(m.plans[i].elems()) * sizeof(float);
         d_g.ensure(bytes);
-        ray_trace_device(m.plans[i].geom(), static_cast<const Solid*>(d_solids.p), int(solids.size()), oversampling,
-                         stopping, d_g.f(), s);
+        ray_trace_device(m.plans[i].geom(), static_cast<const Solid*>(m.dev.solids.p), n, oversampling, stopping,
+                         d_g.f(), s);
         d2h(g[i], d_g.p, bytes, s);
         PETL_HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -1040,6 +1049,31 @@ bool MLEM_gpu(float** g, float* f, int numIter, float delta, float beta, int pri
     Model* m = M();
     return guarded("MLEM_gpu",
                    [&] { return run_mlem(*m, g, f, numIter, delta, beta, prior, p, numSubsets, mask, response, loglik); });
+}
+
+bool ray_trace_gpu(float** d_g, int oversampling, int stopping, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !m->geometryDefined())
+        return false;
+    for (size_t i = 0; i < m->plans.size(); i++)
+        if (!d_g[i])
+            return false;
+    return guarded("ray_trace_gpu", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (stopping && m->modules.empty()) {
+            for (size_t i = 0; i < m->plans.size(); i++)
+                launch_fill(d_g[i], 1.0f, m->plans[i].elems(), s);
+        } else {
+            const int n = upload_solids(*m, stopping != 0, s);
+            for (size_t i = 0; i < m->plans.size(); i++)
+                ray_trace_device(m->plans[i].geom(), static_cast<const Solid*>(m->dev.solids.p), n, oversampling,
+                                 stopping != 0, d_g[i], s);
+        }
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
 }
 
 bool RWLS_gpu(float** g, float* f, int numIter, float delta, float beta, int weight_mode, float** W, float w_scalar,

@@ -54,6 +54,7 @@ struct DeviceState {
     BinScratch bin;                               // list-mode binning
     DevBuf events, counter;                       // staged coincidences, accepted-event counters
     DevBuf stage;                                 // fp32 upload staging for storage-format data
+    DevBuf solids;                                // the ray tracer's solids
     ~DeviceState();
 };
 

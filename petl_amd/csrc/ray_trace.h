@@ -19,6 +19,8 @@ struct Solid {
     int rotated;
     int nclip;
     float cone[2];       // axial clip range of a cone about its apex
+    // bounding sphere of the unclipped solid, for culling (radius < 0: always test)
+    float bc[3], br;
 };
 
 // geometricObject::init: false for a null centre / radius or an unknown type.  A and clip may be null.

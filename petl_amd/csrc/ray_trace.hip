@@ -5,9 +5,10 @@
 // mix of the reference's device code is kept (each quantity is formed in the type it is formed in there),
 // with contraction off.
 //
-// Layout: one thread per bin, the solids read through uniform (scalar) loads.  The reference gives every
-// thread 4n floats and n ints of global scratch for its sorted end points; here a thread keeps the solids
-// it hits (up to MAXHIT) in registers and sorts their end points there.  A ray that hits more solids walks
+// Layout: one thread per bin, the solids read through uniform (scalar) loads; a solid whose bounding
+// sphere the ray misses by a clear margin is skipped before the exact test (it could not report a chord).
+// The reference gives every thread 4n floats and n ints of global scratch for its sorted end points; here
+// a thread keeps the solids it hits (up to MAXHIT) in registers and sorts their end points there.  A ray that hits more solids walks
 // the distinct end points in increasing order without storing them, recomputing the intersections it
 // needs: the same pieces, in the same order, as sorting all end points.
 #include <cmath>
@@ -234,10 +235,30 @@ __device__ bool centered(const D3& p, const D3& r, F2& ts, const Solid& s)
     return true;
 }
 
-// intersectionEndPoints (src/analytic_ray_tracing_gpu.cu:465-518)
+// true when the line p + t r certainly misses the solid's bounding sphere: the distance from the centre
+// to the line exceeds the radius by a margin far above the rounding of this test and of the exact one
+__device__ __forceinline__ bool culled(const float* p, const float* r, const Solid& s)
+{
+    if (s.br < 0.0f)
+        return false;
+    const float wx = s.bc[0] - p[0], wy = s.bc[1] - p[1], wz = s.bc[2] - p[2];
+    const float ww = wx * wx + wy * wy + wz * wz;
+    const float wr = wx * r[0] + wy * r[1] + wz * r[2];
+    const float rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const float lim = s.br * 1.01f + 0.1f;
+    return ww - wr * (wr / rr) > lim * lim;
+}
+
+// intersectionEndPoints (src/analytic_ray_tracing_gpu.cu:465-518); a culled solid is a miss, as the exact
+// test finds it
 __device__ bool intersect(const float* p, const float* r, F2& ts, const Solid& s)
 {
 #pragma clang fp contract(off)
+    if (culled(p, r, s)) {
+        ts.x = -OOB;
+        ts.y = ts.x;
+        return false;
+    }
     D3 q, mr;
     if (!s.rotated) {
         q.x = (p[0] - s.c[0]) / s.r[0];
@@ -451,6 +472,11 @@ bool solid_init(int type, const float* c, const float* r, float val, const float
                 o.nclip++;
             }
     o.val = val;
+    // ellipsoids, boxes and cylinders lie within |r| of their centre whatever their rotation or clipping;
+    // cones are always tested (their scaling leaves the axial clip range unscaled)
+    for (int i = 0; i < 3; i++)
+        o.bc[i] = c[i];
+    o.br = (type <= 4) ? float(std::sqrt(double(r[0]) * r[0] + double(r[1]) * r[1] + double(r[2]) * r[2])) : -1.0f;
     if (A != nullptr)
         for (int i = 0; i < 9; i++) {
             o.A[i] = A[i];
@@ -493,6 +519,11 @@ void solids_scale(std::vector<Solid>& s, float sx, float sy, float sz)
             o.clip[k][2] *= sz;
             o.clip[k][3] *= sx;
         }
+        o.bc[0] *= sx;
+        o.bc[1] *= sy;
+        o.bc[2] *= sz;
+        if (o.br >= 0.0f)
+            o.br *= std::max(std::fabs(sx), std::max(std::fabs(sy), std::fabs(sz)));
     }
 }
 

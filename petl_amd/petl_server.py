@@ -73,6 +73,8 @@ _SIGNATURES = {
     "ray_trace": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_int]),
     "stopping_power": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_int]),
     "set_solid_angle_correction": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_bool]),
+    "ray_trace_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_bool]),
     "RWLS_gpu": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_int, ctypes.POINTER(_c_float_p), ctypes.c_float, ctypes.c_int, ctypes.c_bool,
                                  ctypes.POINTER(_c_float_p), ctypes.c_void_p, ctypes.c_void_p]),
@@ -525,6 +527,14 @@ class PETLserver:
             self.gaussian_filter(g, 0.6571)
             return g
         return None
+
+    def ray_trace_gpu(self, g, oversampling=1, stopping=False, stream=None, sync=True):
+        """ray_trace (or, with stopping=True, the unsmoothed stopping power) into float32 device tensors."""
+        self.set_model()
+        if not self.libprojectors.ray_trace_gpu(self._dev_ptrs(g), int(oversampling), int(bool(stopping)), stream,
+                                                bool(sync)):
+            raise TypeError('ray_trace_gpu failed')
+        return g
 
     def apply_solid_angle_correction(self, g=None, do_inverse=False):
         """Each view times 2 pi (1 + v0^2 + v1^2)^1.5 / (T_v1 T_v0), or its reciprocal (src/petl_server.py:1024-1043)."""

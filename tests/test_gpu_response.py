@@ -142,3 +142,16 @@ def test_calc_response_matches_oracle_composition(oracle):
         st = st * oracle.solid_angle_factor(p, True)[:, :, None, None]
         want = st / np.exp(pm.astype(np.float64))
         compare(gi, want, rtol=2e-5, atol=1e-9)
+
+
+def test_device_entry_point_matches_host():
+    torch = pytest.importorskip("torch")
+    s = server()
+    for c, r in modules(GEO["D"], GEO["L"], GEO["H"]):
+        s.add_module(c, r)
+    host = s.allocate_projections()
+    assert s.libprojectors.stopping_power(s.floatstarstar(host), 3)
+    dev = [torch.empty(a.shape, dtype=torch.float32, device="cuda") for a in host]
+    s.ray_trace_gpu(dev, oversampling=3, stopping=True)
+    for a, b in zip(host, dev):
+        np.testing.assert_array_equal(b.cpu().numpy(), a)
