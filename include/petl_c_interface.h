@@ -4,8 +4,8 @@
  * Part 1 keeps every export of the reference's src/petl_c_interface.h:14-97 with the same name, C
  * signature and bool conventions, so the reference's ctypes bindings (src/petl_server.py) load this
  * library unchanged.  Each declaration cites the reference line it replaces.  Exports of subsystems
- * this library does not provide (voxelisation, FBP/PFDR, scatter simulation, bad-pixel correction, TV
- * denoising and diffusion) are present so symbol lookup succeeds, but they print an error and return
+ * this library does not provide (voxelisation, scatter simulation, bad-pixel correction, TV denoising
+ * and diffusion) are present so symbol lookup succeeds, but they print an error and return
  * false / 0.
  *
  * Part 2 adds entry points the reference does not have: device-resident projection on caller-owned
@@ -100,6 +100,12 @@ PETL_EXTERN float TV_quadForm(float* f, float* d, int N_1, int N_2, int N_3, flo
 /* Gaussian smoothing of a host array in place: numDims 3 all axes, 2 the last two, 1 the first. */
 PETL_EXTERN bool gaussian_filter(float* f, int N_1, int N_2, int N_3, float FWHM, int numDims); /* :97 */
 
+/* analytic reconstruction: PFDR rebins host planograms to one v1 row each (g_reb: [N_v0][N_u1][N_u0]) and
+ * leaves the model's planograms at N_v1 = 1, v1_0 = 0; FBP = PFDR (on a copy of the geometry, when N_v1 > 1)
+ * + ramp filter + planogram weight + back projection into f (already rebinned data are filtered in place). */
+PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
+PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
+
 /* response model: geometric solids (type 0 ellipsoid, 1 parallelepiped, 2-4 cylinder x/y/z, 5-7 cone x/y/z;
  * c, r: 3 floats; A: 3x3 row-major rotation or NULL; clip: 3 floats or NULL), a ray-tracing phantom and the
  * scintillator modules; ray_trace / stopping_power fill host planograms (oversampling: odd, 1..11). */
@@ -115,8 +121,6 @@ PETL_EXTERN bool set_solid_angle_correction(float** g, bool do_inverse);        
 /* exports outside the projector / MLEM path: present, return false / 0 with a message */
 
 PETL_EXTERN bool simulate_scatter(float** g, float* mu);                                   /* :57 */
-PETL_EXTERN bool FBP(float** g, float* f);                                                 /* :61 */
-PETL_EXTERN bool PFDR(float** g, float** g_reb);                                           /* :62 */
 PETL_EXTERN bool voxelize(float* f, int oversampling);                                     /* :69 */
 PETL_EXTERN bool apply_corrections(float** g, float** r, float threshold);                 /* :73 */
 PETL_EXTERN bool diffuse(float* f, int N_1, int N_2, int N_3, float delta, float p, int numIter,

@@ -450,6 +450,124 @@ def solid_angle_factor(plan, do_inverse=False):
     return c
 
 
+# --------------------------------------------------------------------------------------------------
+# analytic reconstruction: PFDR rebinning, ramp filter, FBP (src/pfdr.cu, src/ramp_filter.cu,
+# src/petl.cpp:310-385), in FP64 with numpy's FFT
+# --------------------------------------------------------------------------------------------------
+def optimal_fft_size(N):
+    """optimalFFTsize (src/vector_ops.cpp:481-505): the smallest 2^(n+1) 3^m >= N, from the reference's
+    own search (which can return a larger value than the true minimum; its choice is what counts)."""
+    import math
+    if N <= 2:
+        return 2
+    c1 = math.log2(N / 2.0) / math.log2(3)
+    c2 = 1.0 / math.log2(3)
+    xbar = math.log2(N / 2.0)
+    best = int(2 ** (int(math.ceil(xbar)) + 1))
+    for x in range(int(math.ceil(xbar))):
+        y = int(math.ceil(c1 - c2 * x))
+        v = int(2 ** (x + 1) * 3 ** y)
+        if v < best and y >= 0:
+            best = v
+    return best
+
+
+def _fgrid(n, T0, start):
+    return (np.arange(n, dtype=np.float32) * np.float32(T0) + np.float32(start)).astype(np.float64)
+
+
+def pfdr(plan, g):
+    """PFDR of one planogram (src/pfdr.cu:159-315): per v1 row the views are weighted by
+    1/sqrt(1 + v1^2 + v0^2), Fourier transformed over (v0, u0) (zero-padded to optimal sizes), and added into
+    the rebinned spectrum when |v1| <= T_v1/2 or |U| > |v1| / (4 T_u1 v_m0); the sum is divided by the number
+    of rows added (except at U = 0) and by the transform size, transformed back and weighted by sqrt(1 + v0^2).
+    As in the reference, a contributing oblique row is read at its own u1 (the kernel forms the shifted u1
+    but indexes with the unshifted one), interpolated with the fractional part that float arithmetic leaves.
+    Returns [N_v0][N_u1][N_u0]."""
+    g = np.asarray(g, np.float64)
+    Nv1, Nv0, Nu1, Nu0 = plan.shape
+    NX, NY = optimal_fft_size(Nu0), optimal_fft_size(Nv0)
+    NXH = NX // 2 + 1
+    T_U = np.float32(1.0 / (NX * plan.T_u0))
+    T_V = np.float32(1.0 / (NY * plan.T_v0))
+    ac = np.float32(1.0 / (2.0 * plan.T_u1 * 2.0 * plan.v_m0))
+    U = np.where(np.arange(NXH) < NX // 2, np.arange(NXH), np.arange(NXH) - NX).astype(np.float32) * T_U
+    v0 = _fgrid(Nv0, plan.T_v0, plan.v0_0)
+    u1 = (np.arange(Nu1, dtype=np.float32) * np.float32(plan.T_u1) + np.float32(plan.u1_0)).astype(np.float32)
+    # the row the kernel interpolates at: (u1 - u1_0) / T_u1 in float
+    iu1o = ((u1 - np.float32(plan.u1_0)) / np.float32(plan.T_u1)).astype(np.float32)
+    valid = (iu1o >= 0) & (iu1o <= np.float32(Nu1 - 1))
+    lo = np.clip(iu1o.astype(np.int64), 0, Nu1 - 1)
+    hi = np.minimum(Nu1 - 1, lo + 1)
+    d = (iu1o - lo.astype(np.float32)).astype(np.float64)
+    Greb = np.zeros((Nu1, NY, NXH), np.complex128)
+    wts = np.zeros((Nu1, NY, NXH), np.float64)
+    T_v1 = np.float32(plan.T_v1)
+    for iv1 in range(Nv1):
+        v1 = float(np.float32(iv1) * np.float32(plan.T_v1) + np.float32(plan.v1_0))
+        buf = np.zeros((Nu1, NY, NX))
+        w = 1.0 / np.sqrt(1.0 + v1 * v1 + v0 * v0)
+        buf[:, :Nv0, :Nu0] = np.transpose(g[iv1] * w[:, None, None], (1, 0, 2))
+        G = np.fft.rfft2(buf)
+        if abs(v1) <= 0.5 * T_v1:
+            Greb += G
+            wts += 1.0
+        else:
+            m = np.abs(U) > abs(v1) * float(ac)
+            Gi = (1.0 - d)[:, None, None] * G[lo] + d[:, None, None] * G[hi]
+            sel = valid[:, None, None] & m[None, None, :]
+            Greb += np.where(sel, Gi, 0)
+            wts += sel
+    scale = np.full(wts.shape, 1.0 / (NY * NX))
+    nz = (wts > 0) & (np.arange(NXH) != 0)[None, None, :]
+    scale[nz] /= wts[nz]
+    real = np.fft.irfft2(Greb * scale, s=(NY, NX)) * (NY * NX)
+    out = np.transpose(real[:, :Nv0, :Nu0], (1, 0, 2)) * np.sqrt(1.0 + v0 * v0)[:, None, None]
+    return out
+
+
+def ramp_kernel(N, T):
+    """rampImpulseResponse (src/ramp_filter.cu:334-349): h[i] = 1 / (T pi (1/4 - s^2)), s the signed index."""
+    i = np.arange(N)
+    s = np.where(i < N // 2, i, i - N).astype(np.float64)
+    return 1.0 / (float(np.float32(T)) * np.pi * (0.25 - s * s))
+
+
+def ramp_filter(g_reb, T_u0):
+    """rampFilter1D (src/ramp_filter.cu:156-332) of rebinned data [N_v0][N_u1][N_u0] along u0: rows
+    zero-padded to optimalFFTsize(2 N_u0), multiplied in frequency by Re(FFT(h)) / N_H."""
+    g_reb = np.asarray(g_reb, np.float64)
+    Nu0 = g_reb.shape[-1]
+    NH = optimal_fft_size(2 * Nu0)
+    H = np.real(np.fft.rfft(ramp_kernel(NH, T_u0))) / NH
+    pad = np.zeros(g_reb.shape[:-1] + (NH,))
+    pad[..., :Nu0] = g_reb
+    return np.fft.irfft(np.fft.rfft(pad, axis=-1) * H, n=NH, axis=-1)[..., :Nu0] * NH
+
+
+def rebinned_plan(plan):
+    """The planogram PFDR leaves behind (src/petl.cpp:378-379): one v1 row at v1 = 0."""
+    import copy
+    p = copy.copy(plan)
+    p.N_v1 = 1
+    p.v1_0 = 0.0
+    return p
+
+
+def fbp(plans, vol, g, nthreads=None):
+    """PETL::doFBP (src/petl.cpp:310-355): PFDR when N_v1 > 1, the ramp filter, the inverse planogram weight
+    sqrt(1 + v0^2) (v1 = 0 after rebinning) and the PETL back projection of the rebinned planograms."""
+    reb, rplans = [], []
+    for p, gi in zip(plans, g):
+        gr = pfdr(p, gi) if p.N_v1 > 1 else np.asarray(gi, np.float64).reshape(p.shape)[0]
+        rp = rebinned_plan(p)
+        v0 = _fgrid(p.N_v0, p.T_v0, p.v0_0)
+        gr = ramp_filter(gr, p.T_u0) * np.sqrt(1.0 + v0 * v0)[:, None, None]
+        reb.append(gr[None].astype(np.float64))
+        rplans.append(rp)
+    return backproject(rplans, vol, reb, mode="petl", dtype=np.float64, nthreads=nthreads)
+
+
 def osem(plans, vol, g, num_iter, num_subsets, f0=None, nthreads=None):
     """OSEM with interleaved v1-row subsets (rows s, s+n, ...), fp32, PETL-mode back projection; one
     iteration = one pass over the subsets.  f0: start image (default ones).  With one subset and the
@@ -585,5 +703,7 @@ def ref_lib():
         R.ref_event_coords.restype = c_int
         R.ref_solid_angle.argtypes = [c_float] * 7 + [c_int, fp]
         R.ref_solid_angle.restype = c_int
+        R.ref_optimal_fft_size.argtypes = [c_int]
+        R.ref_optimal_fft_size.restype = c_int
         _ref = R
     return _ref

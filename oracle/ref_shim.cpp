@@ -7,6 +7,7 @@
 #include "petl_defines.h"
 #include "planogram.h"
 #include "data_cube.h"
+#include "vector_ops.h"
 
 extern "C" {
 
@@ -93,5 +94,8 @@ int ref_solid_angle(float psi, float D, float L, float H, float v_m0, float v_m1
     p.data = nullptr;
     return 1;
 }
+
+// The reference's own FFT size rule (src/vector_ops.cpp:481-505).
+int ref_optimal_fft_size(int N) { return optimalFFTsize(N); }
 
 }

@@ -23,7 +23,8 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
 ]
-LDFLAGS = ["-shared", "-fopenmp", f"--offload-arch={ARCH}"]
+LDFLAGS = ["-shared", "-fopenmp", f"--offload-arch={ARCH}", "-L/opt/rocm/lib", "-lhipfft",
+           "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def sources():

@@ -15,6 +15,7 @@
 #include "mlem_job.h"
 #include "rwls_job.h"
 #include "lsq_kernels.h"
+#include "pfdr.h"
 #include "binning.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
@@ -370,6 +371,78 @@ bool run_ray_trace(Model& m, float** g, int oversampling, bool stopping)
         d2h(g[i], d_g.p, bytes, s);
         PETL_HIP_CHECK(hipStreamSynchronize(s));
     }
+    return true;
+}
+
+// PETL::doPFDR (src/petl.cpp:357-385): every planogram rebinned to one v1 row (g_reb: [N_v0][N_u1][N_u0]);
+// afterwards the model's planograms have N_v1 = 1 and v1_0 = 0, as the reference leaves them
+bool run_pfdr(Model& m, float** g, float** g_reb)
+{
+    for (size_t i = 0; i < m.plans.size(); i++)
+        if (g[i] == nullptr || g_reb[i] == nullptr)
+            return false;
+    hipStream_t s = m.stream();
+    DevBuf dg, dr;
+    for (size_t i = 0; i < m.plans.size(); i++) {
+        const Planogram& p = m.plans[i];
+        const size_t reb = size_t(p.N_v0) * p.N_u1 * p.N_u0;
+        dg.ensure(size_t(p.elems()) * sizeof(float));
+        dr.ensure(reb * sizeof(float));
+        h2d(dg.p, g[i], size_t(p.elems()) * sizeof(float), s);
+        pfdr_device(p.geom(), p.v_m0, dg.f(), dr.f(), s);
+        d2h(g_reb[i], dr.p, reb * sizeof(float), s);
+        PETL_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    for (Planogram& p : m.plans) {
+        p.N_v1 = 1;
+        p.v1_0 = 0.0f;
+    }
+    m.touch();
+    return true;
+}
+
+// PETL::doFBP (src/petl.cpp:310-355): PFDR when N_v1 > 1 (on a copy of the geometry), then the ramp filter
+// and the inverse planogram weight of the rebinned views, and the back projection of the rebinned
+// planograms into f.  Data that are already rebinned (N_v1 = 1) are filtered and weighted in place, as the
+// reference does to the caller's arrays.
+bool run_fbp(Model& m, float** g, float* f)
+{
+    for (size_t i = 0; i < m.plans.size(); i++)
+        if (g[i] == nullptr)
+            return false;
+    Model reb;
+    reb.copy_params_from(m);
+    reb.back_mode = m.back_mode;
+    reb.path = m.path;
+    for (Planogram& p : reb.plans) {
+        p.N_v1 = 1;
+        p.v1_0 = 0.0f;
+    }
+    hipStream_t s = reb.stream();
+    std::vector<DevBuf> dreb(m.plans.size());
+    DevBuf dg;
+    for (size_t i = 0; i < m.plans.size(); i++) {
+        const Planogram& p = m.plans[i];
+        const size_t n_reb = size_t(p.N_v0) * p.N_u1 * p.N_u0;
+        dreb[i].ensure(n_reb * sizeof(float));
+        if (p.N_v1 > 1) {
+            dg.ensure(size_t(p.elems()) * sizeof(float));
+            h2d(dg.p, g[i], size_t(p.elems()) * sizeof(float), s);
+            pfdr_device(p.geom(), p.v_m0, dg.f(), dreb[i].f(), s);
+        } else {
+            h2d(dreb[i].p, g[i], n_reb * sizeof(float), s);
+        }
+        fbp_filter_device(dreb[i].f(), p.N_v0, p.N_u1, p.N_u0, p.T_u0, p.T_v0, p.v0_0, s);
+        if (p.N_v1 == 1)
+            d2h(g[i], dreb[i].p, n_reb * sizeof(float), s);
+    }
+    dg.release();
+    DevBuf df;
+    df.ensure(size_t(reb.vol_elems()) * sizeof(float));
+    std::vector<float*> ptrs = dev_ptrs(dreb);
+    reb.back(ptrs.data(), df.f(), reb.back_mode, s);
+    d2h(f, df.p, size_t(reb.vol_elems()) * sizeof(float), s);
+    PETL_HIP_CHECK(hipStreamSynchronize(s));
     return true;
 }
 
@@ -868,8 +941,21 @@ bool set_solid_angle_correction(float** g, bool do_inverse)
 // ---- outside the projector / MLEM path ----
 
 bool simulate_scatter(float**, float*) { return unsupported("simulate_scatter"); }
-bool FBP(float**, float*) { return unsupported("FBP"); }
-bool PFDR(float**, float**) { return unsupported("PFDR"); }
+// src/petl_c_interface.cpp:261-269, src/petl.cpp:310-385
+bool FBP(float** g, float* f)
+{
+    Model* m = M();
+    if (g == nullptr || f == nullptr || !m->allDefined())
+        return false;
+    return guarded("FBP", [&] { return run_fbp(*m, g, f); });
+}
+bool PFDR(float** g, float** g_reb)
+{
+    Model* m = M();
+    if (g == nullptr || g_reb == nullptr || !m->geometryDefined())
+        return false;
+    return guarded("PFDR", [&] { return run_pfdr(*m, g, g_reb); });
+}
 bool voxelize(float*, int) { return unsupported("voxelize"); }
 bool apply_corrections(float**, float**, float) { return unsupported("apply_corrections"); }
 bool diffuse(float*, int, int, int, float, float, int, bool) { return unsupported("diffuse"); }

@@ -65,6 +65,8 @@ _SIGNATURES = {
     "TV_quadForm": (ctypes.c_float, [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_float] * 3 +
                     [ctypes.c_bool]),
     "gaussian_filter": (ctypes.c_bool, [_f32] + [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int]),
+    "FBP": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32]),
+    "PFDR": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p)]),
     "add_object": (ctypes.c_bool, [ctypes.c_int, _f32, _f32, ctypes.c_float, _f32, _f32]),
     "add_module": (ctypes.c_bool, [ctypes.c_int, _f32, _f32, ctypes.c_float, _f32, _f32]),
     "clear_phantom": (ctypes.c_bool, []),
@@ -257,6 +259,40 @@ class PETLserver:
             return [np.zeros(tuple(int(s) for s in self.get_planogramSize(n)), dtype=np.float32)
                     for n in range(self.get_numPlanograms())]
         return None
+
+    def allocate_rebinned_projections(self):
+        """Zeros of the rebinned planogram shapes [N_v0][N_u1][N_u0] (src/petl_server.py:432-448)."""
+        if not self.geometry_defined():
+            return None
+        out = []
+        for n in range(self.get_numPlanograms()):
+            sh = self.get_planogramSize(n)
+            out.append(np.zeros((int(sh[1]), int(sh[2]), int(sh[3])), dtype=np.float32))
+        return out
+
+    def FBP(self, g, f=None):
+        """PFDR rebinning (when the data are not rebinned yet) + ramp filter + back projection
+        (src/petl_server.py:757-784).  The geometry is kept; data that are already rebinned are filtered in place."""
+        if not self.all_defined():
+            raise ValueError('Error: volume and PET geometry must be defined')
+        if f is None:
+            f = self.allocate_volume()
+        self.set_model()
+        if not self.libprojectors.FBP(self.floatstarstar(g), f):
+            raise TypeError('FBP failed')
+        return f
+
+    def PFDR(self, g, g_reb=None):
+        """Planogram frequency-distance rebinning (src/petl_server.py:786-811).  As in the reference this
+        changes the planogram specification to the rebinned one (N_v1 = 1)."""
+        if not self.all_defined():
+            raise ValueError('Error: volume and PET geometry must be defined')
+        if g_reb is None:
+            g_reb = self.allocate_rebinned_projections()
+        self.set_model()
+        if not self.libprojectors.PFDR(self.floatstarstar(g), self.floatstarstar(g_reb)):
+            raise TypeError('PFDR failed')
+        return g_reb
 
     def copy_data(self, x):
         if isinstance(x, list):
