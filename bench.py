@@ -40,6 +40,11 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (reference for HBM byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--pair-only", action="store_true",
+                    help="measure only the projection pair and its 16-bit storage forms (for profiling runs)")
+    ap.add_argument("--sections", default="all",
+                    help="comma list of the measurements after the projection pair: binning, autograd, response, "
+                         "recon, serving, cpu (default all)")
     ap.add_argument("--serve-rates", default="5,20,50",
                     help="request arrival rates (req/s), comma separated: below, near and above the service's "
                          "capacity")
@@ -160,6 +165,16 @@ def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
 
 def main():
     a = parse()
+    chosen = set(x.strip() for x in a.sections.split(",") if x.strip())
+
+    def want(name):
+        if a.pair_only:
+            return False
+        if name == "serving" and a.no_serve:
+            return False
+        if name == "cpu" and a.no_cpu:
+            return False
+        return "all" in chosen or name in chosen
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU per box; multi-GPU scaling is not measured",
                           "n_gpus": a.gpus}))
@@ -296,126 +311,130 @@ def main():
         del g16, f16
     out["storage_forms"] = storage_forms
 
-    # list-mode binning: device-resident coincidences between the two panel pairs into both planograms
-    # (bilinear u bins, chest-wall origin: the reference defaults); events/s of one binning call
-    nev = 1 << 24
-    gen = torch.Generator(device=dev).manual_seed(7)
-    half = nev // 2
-    R, L, H = geo["D"] / 2, geo["L"] / 2, geo["H"]
-    def panel(n, fixed_axis, sign):
-        p = torch.empty((n, 3), device=dev)
-        p[:, 0].uniform_(-L, L, generator=gen)
-        p[:, 1].uniform_(-L, L, generator=gen)
-        p[:, 2].uniform_(0.0, H, generator=gen)
-        p[:, fixed_axis] = sign * R
-        return p
-    blue = torch.cat([panel(half, 1, 1.0), panel(nev - half, 0, 1.0)]).contiguous()
-    pink = torch.cat([panel(half, 1, -1.0), panel(nev - half, 0, -1.0)]).contiguous()
-    gb = [torch.zeros(sh, dtype=torch.float32, device=dev) for sh in shapes]
-    s.bin_events_gpu(blue, pink, gb, stream=stream, sync=True)  # warm-up (allocates the sort workspace)
-    eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    reps = 3
-    eb[0].record()
-    for _ in range(reps):
-        s.bin_events_gpu(blue, pink, gb, stream=stream, sync=False)
-    eb[1].record()
-    torch.cuda.synchronize()
-    bin_ms = eb[0].elapsed_time(eb[1]) / reps
-    out["binning"] = {"events": nev, "planograms": nplan, "ms_per_call": bin_ms,
-                      "events_per_s": nev / (bin_ms * 1e-3),
-                      "workload": "uniform coincidences between the two panel pairs of the benchmark geometry, "
-                                  "device-resident; bilinear u bins, chest-wall origin; bit-exact event-order sums"}
-    del blue, pink, gb
-
-    # batched form: a batch of 8 seeded 64^3 volumes through the autograd projector (forward into both
-    # 64x64-crystal planograms, then the matched back projection as the backward), volumes per second
-    from petl_amd import autograd as pa
-    sb = PETLserver()
-    sb.print_warnings = False
-    setup_server(sb, "panel64")
-    fbatch = torch.from_numpy(np.stack([ellipsoid_phantom(64, 2.0, 6, seed=3 + b) for b in range(8)])).to(dev)
-    def ag_step():
-        x = fbatch.clone().requires_grad_(True)
-        q = pa.project(sb, x)
-        (gx,) = torch.autograd.grad(q, x, grad_outputs=[torch.ones_like(t) for t in q])
-        return gx
-    for _ in range(2):
-        ag_step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    nag = 5
-    for _ in range(nag):
-        ag_step()
-    torch.cuda.synchronize()
-    ag_s = (time.perf_counter() - t1) / nag
-    out["batched_autograd"] = {"workload": "batch of 8 seeded 64^3 volumes, two 64x64-crystal planograms "
-                                           "(63,101,64,164); forward + matched backward via torch.autograd",
-                               "ms_per_batch": 1e3 * ag_s, "volumes_per_s": 8 / ag_s}
-    del fbatch
-
-    # response model at the benchmark geometry: scintillator stopping power through 4 x 64 box modules
-    # (30.32 mm blocks, 1.28 mm gaps, 20 mm deep, behind each of the four panels), device planograms
-    modules = []
-    blk, gap, depth = 30.32, 1.28, 20.0
-    npan = int(geo["L"] // (blk + gap))
-    for m_ in range(npan):
-        cz = (m_ + 0.5) * (blk + gap)
-        for n_ in range(npan):
-            cx = (n_ - (npan - 1) / 2) * (blk + gap)
-            for c_, r_ in (([cx, geo["D"] / 2 + depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
-                           ([cx, -geo["D"] / 2 - depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
-                           ([geo["D"] / 2 + depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2]),
-                           ([-geo["D"] / 2 - depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2])):
-                modules.append((c_, r_))
-    for c_, r_ in modules:
-        s.add_module(c_, r_)
-    resp = {}
-    for os_ in (1, 3):
-        s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=True)  # warm-up
-        er = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        er[0].record()
-        s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=False)
-        er[1].record()
+    if want("binning"):
+        # list-mode binning: device-resident coincidences between the two panel pairs into both planograms
+        # (bilinear u bins, chest-wall origin: the reference defaults); events/s of one binning call
+        nev = 1 << 24
+        gen = torch.Generator(device=dev).manual_seed(7)
+        half = nev // 2
+        R, L, H = geo["D"] / 2, geo["L"] / 2, geo["H"]
+        def panel(n, fixed_axis, sign):
+            p = torch.empty((n, 3), device=dev)
+            p[:, 0].uniform_(-L, L, generator=gen)
+            p[:, 1].uniform_(-L, L, generator=gen)
+            p[:, 2].uniform_(0.0, H, generator=gen)
+            p[:, fixed_axis] = sign * R
+            return p
+        blue = torch.cat([panel(half, 1, 1.0), panel(nev - half, 0, 1.0)]).contiguous()
+        pink = torch.cat([panel(half, 1, -1.0), panel(nev - half, 0, -1.0)]).contiguous()
+        gb = [torch.zeros(sh, dtype=torch.float32, device=dev) for sh in shapes]
+        s.bin_events_gpu(blue, pink, gb, stream=stream, sync=True)  # warm-up (allocates the sort workspace)
+        eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 3
+        eb[0].record()
+        for _ in range(reps):
+            s.bin_events_gpu(blue, pink, gb, stream=stream, sync=False)
+        eb[1].record()
         torch.cuda.synchronize()
-        ms = er[0].elapsed_time(er[1])
-        resp[f"oversampling_{os_}"] = {"ms_both_planograms": ms,
-                                       "rays_per_s": os_ * os_ * sum(int(np.prod(sh)) for sh in shapes) / (ms * 1e-3)}
-    s.clear_modules()
-    out["response"] = dict(resp, workload=f"stopping power of every bin of both planograms through {len(modules)} "
-                                          "box modules (two half rays each, oversampling^2 sub-rays per bin)")
+        bin_ms = eb[0].elapsed_time(eb[1]) / reps
+        out["binning"] = {"events": nev, "planograms": nplan, "ms_per_call": bin_ms,
+                          "events_per_s": nev / (bin_ms * 1e-3),
+                          "workload": "uniform coincidences between the two panel pairs of the benchmark geometry, "
+                                      "device-resident; bilinear u bins, chest-wall origin; bit-exact event-order sums"}
+        del blue, pink, gb
 
-    # reconstruction loops at the benchmark geometry, host data in and image out as PETLserver.MLEM /
-    # PETLserver.RWLS take them: per-iteration cost from the difference of a short and a long run (the
-    # upload, weights, sensitivity and preconditioner set-up cancel)
-    s.project_gpu(g, f, stream=stream, sync=True)
-    tot = float(sum(float(x.double().sum()) for x in g))
-    gen_y = torch.Generator(device=dev).manual_seed(11)
-    y_host = [torch.poisson(x * (1e8 / tot), generator=gen_y).cpu().numpy() for x in g]
-    def timed(fn):
-        t = time.perf_counter()
-        fn()
-        return time.perf_counter() - t
-    s.MLEM(y_host, None, 1)  # warm-up
-    k1, k2 = 2, 12
-    m1 = timed(lambda: s.MLEM(y_host, None, k1, delta=2.0, beta=0.01))
-    m2 = timed(lambda: s.MLEM(y_host, None, k2, delta=2.0, beta=0.01))
-    r1 = timed(lambda: s.RWLS(y_host, None, k1, delta=0.5, beta=0.02, preconditioner="SQS"))
-    r2 = timed(lambda: s.RWLS(y_host, None, k2, delta=0.5, beta=0.02, preconditioner="SQS"))
-    f1 = timed(lambda: s.FBP([a.copy() for a in y_host]))
-    out["reconstruction"] = {
-        "workload": "benchmark geometry, Poisson data at 1e8 counts; MLEM with the relative-difference prior "
-                    "(beta 0.01, delta 2); RWLS with default weights, SQS and TV (beta 0.02, delta 0.5)",
-        "mlem_ms_per_iteration": 1e3 * (m2 - m1) / (k2 - k1),
-        "mlem_kernel_floor_ms": fwd_ms + back_ms,
-        "mlem_20_iterations_s": m1 + (m2 - m1) * (20 - k1) / (k2 - k1),
-        "rwls_ms_per_iteration": 1e3 * (r2 - r1) / (k2 - k1),
-        "rwls_kernel_floor_ms": 2 * fwd_ms + back_ms,
-        "fbp_s": f1,
-        "fbp_note": "PFDR of both planograms (hipFFT) + ramp filter + back projection, host arrays in and out",
-    }
-    del y_host
+    if want("autograd"):
+        # batched form: a batch of 8 seeded 64^3 volumes through the autograd projector (forward into both
+        # 64x64-crystal planograms, then the matched back projection as the backward), volumes per second
+        from petl_amd import autograd as pa
+        sb = PETLserver()
+        sb.print_warnings = False
+        setup_server(sb, "panel64")
+        fbatch = torch.from_numpy(np.stack([ellipsoid_phantom(64, 2.0, 6, seed=3 + b) for b in range(8)])).to(dev)
+        def ag_step():
+            x = fbatch.clone().requires_grad_(True)
+            q = pa.project(sb, x)
+            (gx,) = torch.autograd.grad(q, x, grad_outputs=[torch.ones_like(t) for t in q])
+            return gx
+        for _ in range(2):
+            ag_step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        nag = 5
+        for _ in range(nag):
+            ag_step()
+        torch.cuda.synchronize()
+        ag_s = (time.perf_counter() - t1) / nag
+        out["batched_autograd"] = {"workload": "batch of 8 seeded 64^3 volumes, two 64x64-crystal planograms "
+                                               "(63,101,64,164); forward + matched backward via torch.autograd",
+                                   "ms_per_batch": 1e3 * ag_s, "volumes_per_s": 8 / ag_s}
+        del fbatch
 
-    if not a.no_serve:
+    if want("response"):
+        # response model at the benchmark geometry: scintillator stopping power through 4 x 64 box modules
+        # (30.32 mm blocks, 1.28 mm gaps, 20 mm deep, behind each of the four panels), device planograms
+        modules = []
+        blk, gap, depth = 30.32, 1.28, 20.0
+        npan = int(geo["L"] // (blk + gap))
+        for m_ in range(npan):
+            cz = (m_ + 0.5) * (blk + gap)
+            for n_ in range(npan):
+                cx = (n_ - (npan - 1) / 2) * (blk + gap)
+                for c_, r_ in (([cx, geo["D"] / 2 + depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
+                               ([cx, -geo["D"] / 2 - depth / 2, cz], [blk / 2, depth / 2, blk / 2]),
+                               ([geo["D"] / 2 + depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2]),
+                               ([-geo["D"] / 2 - depth / 2, cx, cz], [depth / 2, blk / 2, blk / 2])):
+                    modules.append((c_, r_))
+        for c_, r_ in modules:
+            s.add_module(c_, r_)
+        resp = {}
+        for os_ in (1, 3):
+            s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=True)  # warm-up
+            er = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            er[0].record()
+            s.ray_trace_gpu(g, oversampling=os_, stopping=True, stream=stream, sync=False)
+            er[1].record()
+            torch.cuda.synchronize()
+            ms = er[0].elapsed_time(er[1])
+            resp[f"oversampling_{os_}"] = {"ms_both_planograms": ms,
+                                           "rays_per_s": os_ * os_ * sum(int(np.prod(sh)) for sh in shapes) / (ms * 1e-3)}
+        s.clear_modules()
+        out["response"] = dict(resp, workload=f"stopping power of every bin of both planograms through {len(modules)} "
+                                              "box modules (two half rays each, oversampling^2 sub-rays per bin)")
+
+    if want("recon"):
+        # reconstruction loops at the benchmark geometry, host data in and image out as PETLserver.MLEM /
+        # PETLserver.RWLS take them: per-iteration cost from the difference of a short and a long run (the
+        # upload, weights, sensitivity and preconditioner set-up cancel)
+        s.project_gpu(g, f, stream=stream, sync=True)
+        tot = float(sum(float(x.double().sum()) for x in g))
+        gen_y = torch.Generator(device=dev).manual_seed(11)
+        y_host = [torch.poisson(x * (1e8 / tot), generator=gen_y).cpu().numpy() for x in g]
+        def timed(fn):
+            t = time.perf_counter()
+            fn()
+            return time.perf_counter() - t
+        s.MLEM(y_host, None, 1)  # warm-up
+        k1, k2 = 2, 12
+        m1 = timed(lambda: s.MLEM(y_host, None, k1, delta=2.0, beta=0.01))
+        m2 = timed(lambda: s.MLEM(y_host, None, k2, delta=2.0, beta=0.01))
+        r1 = timed(lambda: s.RWLS(y_host, None, k1, delta=0.5, beta=0.02, preconditioner="SQS"))
+        r2 = timed(lambda: s.RWLS(y_host, None, k2, delta=0.5, beta=0.02, preconditioner="SQS"))
+        f1 = timed(lambda: s.FBP([x.copy() for x in y_host]))
+        out["reconstruction"] = {
+            "workload": "benchmark geometry, Poisson data at 1e8 counts; MLEM with the relative-difference prior "
+                        "(beta 0.01, delta 2); RWLS with default weights, SQS and TV (beta 0.02, delta 0.5)",
+            "mlem_ms_per_iteration": 1e3 * (m2 - m1) / (k2 - k1),
+            "mlem_kernel_floor_ms": fwd_ms + back_ms,
+            "mlem_20_iterations_s": m1 + (m2 - m1) * (20 - k1) / (k2 - k1),
+            "rwls_ms_per_iteration": 1e3 * (r2 - r1) / (k2 - k1),
+            "rwls_kernel_floor_ms": 2 * fwd_ms + back_ms,
+            "fbp_s": f1,
+            "fbp_note": "PFDR of both planograms (hipFFT) + ramp filter + back projection, host arrays in and out",
+        }
+        del y_host
+
+    if want("serving"):
         del g, f, fb
         torch.cuda.empty_cache()
         data = serving_datasets(torch, dev, seed=500)
@@ -434,7 +453,7 @@ def main():
             "requests": a.serve_requests,
             "runs": runs,
         }
-    if not a.no_cpu:
+    if want("cpu"):
         v, cores, desc = cpu_baseline(a.cpu_rows, f_host)
         out["cpu_baseline"] = {"value": v, "unit": "fwd+back projection pairs/s", "cores": cores, "kind": "port",
                                "sample": desc}

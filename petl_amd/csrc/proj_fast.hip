@@ -142,8 +142,10 @@ __device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, float* hb, flo
 }
 
 // J views per wave, C output chunks of 64 bins, SB-slice LDS blocks, R u1 rows per workgroup, storage SF of
-// the volume and SG of the planogram and the data y.
-template <int J, int C, int SB, int R, int SF, int SG>
+// the volume and SG of the planogram and the data y.  RES: the least-squares residual epilogue
+// (EPI_RESID) in place of the runtime epilogue, a separate instance so that the projection kernels keep
+// no weight pointer live.
+template <int J, int C, int SB, int R, int SF, int SG, bool RES = false>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
                                                                const typename Stor<SF>::T* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ ftab,
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
                     TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
-                    if (epilogue == EPI_RESID) {
+                    if (RES) {
                         // residual of a zero projection: (resp * 0 - y) * w
                         for (int p = lane; p < Nu0; p += 64) {
 #pragma clang fp contract(off)
@@ -345,14 +347,14 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                         const int64_t gi = int64_t(gr - g) + p;
                         if (resp)
                             out = out * resp[gi];
-                        if (epilogue == EPI_RATIO) {
-                            out = (out == 0.0f) ? 0.0f : Stor<SG>::ld(yv[gi], A.y_scale) / out;
-                            if (resp)
-                                out = out * resp[gi];
-                        } else if (epilogue == EPI_RESID) {
+                        if (RES) {
                             out = out - Stor<SG>::ld(yv[gi], A.y_scale);
                             if (A.wts)
                                 out = out * A.wts[gi];
+                        } else if (epilogue == EPI_RATIO) {
+                            out = (out == 0.0f) ? 0.0f : Stor<SG>::ld(yv[gi], A.y_scale) / out;
+                            if (resp)
+                                out = out * resp[gi];
                         }
                         gr[p] = Stor<SG>::st(out, A.g_inv_scale);
                     }
@@ -557,8 +559,25 @@ using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdA
 // FWD_J views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
 // two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower)
 template <int SF, int SG>
-fwd_fn<SF, SG> pick_fwd(int C)
+fwd_fn<SF, SG> pick_fwd(int C, bool res = false)
 {
+    if constexpr (SF == STORE_F32 && SG == STORE_F32) {
+        if (res) {
+            switch (C) {
+            case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG, true>;
+            case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG, true>;
+            case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG, true>;
+            case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG, true>;
+            case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG, true>;
+            case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG, true>;
+            case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG, true>;
+            case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG, true>;
+            default: return nullptr;
+            }
+        }
+    } else if (res) {
+        return nullptr;  // the residual epilogue is fp32 only
+    }
     switch (C) {
     case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG>;
     case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG>;
@@ -850,7 +869,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
     A.wts = wts;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C);
+    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, epilogue == EPI_RESID);
     if (!fn || fp.J != FWD_J || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
