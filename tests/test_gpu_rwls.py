@@ -190,6 +190,16 @@ def test_device_loop_matches_host_composed_loop(oracle):
     assert rel(got, want) <= 1e-5
 
 
+def test_device_loop_restarts_like_the_host_loop(oracle):
+    """55 iterations cross the restart at n = 50 (d = u, src/petl_server.py:1807-1808): the device loop and
+    the host-composed loop take the same directions (the same kernels; only the order of the fp32 inner
+    products differs, which 55 conjugate-gradient steps amplify to ~1e-4)."""
+    s, plans, vol, y = problem(oracle, psis=(0.0,), n=20, L=40.0, D=80.0, counts=1e6)
+    got = s.RWLS(y, None, 55, delta=0.5, beta=0.02, preconditioner="SQS")
+    want = host_rwls(s, y, 55, delta=0.5, beta=0.02)
+    assert rel(got, want) <= 1e-3, rel(got, want)
+
+
 def host_rwls(s, g, numIter, delta, beta):
     """src/petl_server.py:1719-1845 with SQS, default weights and the nonnegativity constraint."""
     W = s.copy_data(g)
