@@ -163,6 +163,18 @@ def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
             "pool_hits": st["pool_hits"]}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary (copy-calibrated FETCH_SIZE +
+    WRITE_SIZE), or None when it has no entry for the kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_panel128.json")
+    try:
+        with open(path) as fh:
+            entry = json.load(fh)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if entry is None else float(entry["traffic_bytes"])
+
+
 def main():
     a = parse()
     chosen = set(x.strip() for x in a.sections.split(",") if x.strip())
@@ -269,7 +281,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, false>"),
+            "traffic_source": "profiles/traffic_panel128.json (table in profiles/kernels_panel128.md): rocprofv3 "
+                              "--pmc FETCH_SIZE and WRITE_SIZE of this kernel (one run each, --pair-only), scaled "
+                              "by the same counters on a 256 MiB copy",
             "alg_bytes_per_launch": alg_bytes,
         },
         # the bound that actually limits this kernel: FP32 vector arithmetic (published peak 157.3 TFLOP/s
