@@ -552,6 +552,21 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
 // ---------------------------------------------------------------------------------------------
 // host-side kernel selection
 // ---------------------------------------------------------------------------------------------
+// z rows per back_z wave: long chunks re-read less of h2 ((ZC+1)/ZC loads per voxel), but the grid
+// has only ceil(Na/64)*Ns*ceil(Nz/ZC) waves, and each wave walks all v1 rows serially.  Chunks
+// shrink until at least BACK_Z_WAVES waves are in flight (16 per CU).  Measured per launch on
+// MI355X: 128^3 volume, 101 v1 rows: 245 / 174 / 155 / 156 us at ZC 32 / 16 / 8 / 4; 64^3, 63 rows:
+// 128 / 76 / 53 / 41 us.
+constexpr int BACK_Z_WAVES = 4096;
+static int back_z_chunk(int Na, int Ns, int Nz)
+{
+    const int64_t cols = int64_t((Na + 63) / 64) * Ns;
+    int zc = 32;
+    while (zc > 4 && cols * ((Nz + zc - 1) / zc) < BACK_Z_WAVES)
+        zc /= 2;
+    return zc;
+}
+
 template <int SF, int SG>
 using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdArgs, const int2*, const float*,
                         const int4*, const float*, int, const typename Stor<SG>::T*, const float*);
@@ -940,8 +955,16 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
                        static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     PETL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(back_z_kernel<32>, dim3((fp.Na + 63) / 64, fp.Ns, (fp.Nz + 31) / 32), dim3(64), 0, s,
-                       workspace, f, A, fp.dev.ztab, rows);
+    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
+    const dim3 zgrid((fp.Na + 63) / 64, fp.Ns, (fp.Nz + zc - 1) / zc);
+    if (zc == 32)
+        hipLaunchKernelGGL(back_z_kernel<32>, zgrid, dim3(64), 0, s, workspace, f, A, fp.dev.ztab, rows);
+    else if (zc == 16)
+        hipLaunchKernelGGL(back_z_kernel<16>, zgrid, dim3(64), 0, s, workspace, f, A, fp.dev.ztab, rows);
+    else if (zc == 8)
+        hipLaunchKernelGGL(back_z_kernel<8>, zgrid, dim3(64), 0, s, workspace, f, A, fp.dev.ztab, rows);
+    else
+        hipLaunchKernelGGL(back_z_kernel<4>, zgrid, dim3(64), 0, s, workspace, f, A, fp.dev.ztab, rows);
     PETL_LAUNCH_CHECK();
 }
 
