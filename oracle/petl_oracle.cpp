@@ -1140,6 +1140,28 @@ void orc_backproject_f64(const orc_plan* P, const orc_vol* V, const double* g, d
 }
 
 // Taps of one bin (for brute-force / point-source checks).  Returns the tap count; writes at most cap.
+// Work measure of one projection: the number of (voxel, bin) pairs with a non-zero fp32 footprint weight
+// over v1 rows [v1_begin, v1_end), unmasked bins only (the visits of the forward loop above,
+// src/projectors.cu:79-183, that add a non-zero term).
+int64_t orc_count_taps(const orc_plan* P, const orc_vol* V, int v1_begin, int v1_end, int nthreads)
+{
+    const int NV = (v1_end - v1_begin) * P->N_v0;
+    int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total)
+    for (int vv = 0; vv < NV; vv++) {
+        const int iv1 = v1_begin + vv / P->N_v0, iv0 = vv % P->N_v0;
+        int64_t n = 0;
+        for (int iu1 = 0; iu1 < P->N_u1; iu1++)
+            for (int iu0 = 0; iu0 < P->N_u0; iu0++)
+                if (!bin_masked(*P, iv1, iv0, iu1, iu0))
+                    enumerate_bin<float>(*P, *V, iv1, iv0, iu1, iu0, [&](int64_t, float w0, float w1) {
+                        n += (w0 * w1 != 0.0f) ? 1 : 0;
+                    });
+        total += n;
+    }
+    return total;
+}
+
 int orc_bin_taps(const orc_plan* P, const orc_vol* V, int iv1, int iv0, int iu1, int iu0, int64_t* idx,
                  double* w, int cap, double* scale)
 {

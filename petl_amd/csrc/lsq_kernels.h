@@ -24,6 +24,8 @@ int64_t red_scratch_doubles(int64_t n);
 // *out = sum_i double(x_i * y_i [* w_i]) (w may be null); fixed reduction order
 void launch_dot(const float* x, const float* y, const float* w, int64_t n, double* scratch, double* out,
                 hipStream_t s);
+// *out = sum of the n FP64 partials in index order (one workgroup)
+void launch_sum_parts(const double* part, int64_t n, double* out, hipStream_t s);
 // *out = p-Huber TV cost of f (src/total_variation.cu:80-216) or, with d != null, the quadratic form
 // <d, R''(f) d> (src/total_variation.cu:218-417); beta/delta rules of TV_gradient apply.
 void launch_tv_reduce(const float* f, const float* d, int N1, int N2, int N3, float delta, float beta, float p,

@@ -424,6 +424,14 @@ void launch_dot(const float* x, const float* y, const float* w, int64_t n, doubl
     PETL_LAUNCH_CHECK();
 }
 
+void launch_sum_parts(const double* part, int64_t n, double* out, hipStream_t s)
+{
+    if (n > INT32_MAX)
+        throw hip_error("launch_sum_parts: too many partials");
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(VT), 0, s, part, int(n), out);
+    PETL_LAUNCH_CHECK();
+}
+
 void launch_tv_reduce(const float* f, const float* d, int N1, int N2, int N3, float delta, float beta, float p,
                       double* scratch, double* out, hipStream_t s)
 {

@@ -89,6 +89,13 @@ struct Model {
     // (0,1) = all.
     void forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
                  int subset = 0, int nsubsets = 1, float* const* wts = nullptr);
+    // Partials of sum_b w_b (resp_b (A_i f)_b)^2 for planogram i without writing the projection (fast
+    // kernels only): returns the number of FP64 partials written to part (at most cap; summed in index
+    // order they give the total), or 0 when planogram i takes the generic kernels or cap is too small.
+    int64_t forward_sumsq_parts(int i, const float* f, const float* resp, const float* wts, double* part,
+                                int64_t cap, hipStream_t s);
+    // FP64 partials forward_sumsq_parts needs for planogram i (0 for the generic kernels)
+    int64_t sumsq_parts(int i);
     // f = sum_i A_i^T g_i (overwrites f)
     void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
     // The same with the volume in storage format sf and the planograms in sg (STORE_*; fast kernels only,

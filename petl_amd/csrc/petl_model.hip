@@ -211,6 +211,25 @@ void Model::forward(float* const* g, const float* f, int epilogue, float* const*
     }
 }
 
+int64_t Model::sumsq_parts(int i)
+{
+    prepare();
+    return dev.fast[i] ? fast_forward_groups(*dev.fast[i]) : 0;
+}
+
+int64_t Model::forward_sumsq_parts(int i, const float* f, const float* resp, const float* wts, double* part,
+                                   int64_t cap, hipStream_t s)
+{
+    prepare();
+    if (!dev.fast[i])
+        return 0;
+    const int64_t n = fast_forward_groups(*dev.fast[i]);
+    if (n > cap)
+        return 0;
+    fast_forward_sumsq(*dev.fast[i], f, resp, wts, part, s);
+    return n;
+}
+
 void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets)
 {
     prepare();

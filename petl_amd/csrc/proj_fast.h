@@ -56,6 +56,12 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
 void fast_forward_st(const FastPlan& fp, int sf, int sg, void* g, const void* f, int epilogue, const void* y,
                      const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale, float y_scale,
                      float g_scale);
+// Weighted sum of squares of one projection, sum_b w_b (resp_b (A f)_b)^2 (w, resp nullable), without
+// writing it: part[k] receives the FP64 partial of forward workgroup k; fast_forward_groups() of them.
+// Summing the partials in index order gives a deterministic total.
+int64_t fast_forward_groups(const FastPlan& fp, int v1_first = 0, int v1_stride = 1);
+void fast_forward_sumsq(const FastPlan& fp, const float* f, const float* resp, const float* wts, double* part,
+                        hipStream_t s, int v1_first = 0, int v1_stride = 1);
 // Bytes of scratch fast_back needs (the intermediate u0-gathered slabs).
 size_t fast_back_workspace_bytes(const FastPlan& fp);
 // f += A^T g (mode BACK_PETL or BACK_MATCHED), one planogram.

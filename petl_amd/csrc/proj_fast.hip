@@ -77,8 +77,14 @@ struct FwdArgs {
     int v1_first, v1_stride;
     int xrow;  // padded row length of the u0 table
     float f_scale, y_scale, g_inv_scale;  // storage scales (16-bit formats; 1 for fp32)
-    const float* wts;                     // EPI_RESID weights (nullable)
+    const float* wts;                     // EPI_RESID / sum-of-squares weights (nullable)
+    double* part;                         // sum-of-squares instance: one partial per workgroup
 };
+
+// compile-time epilogue of a forward instance: the runtime epilogue argument (none / ratio), the
+// least-squares residual (g = (resp * Af - y) * w), or the weighted sum of squares sum_b w_b (resp * Af)_b^2
+// with no store (per-workgroup FP64 partials in a fixed order)
+constexpr int EK_RUNTIME = 0, EK_RESID = 1, EK_SUMSQ = 2;
 
 // =============================================================================================
 // forward
@@ -141,11 +147,46 @@ __device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, float* hb, flo
             hb[L.dst[e]] = L.wa[e] * Stor<ST>::ld(L.a[e], f_scale) + L.wb[e] * Stor<ST>::ld(L.b[e], f_scale);
 }
 
+// Inputs that the forward epilogue reads per bin of one view row: data y (ratio and residual), weights
+// (residual) and response.  Every lane loads at its bin clamped to the row, with no per-bin branch, so the
+// loads of a row issue back to back; the epilogue loads row j+1 before it stores row j, because loads and
+// stores retire through one in-order counter and a load issued after a store waits for that store.
+template <int C>
+struct EpiIn {
+    float y[C], w[C], rs[C];
+};
+
+template <int C, int SG, int EK>
+__device__ __forceinline__ void epi_load(EpiIn<C>& in, const FwdArgs& A, int epilogue,
+                                         const typename Stor<SG>::T* __restrict__ yv,
+                                         const float* __restrict__ resp, int64_t row, int Nu0, int lane)
+{
+    int64_t gi[C];
+#pragma unroll
+    for (int c = 0; c < C; c++)
+        gi[c] = row + min(64 * c + lane, Nu0 - 1);
+    if (EK == EK_RESID || (EK == EK_RUNTIME && epilogue == EPI_RATIO)) {
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            in.y[c] = Stor<SG>::ld(yv[gi[c]], A.y_scale);
+    }
+    if (EK != EK_RUNTIME && A.wts) {
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            in.w[c] = A.wts[gi[c]];
+    }
+    if (resp) {
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            in.rs[c] = resp[gi[c]];
+    }
+}
+
 // J views per wave, C output chunks of 64 bins, SB-slice LDS blocks, R u1 rows per workgroup, storage SF of
-// the volume and SG of the planogram and the data y.  RES: the least-squares residual epilogue
-// (EPI_RESID) in place of the runtime epilogue, a separate instance so that the projection kernels keep
-// no weight pointer live.
-template <int J, int C, int SB, int R, int SF, int SG, bool RES = false>
+// the volume and SG of the planogram and the data y.  EK: the compile-time epilogue (EK_*); the residual
+// and sum-of-squares forms are separate instances so that the projection kernels keep no weight pointer
+// live.
+template <int J, int C, int SB, int R, int SF, int SG, int EK = EK_RUNTIME>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
                                                                const typename Stor<SF>::T* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ ftab,
@@ -156,8 +197,10 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                                                                const float* __restrict__ resp)
 {
     using TS = typename Stor<SG>::T;
+    constexpr bool RES = EK == EK_RESID, SQ = EK == EK_SUMSQ;
     extern __shared__ float hbuf[];
     __shared__ int s_lo, s_hi;
+    __shared__ double s_sq[FWD_WAVES];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -188,27 +231,34 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
             for (int j = 0; j < J; j++) {
                 const int iv0 = iv0_base + FWD_WAVES * j;
                 if (iv0 < Nv0) {
-                    TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                    const int64_t row = ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
                     if (RES) {
                         // residual of a zero projection: (resp * 0 - y) * w
-                        for (int p = lane; p < Nu0; p += 64) {
+                        EpiIn<C> in;
+                        epi_load<C, SG, EK>(in, A, epilogue, yv, resp, row, Nu0, lane);
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
 #pragma clang fp contract(off)
-                            const int64_t gi = int64_t(gr - g) + p;
+                            const int p = 64 * c + lane;
                             float out = 0.0f;
                             if (resp)
-                                out = out * resp[gi];
-                            out = out - Stor<SG>::ld(yv[gi], A.y_scale);
+                                out = out * in.rs[c];
+                            out = out - in.y[c];
                             if (A.wts)
-                                out = out * A.wts[gi];
-                            gr[p] = Stor<SG>::st(out, A.g_inv_scale);
+                                out = out * in.w[c];
+                            if (p < Nu0)
+                                g[row + p] = Stor<SG>::st(out, A.g_inv_scale);
                         }
                         continue;
                     }
-                    for (int p = lane; p < Nu0; p += 64)
-                        gr[p] = Stor<SG>::st(0.0f, 1.0f);
+                    if (!SQ)
+                        for (int p = lane; p < Nu0; p += 64)
+                            g[row + p] = Stor<SG>::st(0.0f, 1.0f);
                 }
             }
         }
+        if (SQ && tid == 0)
+            A.part[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = 0.0;
         return;
     }
 
@@ -319,47 +369,80 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         }
     }
 
-    // epilogue: out[p] = (Q[p] + W[p+1]) * scale, masks, optional ratio
+    // epilogue: out[p] = (Q[p] + W[p+1]) * scale, masks, optional response / ratio / residual / square sum
+    double sq = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (r >= nrows)
             continue;
+        const int64_t row_r = int64_t(iv1) * Nv0 * Nu1 + iu1_0 + r;
+        EpiIn<C> in[2];
+        epi_load<C, SG, EK>(in[0], A, epilogue, yv, resp, (row_r + int64_t(min(iv0_base, Nv0 - 1)) * Nu1) * Nu0,
+                            Nu0, lane);
 #pragma unroll
         for (int j = 0; j < J; j++) {
+            if (j + 1 < J) {
+                const int iv0n = min(iv0_base + FWD_WAVES * (j + 1), Nv0 - 1);
+                epi_load<C, SG, EK>(in[(j + 1) & 1], A, epilogue, yv, resp, (row_r + int64_t(iv0n) * Nu1) * Nu0,
+                                    Nu0, lane);
+            }
+            const EpiIn<C>& cur = in[j & 1];
             const int iv0 = iv0_base + FWD_WAVES * j;
             if (iv0 < Nv0) {
                 const float sc = rmask[r] ? 0.0f : fscale[int64_t(iv1) * Nv0 + iv0];
                 const float v0 = coordf(iv0, P.Tv0, P.v0_0);
-                TS* gr = g + ((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1_0 + r) * Nu0;
+                TS* gr = g + (row_r + int64_t(iv0) * Nu1) * Nu0;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
+#pragma clang fp contract(off)
                     float wn = __shfl_down(W[r][j][c], 1);
                     const int cn = (c + 1 < C) ? c + 1 : c;
                     const float w0 = (c + 1 < C) ? __shfl(W[r][j][cn], 0) : 0.0f;
                     if (lane == 63)
                         wn = w0;
                     const int p = 64 * c + lane;
-                    if (p < Nu0) {
-#pragma clang fp contract(off)
-                        float out = (Q[r][j][c] + wn) * sc;
-                        if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
-                            out = 0.0f;
-                        const int64_t gi = int64_t(gr - g) + p;
-                        if (resp)
-                            out = out * resp[gi];
-                        if (RES) {
-                            out = out - Stor<SG>::ld(yv[gi], A.y_scale);
-                            if (A.wts)
-                                out = out * A.wts[gi];
-                        } else if (epilogue == EPI_RATIO) {
-                            out = (out == 0.0f) ? 0.0f : Stor<SG>::ld(yv[gi], A.y_scale) / out;
-                            if (resp)
-                                out = out * resp[gi];
-                        }
-                        gr[p] = Stor<SG>::st(out, A.g_inv_scale);
+                    float out = (Q[r][j][c] + wn) * sc;
+                    if (u0_masked(P, coordf(p, P.Tu0, P.u0_0), v0))
+                        out = 0.0f;
+                    if (resp)
+                        out = out * cur.rs[c];
+                    if (SQ) {
+                        // the weighted inner product's term: float(v * v) * w, summed in FP64
+                        float t = out * out;
+                        if (A.wts)
+                            t = t * cur.w[c];
+                        if (p < Nu0)
+                            sq += double(t);
+                        continue;
                     }
+                    if (RES) {
+                        out = out - cur.y[c];
+                        if (A.wts)
+                            out = out * cur.w[c];
+                    } else if (epilogue == EPI_RATIO) {
+                        out = (out == 0.0f) ? 0.0f : cur.y[c] / out;
+                        if (resp)
+                            out = out * cur.rs[c];
+                    }
+                    if (p < Nu0)
+                        gr[p] = Stor<SG>::st(out, A.g_inv_scale);
                 }
             }
+        }
+    }
+    if (SQ) {
+        // fixed-order workgroup sum: butterfly within each wave, then the waves in order
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+            sq += __shfl_xor(sq, o);
+        if (lane == 0)
+            s_sq[wave] = sq;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < FWD_WAVES; w++)
+                t += s_sq[w];
+            A.part[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
         }
     }
 }
@@ -574,36 +657,30 @@ using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdA
 // FWD_J views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
 // two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower)
 template <int SF, int SG>
-fwd_fn<SF, SG> pick_fwd(int C, bool res = false)
+fwd_fn<SF, SG> pick_fwd(int C, int ek = EK_RUNTIME)
 {
+#define PETL_FWD_BY_C(EKV)                                                \
+    switch (C) {                                                          \
+    case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG, EKV>;          \
+    case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG, EKV>;          \
+    case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG, EKV>;          \
+    case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG, EKV>;          \
+    case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG, EKV>;          \
+    case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG, EKV>;          \
+    case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG, EKV>;          \
+    case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG, EKV>;          \
+    default: return nullptr;                                              \
+    }
     if constexpr (SF == STORE_F32 && SG == STORE_F32) {
-        if (res) {
-            switch (C) {
-            case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG, true>;
-            case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG, true>;
-            case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG, true>;
-            case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG, true>;
-            case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG, true>;
-            case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG, true>;
-            case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG, true>;
-            case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG, true>;
-            default: return nullptr;
-            }
-        }
-    } else if (res) {
-        return nullptr;  // the residual epilogue is fp32 only
+        if (ek == EK_RESID)
+            PETL_FWD_BY_C(EK_RESID)
+        if (ek == EK_SUMSQ)
+            PETL_FWD_BY_C(EK_SUMSQ)
+    } else if (ek != EK_RUNTIME) {
+        return nullptr;  // the residual and sum-of-squares epilogues are fp32 only
     }
-    switch (C) {
-    case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG>;
-    case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG>;
-    case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG>;
-    case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG>;
-    case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG>;
-    case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG>;
-    case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG>;
-    case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG>;
-    default: return nullptr;
-    }
+    PETL_FWD_BY_C(EK_RUNTIME)
+#undef PETL_FWD_BY_C
 }
 
 inline float fbits(int i)
@@ -861,7 +938,8 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
 template <int SF, int SG>
 static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
                            const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
-                           float y_scale, float g_scale, const float* wts = nullptr)
+                           float y_scale, float g_scale, const float* wts = nullptr, int ek = -1,
+                           double* part = nullptr)
 {
     using TG = typename Stor<SG>::T;
     using TF = typename Stor<SF>::T;
@@ -883,8 +961,11 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.y_scale = (SG == STORE_F16) ? y_scale : 1.0f;
     A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
     A.wts = wts;
+    A.part = part;
+    if (ek < 0)
+        ek = (epilogue == EPI_RESID) ? EK_RESID : EK_RUNTIME;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, epilogue == EPI_RESID);
+    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, ek);
     if (!fn || fp.J != FWD_J || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
     const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
@@ -899,6 +980,19 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
 {
     fast_forward_t<STORE_F32, STORE_F32>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, 1.0f, 1.0f, 1.0f,
                                          wts);
+}
+
+int64_t fast_forward_groups(const FastPlan& fp, int v1_first, int v1_stride)
+{
+    const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    return int64_t(fp.ngroups) * ((fp.P.Nu1 + fp.R - 1) / fp.R) * nv1;
+}
+
+void fast_forward_sumsq(const FastPlan& fp, const float* f, const float* resp, const float* wts, double* part,
+                        hipStream_t s, int v1_first, int v1_stride)
+{
+    fast_forward_t<STORE_F32, STORE_F32>(fp, nullptr, f, EPI_NONE, nullptr, resp, s, v1_first, v1_stride, 1.0f,
+                                         1.0f, 1.0f, wts, EK_SUMSQ, part);
 }
 
 // (volume, planogram) formats: both fp32, both 16-bit of one kind, or an fp32 volume with 16-bit planograms

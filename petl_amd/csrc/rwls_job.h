@@ -53,6 +53,8 @@ private:
     std::vector<float*> gp_, rp_, Pdp_, Wp_, respp_;
     double* slots_ = nullptr;    // device result slots
     double* scratch_ = nullptr;  // reduction partials
+    int64_t scratch_cap_ = 0;    // doubles at scratch_
+    bool fuse_sumsq_ = false;    // <Pd, Pd>_W straight from the forward (Pd itself is not needed)
     double gg_old_ = 0.0;
     bool r_weighted_ = false;  // r already holds W r (weighting fused into the residual projection)
 };

@@ -91,7 +91,16 @@ void RwlsJob::setup(float* const* g, const float* f0, bool f0_nonzero, float* co
     int64_t nmax = V_;
     for (int64_t e : G_)
         nmax = std::max(nmax, e);
-    red_.ensure(sizeof(double) * size_t(NSLOTS + red_scratch_doubles(nmax)));
+    // with the non-negativity constraint Pd only feeds <Pd, Pd>_W, which the fast forward can sum without
+    // writing Pd (the unconstrained update reads Pd itself)
+    scratch_cap_ = red_scratch_doubles(nmax);
+    fuse_sumsq_ = prm_.nonneg;
+    for (size_t i = 0; i < nP_; i++) {
+        const int64_t np = m.sumsq_parts(int(i));
+        fuse_sumsq_ = fuse_sumsq_ && np > 0;
+        scratch_cap_ = std::max(scratch_cap_, np);
+    }
+    red_.ensure(sizeof(double) * size_t(NSLOTS + scratch_cap_));
     slots_ = static_cast<double*>(red_.p);
     scratch_ = slots_ + NSLOTS;
 
@@ -189,11 +198,21 @@ bool RwlsJob::iterate(int n, hipStream_t s)
     gg_old_ = double(u_grad);
     launch_copy_vec(grad_old_.f(), grad_.f(), V_, s);
 
-    project(d_.f(), Pdp_, s);
     // num = <d, grad>; denominator <Pd, Pd>_W (+ <d, R''(f) d>), all read back at once
+    if (fuse_sumsq_) {
+        for (size_t i = 0; i < nP_; i++) {
+            const int64_t np = m_.forward_sumsq_parts(int(i), d_.f(), prm_.response ? respp_[i] : nullptr,
+                                                      weighted ? Wp_[i] : nullptr, scratch_, scratch_cap_, s);
+            if (np == 0)
+                throw hip_error("RWLS: the fused <Pd, Pd> forward is unavailable for this planogram");
+            launch_sum_parts(scratch_, np, slots_ + 2 + i, s);
+        }
+    } else {
+        project(d_.f(), Pdp_, s);
+        for (size_t i = 0; i < nP_; i++)
+            launch_dot(Pdp_[i], Pdp_[i], weighted ? Wp_[i] : nullptr, G_[i], scratch_, slots_ + 2 + i, s);
+    }
     launch_dot(d_.f(), grad_.f(), nullptr, V_, scratch_, slots_ + 0, s);
-    for (size_t i = 0; i < nP_; i++)
-        launch_dot(Pdp_[i], Pdp_[i], weighted ? Wp_[i] : nullptr, G_[i], scratch_, slots_ + 2 + i, s);
     if (prm_.beta > 0.0f)
         launch_tv_reduce(f_.f(), d_.f(), m_.numZ, m_.numY, m_.numX, prm_.delta, prm_.beta, 1.0f, scratch_,
                          slots_ + 1, s);
