@@ -76,7 +76,7 @@ def lib():
         L.orc_bin_taps.argtypes = [P, V, c_int, c_int, c_int, c_int, i64p, dp, c_int,
                                    ctypes.POINTER(c_double)]
         L.orc_bin_taps.restype = c_int
-        L.orc_count_taps.argtypes = [P, V, c_int, c_int, c_int]
+        L.orc_count_taps.argtypes = [P, V, c_int, c_int, c_int, c_int]
         L.orc_count_taps.restype = ctypes.c_int64
         L.orc_rd_gradient_f32.argtypes = [fp, fp, c_int, c_int, c_int, c_float, c_float]
         L.orc_rd_gradient_f64.argtypes = [dp, dp, c_int, c_int, c_int, c_double, c_double]
@@ -184,10 +184,12 @@ def bin_taps(plan, vol, iv1, iv0, iu1, iu0, cap=1 << 16):
     return idx[:n], w[:n], scale.value
 
 
-def count_taps(plan, vol, v1_begin=0, v1_end=None, nthreads=None):
-    """Non-zero fp32 footprint taps of one projection over v1 rows [v1_begin, v1_end) (unmasked bins)."""
+def count_taps(plan, vol, v1_begin=0, v1_end=None, f64=False, nthreads=None):
+    """Non-zero footprint taps of one projection over v1 rows [v1_begin, v1_end) (unmasked bins), weights
+    formed in fp32 (default) or FP64."""
     v1_end = plan.N_v1 if v1_end is None else v1_end
-    return int(lib().orc_count_taps(ctypes.byref(plan), ctypes.byref(vol), int(v1_begin), int(v1_end), _nt(nthreads)))
+    return int(lib().orc_count_taps(ctypes.byref(plan), ctypes.byref(vol), int(v1_begin), int(v1_end), int(f64),
+                                    _nt(nthreads)))
 
 
 def bin_masked(plan, iv1, iv0, iu1, iu0):

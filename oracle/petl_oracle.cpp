@@ -1095,6 +1095,30 @@ void ray_trace_plan(const orc_plan& P, const orc_solid* s, int n, int oversampli
 // ------------------------------------------------------------------------------------------------
 // C entry points (loaded with ctypes by the tests)
 // ------------------------------------------------------------------------------------------------
+// Work measure of one projection: the number of (voxel, bin) pairs with a non-zero footprint weight over v1
+// rows [v1_begin, v1_end), unmasked bins only (the visits of the forward loop above,
+// src/projectors.cu:79-183, that add a non-zero term), with the weights formed in fp32 (f64 = 0, the GPU's
+// arithmetic) or FP64 (f64 = 1; a sliver overlap that rounds to zero in fp32 still counts).
+template <typename R>
+static int64_t count_taps(const orc_plan& P, const orc_vol& V, int v1_begin, int v1_end, int nthreads)
+{
+    const int NV = (v1_end - v1_begin) * P.N_v0;
+    int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total)
+    for (int vv = 0; vv < NV; vv++) {
+        const int iv1 = v1_begin + vv / P.N_v0, iv0 = vv % P.N_v0;
+        int64_t n = 0;
+        for (int iu1 = 0; iu1 < P.N_u1; iu1++)
+            for (int iu0 = 0; iu0 < P.N_u0; iu0++)
+                if (!bin_masked(P, iv1, iv0, iu1, iu0))
+                    enumerate_bin<R>(P, V, iv1, iv0, iu1, iu0, [&](int64_t, R w0, R w1) {
+                        n += (w0 * w1 != R(0)) ? 1 : 0;
+                    });
+        total += n;
+    }
+    return total;
+}
+
 extern "C" {
 
 int orc_plan_init(float psi_deg, float D, float L, float H, float v_m0, float v_m1, float T, orc_plan* out)
@@ -1140,26 +1164,10 @@ void orc_backproject_f64(const orc_plan* P, const orc_vol* V, const double* g, d
 }
 
 // Taps of one bin (for brute-force / point-source checks).  Returns the tap count; writes at most cap.
-// Work measure of one projection: the number of (voxel, bin) pairs with a non-zero fp32 footprint weight
-// over v1 rows [v1_begin, v1_end), unmasked bins only (the visits of the forward loop above,
-// src/projectors.cu:79-183, that add a non-zero term).
-int64_t orc_count_taps(const orc_plan* P, const orc_vol* V, int v1_begin, int v1_end, int nthreads)
+int64_t orc_count_taps(const orc_plan* P, const orc_vol* V, int v1_begin, int v1_end, int f64, int nthreads)
 {
-    const int NV = (v1_end - v1_begin) * P->N_v0;
-    int64_t total = 0;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total)
-    for (int vv = 0; vv < NV; vv++) {
-        const int iv1 = v1_begin + vv / P->N_v0, iv0 = vv % P->N_v0;
-        int64_t n = 0;
-        for (int iu1 = 0; iu1 < P->N_u1; iu1++)
-            for (int iu0 = 0; iu0 < P->N_u0; iu0++)
-                if (!bin_masked(*P, iv1, iv0, iu1, iu0))
-                    enumerate_bin<float>(*P, *V, iv1, iv0, iu1, iu0, [&](int64_t, float w0, float w1) {
-                        n += (w0 * w1 != 0.0f) ? 1 : 0;
-                    });
-        total += n;
-    }
-    return total;
+    return f64 ? count_taps<double>(*P, *V, v1_begin, v1_end, nthreads)
+               : count_taps<float>(*P, *V, v1_begin, v1_end, nthreads);
 }
 
 int orc_bin_taps(const orc_plan* P, const orc_vol* V, int iv1, int iv0, int iu1, int iu0, int64_t* idx,

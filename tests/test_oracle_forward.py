@@ -149,3 +149,24 @@ def test_fp32_matches_fp64(oracle, psi):
     g32 = oracle.project([p], vol, f)[0]
     g64 = oracle.project([p], vol, f.astype(np.float64), dtype=np.float64)[0]
     assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 1e-6
+
+
+def test_tap_count_matches_the_per_bin_listing(oracle):
+    """count_taps (the work measure of one projection) in FP64 equals, row by row, the number of non-zero
+    weights the per-bin tap listing (FP64) returns for every unmasked bin.  The fp32 count is the number of
+    listed weights above 1e-6 of the row's largest: what FP64 leaves below that are rounding-level slivers
+    (voxel faces that coincide with footprint edges) that fp32 forms as exact zeros.  Both stay under the
+    4 N_v1 N_v0 V bound."""
+    p = oracle.planogram(0.0, D=100.0, L=32.0, H=32.0, T=2.0)
+    vol = oracle.volume(16, 16, 16, 2.0)
+    for iv1 in (0, p.N_v1 // 2):
+        ws = []
+        for iv0 in range(p.N_v0):
+            for iu1 in range(p.N_u1):
+                for iu0 in range(p.N_u0):
+                    ws.append(oracle.bin_taps(p, vol, iv1, iv0, iu1, iu0)[1])
+        w = np.concatenate(ws)
+        assert oracle.count_taps(p, vol, iv1, iv1 + 1, f64=True) == int(np.count_nonzero(w))
+        assert oracle.count_taps(p, vol, iv1, iv1 + 1) == int(np.count_nonzero(w > 1e-6 * w.max()))
+    bound = 4 * p.N_v1 * p.N_v0 * 16 ** 3
+    assert oracle.count_taps(p, vol) <= oracle.count_taps(p, vol, f64=True) <= bound
