@@ -5,7 +5,10 @@
 * closed form of a uniform volume: bins whose footprint stays inside the volume in every slice equal
   (128 slices x 2 mm) / sqrt(1 + v0^2 + v1^2);
 * linearity A(2a - 3b) = 2Aa - 3Ab;
-* the oracle agrees with the device result on a sampled v1 row (the full oracle projection takes minutes);
+* the oracle agrees with the device result on a sampled v1 row, forward and PETL-mode back (the full
+  oracle projection takes minutes);
+* list-mode binning of 2M coincidences between the benchmark panels, bit-exact against the oracle;
+* FBP linearity, FBP(2a - 3b) = 2 FBP(a) - 3 FBP(b);
 * MLEM: count preservation, non-negativity, non-decreasing log-likelihood, bf16 storage consistency;
 * RWLS: the first conjugate-gradient step in closed form and its exact line search; a decreasing cost.
 """
@@ -111,6 +114,62 @@ def test_sampled_rows_match_oracle(full, oracle):
         oracle.project_rows(p, vol, f, ref, 77, 78)
         got = gi[77].cpu().numpy()
         assert np.linalg.norm(got - ref[77]) / np.linalg.norm(ref[77]) < 1e-5
+
+
+def test_sampled_row_back_matches_oracle(full, oracle):
+    """PETL-mode back projection of planograms that are zero except one v1 row, against the oracle's back
+    projection of that row (both planograms summed)."""
+    s, shapes = full
+    s.set_backprojector_mode("petl")
+    rng = np.random.default_rng(61)
+    plans = [oracle.planogram(psi, 200.0, 256.0, 256.0, 2.0) for psi in (0.0, 90.0)]
+    vol = oracle.volume(128, 128, 128, 2.0)
+    ref = np.zeros(vol.shape, np.float32)
+    g = []
+    for p, sh in zip(plans, shapes):
+        host = np.zeros(sh, np.float32)
+        host[23] = rng.random(sh[1:], dtype=np.float32)
+        oracle.backproject_petl_rows(p, vol, host, ref, 23, 24)
+        g.append(torch.from_numpy(host).to(DEV))
+    got = back(s, g).cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_bin_events_full_size(full, oracle):
+    """2M coincidences between the benchmark's two panel pairs (y = +-100 mm and x = +-100 mm, 256 mm
+    wide, chest-wall z in [0, 256]), binned bilinearly on the device: bit-exact against the oracle."""
+    s, shapes = full
+    rng = np.random.default_rng(62)
+    n, half = 1 << 21, 1 << 20
+    def side(fixed, sign, m):
+        p = np.empty((m, 3), np.float32)
+        p[:, 1 - fixed] = rng.uniform(-128.0, 128.0, m)
+        p[:, fixed] = sign * 100.0
+        p[:, 2] = rng.uniform(0.0, 256.0, m)
+        return p
+    blue = np.vstack([side(1, 1, half), side(0, 1, n - half)])
+    pink = np.vstack([side(1, -1, half), side(0, -1, n - half)])
+    perm = rng.permutation(n)
+    blue, pink = blue[perm], pink[perm]
+    got, acc = s.bin_events(blue, pink)
+    plans = [oracle.planogram(psi, 200.0, 256.0, 256.0, 2.0) for psi in (0.0, 90.0)]
+    ref, racc = oracle.bin_events(plans, blue, pink)
+    assert acc == racc and min(acc) > 100000
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fbp_linearity_full_size(full):
+    s, shapes = full
+    gen = torch.Generator(device=DEV).manual_seed(63)
+    fa = torch.from_numpy(ellipsoid_phantom(128, 2.0, 20, seed=4)).to(DEV)
+    a = [x.cpu().numpy() for x in proj(s, shapes, fa)]
+    b = [torch.rand(sh, generator=gen, device=DEV).cpu().numpy() for sh in shapes]
+    ra = s.FBP([x.copy() for x in a])
+    rb = s.FBP([x.copy() for x in b])
+    rc = s.FBP([2.0 * x - 3.0 * y for x, y in zip(a, b)])
+    lin = 2.0 * ra.astype(np.float64) - 3.0 * rb.astype(np.float64)
+    assert np.linalg.norm(rc - lin) / np.linalg.norm(lin) < 1e-5
 
 
 @pytest.fixture(scope="module")
