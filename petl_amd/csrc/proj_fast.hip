@@ -463,7 +463,11 @@ struct BackArgs {
     float g_scale;  // storage scale of the planogram (16-bit formats; 1 for fp32)
 };
 
-template <int AG, int ST>
+// DMA (fp32 planograms, PETL mode): the next block of v0 rows goes from HBM straight into the other LDS
+// buffer (global_load_lds, one 64-bin row segment per wave instruction) while the current block
+// accumulates, and each view's scale is folded into its two interpolation weights.  Otherwise the block
+// is read through registers and scaled (and, in matched mode, masked per bin) on its way into LDS.
+template <int AG, int ST, bool DMA = false>
 __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
                                                       float* __restrict__ h2, BackArgs A,
                                                       const int2* __restrict__ xtab, int xrow,
@@ -492,31 +496,106 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
         sbuf[i] = 0.0f;
     __syncthreads();
 
+    // DMA form: bins [0, Nu0) of each row come from the planogram; the LDS positions around them keep the
+    // zeros written above (no lane writes there), and rows past Nv0 are never read
+    const int nseg = (Nu0 + 63) / 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthr >> 6, lane = tid & 63;
+    const int64_t g_row0 = (int64_t(iv1) * Nv0 * Nu1 + iu1) * Nu0;  // bin 0 of view 0 in this (v1, u1) row
+    auto dma = [&](int blk, int b) {
+        for (int t = wave; t < BACK_RB * nseg; t += nwaves) {
+            const int r = t / nseg, seg = t - r * nseg;
+            const int iv0 = blk * BACK_RB + r;
+            const int p = seg * 64 + lane;
+            if (iv0 < Nv0 && p < Nu0)
+                __builtin_amdgcn_global_load_lds(
+                    reinterpret_cast<const void*>(g + g_row0 + int64_t(iv0) * Nu1 * Nu0 + p),
+                    (__attribute__((address_space(3))) void*)(sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow +
+                                                               seg * 64),
+                    4, 0, 0);
+        }
+    };
+
+    // register form: elements idx = tid + k * nthr of the block's BACK_RB * Nu0 values, four at a time:
+    // the four loads (clamped in bounds) issue together, then the scaled, masked values go to LDS.
+    // Element -> (row, bin) by a float reciprocal: (idx + 0.5) / Nu0 lies at least 0.5 / Nu0 from an
+    // integer, far above the rounding error of the product.
+    const float inv_nu0 = 1.0f / float(Nu0);
     auto load = [&](int blk, int b) {
         float* sb = sbuf + b * BACK_RB * A.srow + A.spad_lo;
         const int n = BACK_RB * Nu0;
-        for (int idx = tid; idx < n; idx += nthr) {
-            const int r = idx / Nu0, p = idx - r * Nu0;
-            const int iv0 = blk * BACK_RB + r;
-            float val = 0.0f;
-            if (iv0 < Nv0) {
-                val = Stor<ST>::ld(g[((int64_t(iv1) * Nv0 + iv0) * Nu1 + iu1) * Nu0 + p], A.g_scale) *
-                      scale[int64_t(iv1) * Nv0 + iv0];
-                if (A.mode == BACK_MATCHED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
-                    val = 0.0f;
+        for (int idx0 = tid; idx0 < n; idx0 += 4 * nthr) {
+            typename Stor<ST>::T raw[4];
+            float sc[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = min(idx0 + u * nthr, n - 1);
+                const int r = int((float(idx) + 0.5f) * inv_nu0), p = idx - r * Nu0;
+                const int iv0 = min(blk * BACK_RB + r, Nv0 - 1);
+                raw[u] = g[g_row0 + int64_t(iv0) * Nu1 * Nu0 + p];
+                sc[u] = scale[int64_t(iv1) * Nv0 + iv0];
             }
-            sb[r * A.srow + p] = val;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = idx0 + u * nthr;
+                if (idx < n) {
+                    const int r = int((float(idx) + 0.5f) * inv_nu0), p = idx - r * Nu0;
+                    const int iv0 = blk * BACK_RB + r;
+                    float val = 0.0f;
+                    if (iv0 < Nv0) {
+                        val = Stor<ST>::ld(raw[u], A.g_scale) * sc[u];
+                        if (A.mode == BACK_MATCHED &&
+                            u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
+                            val = 0.0f;
+                    }
+                    sb[r * A.srow + p] = val;
+                }
+            }
         }
     };
 
     const int nblk = (Nv0 + BACK_RB - 1) / BACK_RB;
-    load(0, 0);
+    if (DMA)
+        dma(0, 0);
+    else
+        load(0, 0);
     __syncthreads();
     for (int blk = 0; blk < nblk; blk++) {
-        if (blk + 1 < nblk)
-            load(blk + 1, (blk + 1) & 1);
         const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
         const int row_off = (is % A.sb) * FWD_HP;
+        if (DMA) {
+            // this block's table entries first, waited for before the next block's DMA is issued: loads
+            // and LDS DMA retire through one in-order counter, and the accumulation must not wait for
+            // the DMA; it then runs on registers and LDS only
+            int2 eb[BACK_RB];
+#pragma unroll
+            for (int r = 0; r < BACK_RB; r++)
+                eb[r] = xtab[int64_t(min(blk * BACK_RB + r, Nv0 - 1)) * xrow + is];
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left open (gfx9 encoding)
+            if (blk + 1 < nblk)
+                dma(blk + 1, (blk + 1) & 1);
+#pragma unroll
+            for (int r = 0; r < BACK_RB; r++) {
+                const int iv0 = blk * BACK_RB + r;
+                if (iv0 < Nv0) {
+                    const int m = -(eb[r].x - row_off) - 1;
+                    const float sc = scale[int64_t(iv1) * Nv0 + iv0];
+                    const float wq = __int_as_float(eb[r].y) * sc, wp = (1.0f - __int_as_float(eb[r].y)) * sc;
+                    const float* s = sb + r * A.srow + m;
+                    float xp = s[0];
+#pragma unroll
+                    for (int a = 0; a < AG; a++) {
+                        const float x = s[a + 1];
+                        acc[a] += wp * xp;
+                        acc[a] += wq * x;
+                        xp = x;
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+        if (blk + 1 < nblk)
+            load(blk + 1, (blk + 1) & 1);
         // the next view's table entry is loaded while the current view's window runs
         int2 e_next = xtab[int64_t(min(blk * BACK_RB, Nv0 - 1)) * xrow + is];
 #pragma unroll 1
@@ -1046,8 +1125,17 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
     const int nthr = ((fp.Ns + 63) / 64) * 64;
     const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
-    hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
-                       static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+    bool dma = false;
+    if constexpr (ST == STORE_F32)
+        dma = (mode == BACK_PETL);
+    if (dma) {
+        if constexpr (ST == STORE_F32)
+            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds,
+                               s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+    } else {
+        hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
+                           static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+    }
     PETL_LAUNCH_CHECK();
     const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
     const dim3 zgrid((fp.Na + 63) / 64, fp.Ns, (fp.Nz + zc - 1) / zc);
