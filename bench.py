@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=12, help="v1 rows of each planogram in the CPU sample")
+    ap.add_argument("--cpu-rows", type=int, default=20, help="v1 rows of each planogram in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--mode", default="petl", choices=["petl", "matched"])
     ap.add_argument("--calib-copy", action="store_true",

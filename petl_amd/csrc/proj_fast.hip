@@ -463,11 +463,11 @@ struct BackArgs {
     float g_scale;  // storage scale of the planogram (16-bit formats; 1 for fp32)
 };
 
-// DMA (fp32 planograms, PETL mode): the next block of v0 rows goes from HBM straight into the other LDS
-// buffer (global_load_lds, one 64-bin row segment per wave instruction) while the current block
-// accumulates, and each view's scale is folded into its two interpolation weights.  Otherwise the block
-// is read through registers and scaled (and, in matched mode, masked per bin) on its way into LDS.
-template <int AG, int ST, bool DMA = false>
+// DMA (fp32 planograms): the next block of v0 rows goes from HBM straight into the other LDS buffer
+// (global_load_lds, one 64-bin row segment per wave instruction) while the current block accumulates,
+// and each view's scale is folded into its two interpolation weights.  Otherwise (16-bit planograms) the
+// block is read through registers and scaled (and, in matched mode, masked per bin) on its way into LDS.
+template <int AG, int ST, bool DMA = false, bool MASKED = false>
 __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
                                                       float* __restrict__ h2, BackArgs A,
                                                       const int2* __restrict__ xtab, int xrow,
@@ -497,7 +497,9 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     __syncthreads();
 
     // DMA form: bins [0, Nu0) of each row come from the planogram; the LDS positions around them keep the
-    // zeros written above (no lane writes there), and rows past Nv0 are never read
+    // zeros written above (no lane writes there), and rows past Nv0 are never read.  In matched mode the
+    // lanes of bins outside the view's u0 mask write a zero instead (they may hold a value from the view
+    // this buffer row had two blocks earlier); the DMA and the zeros reach disjoint positions.
     const int nseg = (Nu0 + 63) / 64;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthr >> 6, lane = tid & 63;
     const int64_t g_row0 = (int64_t(iv1) * Nv0 * Nu1 + iu1) * Nu0;  // bin 0 of view 0 in this (v1, u1) row
@@ -506,12 +508,15 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
             const int r = t / nseg, seg = t - r * nseg;
             const int iv0 = blk * BACK_RB + r;
             const int p = seg * 64 + lane;
-            if (iv0 < Nv0 && p < Nu0)
-                __builtin_amdgcn_global_load_lds(
-                    reinterpret_cast<const void*>(g + g_row0 + int64_t(iv0) * Nu1 * Nu0 + p),
-                    (__attribute__((address_space(3))) void*)(sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow +
-                                                               seg * 64),
-                    4, 0, 0);
+            if (iv0 < Nv0 && p < Nu0) {
+                float* row = sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow;
+                if (MASKED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
+                    row[p] = 0.0f;
+                else
+                    __builtin_amdgcn_global_load_lds(
+                        reinterpret_cast<const void*>(g + g_row0 + int64_t(iv0) * Nu1 * Nu0 + p),
+                        (__attribute__((address_space(3))) void*)(row + seg * 64), 4, 0, 0);
+            }
         }
     };
 
@@ -1127,11 +1132,18 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
     bool dma = false;
     if constexpr (ST == STORE_F32)
-        dma = (mode == BACK_PETL);
+        dma = true;
     if (dma) {
-        if constexpr (ST == STORE_F32)
-            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds,
-                               s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+        // MASKED: matched mode's per-bin u0 mask, a separate instance so that PETL mode carries none of it
+        if constexpr (ST == STORE_F32) {
+            if (mode == BACK_MATCHED)
+                hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true, true>), dim3(fp.nagroups, fp.P.Nu1, nv1),
+                                   dim3(nthr), lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow,
+                                   scale, rows);
+            else
+                hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr),
+                                   lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
+        }
     } else {
         hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
                            static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
