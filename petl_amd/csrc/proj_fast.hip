@@ -99,11 +99,12 @@ struct ZLoads {
     int dst[MAXE];
 };
 
+// zs: this v1 row's z table (one entry per slice) in LDS, so that the volume addresses do not wait for a
+// global load
 template <int SB, int R, int MAXE, int ST>
 __device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
-                                      const typename Stor<ST>::T* __restrict__ f,
-                                      const int4* __restrict__ ztab, int iv1, int iu1_0, int nrows, int is0,
-                                      int tid)
+                                      const typename Stor<ST>::T* __restrict__ f, const int4* zs, int iu1_0,
+                                      int nrows, int is0, int tid)
 {
     // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
     // the loads stay in flight until zstore consumes them
@@ -127,7 +128,7 @@ __device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
         const int is = is0 + sl;
         const bool live = is < A.Ns && r < nrows;
         const int isc = min(is, A.Ns - 1);
-        const int4 z = ztab[int64_t(iv1) * A.Ns + isc];
+        const int4 z = zs[isc];
         const int iza = iu1_0 + r - z.x - 1;
         const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
         const typename Stor<ST>::T* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
@@ -198,7 +199,8 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
 {
     using TS = typename Stor<SG>::T;
     constexpr bool RES = EK == EK_RESID, SQ = EK == EK_SUMSQ;
-    extern __shared__ float hbuf[];
+    static_assert((2 * R * SB) % 4 == 0, "the z table after the row buffers must be 16-byte aligned");
+    extern __shared__ __align__(16) float hbuf[];
     __shared__ int s_lo, s_hi;
     __shared__ double s_sq[FWD_WAVES];
     const int tid = threadIdx.x;
@@ -268,11 +270,14 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         s_hi = -1;
     }
     const int hbuf_elems = 2 * R * SB * FWD_HP;
+    // LDS after the two row buffers: this v1 row's z table (16-byte aligned: hbuf_elems is a multiple of 4)
+    int4* s_z = reinterpret_cast<int4*>(hbuf + hbuf_elems);
     for (int i = tid; i < hbuf_elems; i += FWD_THREADS)
         hbuf[i] = 0.0f;
     __syncthreads();
     for (int is = tid; is < Ns; is += FWD_THREADS) {
         const int4 e = ztab[int64_t(iv1) * Ns + is];
+        s_z[is] = e;
         bool nz = false;
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -308,13 +313,13 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         const int nblk = (is_hi - is_first) / SB + 1;
         constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
         ZLoads<MAXE, SF> L;
-        zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first, tid);
+        zload<SB, R>(L, A, f, s_z, iu1_0, nrows, is_first, tid);
         zstore(L, hbuf + A.hpad, A.f_scale);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
             const bool more = blk + 1 < nblk;
             if (more)
-                zload<SB, R>(L, A, f, ztab, iv1, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
+                zload<SB, R>(L, A, f, s_z, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
             const int is0 = is_first + blk * SB;
             const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
             // this block's table rows: [block][view] x {SB entries {(is % SB) * FWD_HP + k, wq}} and
@@ -1052,7 +1057,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, ek);
     if (!fn || fp.J != FWD_J || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
-    const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float);
+    const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float) + size_t(fp.Ns) * sizeof(int4);
     hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s,
                        static_cast<TG*>(g), static_cast<const TF*>(f), A, fp.dev.ftab, fp.dev.fwp, fp.dev.ztab,
                        fp.dev.fscale, epilogue, static_cast<const TG*>(y), resp);
