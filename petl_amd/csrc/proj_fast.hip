@@ -96,18 +96,19 @@ template <int MAXE, int ST>
 struct ZLoads {
     typename Stor<ST>::T a[MAXE], b[MAXE];
     float wa[MAXE], wb[MAXE];
-    int dst[MAXE];
 };
 
-// zs: this v1 row's z table (one entry per slice) in LDS, so that the volume addresses do not wait for a
-// global load
-template <int SB, int R, int MAXE, int ST>
-__device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
-                                      const typename Stor<ST>::T* __restrict__ f, const int4* zs, int iu1_0,
-                                      int nrows, int is0, int tid)
+// The thread's elements of a block, the same for every block: u1 row r, slice sl within the block,
+// column ia, and the LDS destination (r * SB + sl) * FWD_HP + ia, or -1 past the block's R * SB * Na
+// values (those elements load clamped addresses and are never stored).
+template <int MAXE>
+struct ZMap {
+    int r[MAXE], sl[MAXE], ia[MAXE], dst[MAXE];
+};
+
+template <int SB, int R, int MAXE>
+__device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int tid)
 {
-    // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
-    // the loads stay in flight until zstore consumes them
     const int per_row = SB * A.Na;
     const int n = R * per_row;
 #pragma unroll
@@ -124,14 +125,31 @@ __device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
             sl = rem / A.Na;
             ia = rem - sl * A.Na;
         }
-        L.dst[e] = (idx0 < n) ? (r * SB + sl) * FWD_HP + ia : -1;
-        const int is = is0 + sl;
-        const bool live = is < A.Ns && r < nrows;
+        M.r[e] = r;
+        M.sl[e] = sl;
+        M.ia[e] = ia;
+        M.dst[e] = (idx0 < n) ? (r * SB + sl) * FWD_HP + ia : -1;
+    }
+}
+
+// zs: this v1 row's z table (one entry per slice) in LDS, so that the volume addresses do not wait for a
+// global load
+template <int SB, int R, int MAXE, int ST>
+__device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const ZMap<MAXE>& M, const FwdArgs& A,
+                                      const typename Stor<ST>::T* __restrict__ f, const int4* zs, int iu1_0,
+                                      int nrows, int is0)
+{
+    // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
+    // the loads stay in flight until zstore consumes them
+#pragma unroll
+    for (int e = 0; e < MAXE; e++) {
+        const int is = is0 + M.sl[e];
+        const bool live = is < A.Ns && M.r[e] < nrows;
         const int isc = min(is, A.Ns - 1);
         const int4 z = zs[isc];
-        const int iza = iu1_0 + r - z.x - 1;
+        const int iza = iu1_0 + M.r[e] - z.x - 1;
         const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
-        const typename Stor<ST>::T* fp = f + int64_t(isc) * A.fs_s + int64_t(ia) * A.fs_a;
+        const typename Stor<ST>::T* fp = f + int64_t(isc) * A.fs_s + int64_t(M.ia[e]) * A.fs_a;
         L.a[e] = fp[int64_t(za) * A.fs_z];
         L.b[e] = fp[int64_t(zb) * A.fs_z];
         L.wa[e] = (live && iza >= 0 && iza < A.Nz) ? __int_as_float(z.y) : 0.0f;
@@ -140,12 +158,12 @@ __device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const FwdArgs& A,
 }
 
 template <int MAXE, int ST>
-__device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, float* hb, float f_scale)
+__device__ __forceinline__ void zstore(const ZLoads<MAXE, ST>& L, const ZMap<MAXE>& M, float* hb, float f_scale)
 {
 #pragma unroll
     for (int e = 0; e < MAXE; e++)
-        if (L.dst[e] >= 0)
-            hb[L.dst[e]] = L.wa[e] * Stor<ST>::ld(L.a[e], f_scale) + L.wb[e] * Stor<ST>::ld(L.b[e], f_scale);
+        if (M.dst[e] >= 0)
+            hb[M.dst[e]] = L.wa[e] * Stor<ST>::ld(L.a[e], f_scale) + L.wb[e] * Stor<ST>::ld(L.b[e], f_scale);
 }
 
 // Inputs that the forward epilogue reads per bin of one view row: data y (ratio and residual), weights
@@ -312,14 +330,16 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         const int is_first = (is_lo / SB) * SB;
         const int nblk = (is_hi - is_first) / SB + 1;
         constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
+        ZMap<MAXE> M;
+        zmap<SB, R>(M, A, tid);
         ZLoads<MAXE, SF> L;
-        zload<SB, R>(L, A, f, s_z, iu1_0, nrows, is_first, tid);
-        zstore(L, hbuf + A.hpad, A.f_scale);
+        zload<SB, R>(L, M, A, f, s_z, iu1_0, nrows, is_first);
+        zstore(L, M, hbuf + A.hpad, A.f_scale);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
             const bool more = blk + 1 < nblk;
             if (more)
-                zload<SB, R>(L, A, f, s_z, iu1_0, nrows, is_first + (blk + 1) * SB, tid);
+                zload<SB, R>(L, M, A, f, s_z, iu1_0, nrows, is_first + (blk + 1) * SB);
             const int is0 = is_first + blk * SB;
             const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
             // this block's table rows: [block][view] x {SB entries {(is % SB) * FWD_HP + k, wq}} and
@@ -369,7 +389,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                 }
             }
             if (more)
-                zstore(L, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad, A.f_scale);
+                zstore(L, M, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad, A.f_scale);
             __syncthreads();
         }
     }
