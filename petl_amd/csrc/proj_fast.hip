@@ -98,16 +98,19 @@ struct ZLoads {
     float wa[MAXE], wb[MAXE];
 };
 
-// The thread's elements of a block, the same for every block: u1 row r, slice sl within the block,
-// column ia, and the LDS destination (r * SB + sl) * FWD_HP + ia, or -1 past the block's R * SB * Na
-// values (those elements load clamped addresses and are never stored).
+// The thread's elements of a block, the same for every block: slice sl within the block, the column's
+// offset in the volume (ia * fs_a), the z-row base iu1_0 + r - 1 of its u1 row, whether that row is live,
+// and the LDS destination (r * SB + sl) * FWD_HP + ia, or -1 past the block's R * SB * Na values (those
+// elements load clamped addresses and are never stored).  Offsets are 32-bit: the fast path takes
+// volumes of fewer than 2^31 voxels.
 template <int MAXE>
 struct ZMap {
-    int r[MAXE], sl[MAXE], ia[MAXE], dst[MAXE];
+    int sl[MAXE], aoff[MAXE], zr[MAXE], dst[MAXE];
+    bool live[MAXE];
 };
 
 template <int SB, int R, int MAXE>
-__device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int tid)
+__device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int iu1_0, int nrows, int tid)
 {
     const int per_row = SB * A.Na;
     const int n = R * per_row;
@@ -125,9 +128,10 @@ __device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int tid)
             sl = rem / A.Na;
             ia = rem - sl * A.Na;
         }
-        M.r[e] = r;
         M.sl[e] = sl;
-        M.ia[e] = ia;
+        M.aoff[e] = ia * int(A.fs_a);
+        M.zr[e] = iu1_0 + r - 1;
+        M.live[e] = r < nrows;
         M.dst[e] = (idx0 < n) ? (r * SB + sl) * FWD_HP + ia : -1;
     }
 }
@@ -136,22 +140,22 @@ __device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int tid)
 // global load
 template <int SB, int R, int MAXE, int ST>
 __device__ __forceinline__ void zload(ZLoads<MAXE, ST>& L, const ZMap<MAXE>& M, const FwdArgs& A,
-                                      const typename Stor<ST>::T* __restrict__ f, const int4* zs, int iu1_0,
-                                      int nrows, int is0)
+                                      const typename Stor<ST>::T* __restrict__ f, const int4* zs, int is0)
 {
     // branch-free: every address is clamped into the volume and out-of-range taps get a zero weight, so
     // the loads stay in flight until zstore consumes them
+    const int fs_s = int(A.fs_s), fs_z = int(A.fs_z);
 #pragma unroll
     for (int e = 0; e < MAXE; e++) {
         const int is = is0 + M.sl[e];
-        const bool live = is < A.Ns && M.r[e] < nrows;
+        const bool live = is < A.Ns && M.live[e];
         const int isc = min(is, A.Ns - 1);
         const int4 z = zs[isc];
-        const int iza = iu1_0 + M.r[e] - z.x - 1;
+        const int iza = M.zr[e] - z.x;
         const int za = min(max(iza, 0), A.Nz - 1), zb = min(max(iza + 1, 0), A.Nz - 1);
-        const typename Stor<ST>::T* fp = f + int64_t(isc) * A.fs_s + int64_t(M.ia[e]) * A.fs_a;
-        L.a[e] = fp[int64_t(za) * A.fs_z];
-        L.b[e] = fp[int64_t(zb) * A.fs_z];
+        const int o = isc * fs_s + M.aoff[e];
+        L.a[e] = f[o + za * fs_z];
+        L.b[e] = f[o + zb * fs_z];
         L.wa[e] = (live && iza >= 0 && iza < A.Nz) ? __int_as_float(z.y) : 0.0f;
         L.wb[e] = (live && iza + 1 >= 0 && iza + 1 < A.Nz) ? __int_as_float(z.z) : 0.0f;
     }
@@ -331,15 +335,15 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         const int nblk = (is_hi - is_first) / SB + 1;
         constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
         ZMap<MAXE> M;
-        zmap<SB, R>(M, A, tid);
+        zmap<SB, R>(M, A, iu1_0, nrows, tid);
         ZLoads<MAXE, SF> L;
-        zload<SB, R>(L, M, A, f, s_z, iu1_0, nrows, is_first);
+        zload<SB, R>(L, M, A, f, s_z, is_first);
         zstore(L, M, hbuf + A.hpad, A.f_scale);
         __syncthreads();
         for (int blk = 0; blk < nblk; blk++) {
             const bool more = blk + 1 < nblk;
             if (more)
-                zload<SB, R>(L, M, A, f, s_z, iu1_0, nrows, is_first + (blk + 1) * SB);
+                zload<SB, R>(L, M, A, f, s_z, is_first + (blk + 1) * SB);
             const int is0 = is_first + blk * SB;
             const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
             // this block's table rows: [block][view] x {SB entries {(is % SB) * FWD_HP + k, wq}} and
@@ -874,6 +878,8 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->RB = BACK_RB;
     fp->ZC = 32;
     if (Ns > 256 || Na > 128)  // one lane per slice in back_h2_kernel; z-interpolated row length
+        return nullptr;
+    if (int64_t(V.Nx) * V.Ny * V.Nz >= (int64_t(1) << 31))  // the forward's 32-bit voxel offsets
         return nullptr;
 
     // ---- FP64 tables ----
