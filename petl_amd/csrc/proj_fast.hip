@@ -532,19 +532,24 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     const int nseg = (Nu0 + 63) / 64;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthr >> 6, lane = tid & 63;
     const int64_t g_row0 = (int64_t(iv1) * Nv0 * Nu1 + iu1) * Nu0;  // bin 0 of view 0 in this (v1, u1) row
+    // rows go to waves round robin (no index division in the issue loop), each row in 64-bin segments
     auto dma = [&](int blk, int b) {
-        for (int t = wave; t < BACK_RB * nseg; t += nwaves) {
-            const int r = t / nseg, seg = t - r * nseg;
+        for (int r = wave; r < BACK_RB; r += nwaves) {
             const int iv0 = blk * BACK_RB + r;
-            const int p = seg * 64 + lane;
-            if (iv0 < Nv0 && p < Nu0) {
-                float* row = sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow;
-                if (MASKED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
-                    row[p] = 0.0f;
-                else
-                    __builtin_amdgcn_global_load_lds(
-                        reinterpret_cast<const void*>(g + g_row0 + int64_t(iv0) * Nu1 * Nu0 + p),
-                        (__attribute__((address_space(3))) void*)(row + seg * 64), 4, 0, 0);
+            if (iv0 >= Nv0)
+                break;
+            float* row = sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow;
+            const typename Stor<ST>::T* src = g + g_row0 + int64_t(iv0) * Nu1 * Nu0;
+            for (int seg = 0; seg < nseg; seg++) {
+                const int p = seg * 64 + lane;
+                if (p < Nu0) {
+                    if (MASKED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
+                        row[p] = 0.0f;
+                    else
+                        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + p),
+                                                         (__attribute__((address_space(3))) void*)(row + seg * 64),
+                                                         4, 0, 0);
+                }
             }
         }
     };
