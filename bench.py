@@ -8,12 +8,20 @@ planograms, on buffers already resident in HBM.
 
 Prints one JSON line.  Besides the headline rate it carries:
   roofline      HBM roofline of the dominant kernel (the fast forward kernel), from HIP events around its
-                launches on the stream they run on; algorithmic bytes per launch = 4 * (voxels + bins)
-  cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference projectors) timed on a bounded
-                sample of the same workload on this host's cores
+                launches on the stream they run on; algorithmic bytes per launch = 4 * (voxels + bins);
+                traffic from the committed rocprofv3 counter table
+  valu_roofline the same kernel against FP32 vector arithmetic, the bound that actually limits it
+  storage_forms the same pair with bfloat16 / binary16 volume and planograms (fp32 arithmetic)
+  binning       list-mode binning of 16.8M device-resident coincidences into both planograms
+  batched_autograd  8 volumes of 64^3 through the autograd projector, forward + matched backward
+  response      scintillator stopping power of every bin through 256 box modules (ray tracing)
+  reconstruction  MLEM and RWLS per-iteration times and FBP, host data in and image out
+  small_config  10 MLEM iterations at 32x32-crystal panels and a 32^3 volume
   serving       reconstructions served per second and p50/p99 latency for a Poisson arrival stream of
                 MLEM requests of mixed panel/volume sizes, through the continuous-batching service and,
                 for comparison, one request at a time
+  cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference projectors) timed on a bounded
+                sample of the same workload on this host's cores, and on the small configuration
 """
 import argparse
 import json
