@@ -11,6 +11,7 @@ Prints one JSON line.  Besides the headline rate it carries:
                 launches on the stream they run on; algorithmic bytes per launch = 4 * (voxels + bins);
                 traffic from the committed rocprofv3 counter table
   valu_roofline the same kernel against FP32 vector arithmetic, the bound that actually limits it
+  lds_roofline  the same kernel's LDS read bandwidth (instruction count from the committed SQ counters)
   storage_forms the same pair with bfloat16 / binary16 volume and planograms (fp32 arithmetic)
   binning       list-mode binning of 16.8M device-resident coincidences into both planograms
   batched_autograd  8 volumes of 64^3 through the autograd projector, forward + matched backward
@@ -201,6 +202,31 @@ def measured_traffic(kernel):
     return None if entry is None else float(entry["traffic_bytes"])
 
 
+def lds_roofline(launch_ms):
+    insts = measured_counter("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>", "SQ_INSTS_LDS")
+    if insts is None:
+        return None
+    achieved = insts * 256.0 / (launch_ms * 1e-3) / 1e9
+    return {"kernel": "fwd_fast_kernel", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / LDS_PEAK_GBS, "lds_instructions_per_launch": insts,
+            "source": "profiles/counters_panel128.json (SQ_INSTS_LDS, rocprofv3 --pmc, --pair-only)"}
+
+
+def measured_counter(kernel, counter):
+    """Mean per dispatch of an SQ counter of `kernel` from the committed counter summary, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "counters_panel128.json")
+    try:
+        with open(path) as fh:
+            entry = json.load(fh)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if entry is None or counter not in entry else float(entry[counter])
+
+
+# LDS bandwidth of MI355X: 128 bytes per clock per CU, 256 CUs, 2.4 GHz
+LDS_PEAK_GBS = 128 * 256 * 2.4
+
+
 def main():
     a = parse()
     chosen = set(x.strip() for x in a.sections.split(",") if x.strip())
@@ -313,6 +339,9 @@ def main():
                               "by the same counters on a 256 MiB copy",
             "alg_bytes_per_launch": alg_bytes,
         },
+        # LDS: the forward's main loop reads one dword per lane per instruction (ds_read_b32, 256 bytes per
+        # wave instruction); instruction count per launch from the committed SQ_INSTS_LDS counter
+        "lds_roofline": lds_roofline(fwd_launch_ms),
         # the bound that actually limits this kernel: FP32 vector arithmetic (published peak 157.3 TFLOP/s
         # counts packed FP32, which this build does not emit; unpacked FP32 peaks at 78.6)
         "valu_roofline": {
