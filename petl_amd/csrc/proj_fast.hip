@@ -50,6 +50,7 @@ constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-inter
 constexpr int BACK_RB = 8;    // v0 rows per LDS block in the back projector
 constexpr int BACK_AG = 128;  // a-values per lane in the back projector
 constexpr int BACK_TW = 32;   // transpose tile width of the back epilogue
+constexpr int BACK_STW = 256; // dwords per staged row of 16-bit planogram bins (Nu0 <= 511)
 
 
 __device__ __forceinline__ float coordf(int i, float T, float s)
@@ -492,10 +493,12 @@ struct BackArgs {
     float g_scale;  // storage scale of the planogram (16-bit formats; 1 for fp32)
 };
 
-// DMA (fp32 planograms): the next block of v0 rows goes from HBM straight into the other LDS buffer
-// (global_load_lds, one 64-bin row segment per wave instruction) while the current block accumulates,
-// and each view's scale is folded into its two interpolation weights.  Otherwise (16-bit planograms) the
-// block is read through registers and scaled (and, in matched mode, masked per bin) on its way into LDS.
+// DMA: the next block of v0 rows goes from HBM straight into LDS (global_load_lds, one 64-dword row
+// segment per wave instruction) while the current block accumulates, and each view's scale is folded
+// into its two interpolation weights.  fp32 rows land in the other row buffer; 16-bit rows (an even
+// number of bins) land as raw dwords in a staging array and one pass per block widens them into the row
+// buffer.  Otherwise the block is read through registers and scaled (and, in matched mode, masked per
+// bin) on its way into LDS.
 template <int AG, int ST, bool DMA = false, bool MASKED = false>
 __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
                                                       float* __restrict__ h2, BackArgs A,
@@ -554,6 +557,47 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
         }
     };
 
+    // 16-bit DMA form: raw row dwords (two bins each) into the staging array, widened by convert()
+    const int nd = Nu0 / 2;  // dwords per row (Nu0 is even on this path)
+    auto dma16 = [&](int blk, uint32_t* stage) {
+        for (int r = wave; r < BACK_RB; r += nwaves) {
+            const int iv0 = blk * BACK_RB + r;
+            if (iv0 >= Nv0)
+                break;
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(g + g_row0 + int64_t(iv0) * Nu1 * Nu0);
+            for (int seg = 0; seg * 64 < nd; seg++) {
+                const int d = seg * 64 + lane;
+                if (d < nd)
+                    __builtin_amdgcn_global_load_lds(
+                        reinterpret_cast<const void*>(src + d),
+                        (__attribute__((address_space(3))) void*)(stage + r * BACK_STW + seg * 64), 4, 0, 0);
+            }
+        }
+    };
+    auto convert = [&](int blk, int b, const uint32_t* stage) {
+        using TS16 = typename Stor<ST>::T;
+        for (int r = 0; r < BACK_RB; r++) {
+            const int iv0 = blk * BACK_RB + r;
+            if (iv0 >= Nv0)
+                break;
+            float* row = sbuf + b * BACK_RB * A.srow + A.spad_lo + r * A.srow;
+            for (int d = tid; d < nd; d += nthr) {
+                const uint32_t w = stage[r * BACK_STW + d];
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const uint16_t bits = uint16_t(w >> (16 * k));
+                    TS16 raw;
+                    __builtin_memcpy(&raw, &bits, 2);
+                    const int p = 2 * d + k;
+                    float v = Stor<ST>::ld(raw, A.g_scale);
+                    if (MASKED && u0_masked(P, coordf(p, P.Tu0, P.u0_0), coordf(iv0, P.Tv0, P.v0_0)))
+                        v = 0.0f;
+                    row[p] = v;
+                }
+            }
+        }
+    };
+
     // register form: elements idx = tid + k * nthr of the block's BACK_RB * Nu0 values, four at a time:
     // the four loads (clamped in bounds) issue together, then the scaled, masked values go to LDS.
     // Element -> (row, bin) by a float reciprocal: (idx + 0.5) / Nu0 lies at least 0.5 / Nu0 from an
@@ -593,10 +637,19 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     };
 
     const int nblk = (Nv0 + BACK_RB - 1) / BACK_RB;
-    if (DMA)
+    // staging of the 16-bit DMA form: its own array, so that the compiler sees that the DMA into it does
+    // not alias the row buffer the accumulation reads
+    constexpr bool STAGED = DMA && ST != STORE_F32;
+    __shared__ __align__(16) uint32_t s_stage[STAGED ? BACK_RB * BACK_STW : 1];
+    if (STAGED) {
+        dma16(0, s_stage);
+        __syncthreads();
+        convert(0, 0, s_stage);
+    } else if (DMA) {
         dma(0, 0);
-    else
+    } else {
         load(0, 0);
+    }
     __syncthreads();
     for (int blk = 0; blk < nblk; blk++) {
         const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
@@ -610,8 +663,12 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
             for (int r = 0; r < BACK_RB; r++)
                 eb[r] = xtab[int64_t(min(blk * BACK_RB + r, Nv0 - 1)) * xrow + is];
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left open (gfx9 encoding)
-            if (blk + 1 < nblk)
-                dma(blk + 1, (blk + 1) & 1);
+            if (blk + 1 < nblk) {
+                if (STAGED)
+                    dma16(blk + 1, s_stage);
+                else
+                    dma(blk + 1, (blk + 1) & 1);
+            }
 #pragma unroll
             for (int r = 0; r < BACK_RB; r++) {
                 const int iv0 = blk * BACK_RB + r;
@@ -631,6 +688,10 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
                 }
             }
             __syncthreads();
+            if (STAGED && blk + 1 < nblk) {
+                convert(blk + 1, (blk + 1) & 1, s_stage);
+                __syncthreads();
+            }
             continue;
         }
         if (blk + 1 < nblk)
@@ -1166,20 +1227,19 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
     const int nthr = ((fp.Ns + 63) / 64) * 64;
     const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
-    bool dma = false;
-    if constexpr (ST == STORE_F32)
-        dma = true;
+    // the 16-bit DMA form moves row dwords: rows of an even number of bins (and so 4-byte aligned); its
+    // matched-mode instance needs registers for two waves per SIMD only, so that mode keeps the register form
+    const bool dma = (ST == STORE_F32) ||
+                     (mode == BACK_PETL && fp.P.Nu0 % 2 == 0 && fp.P.Nu0 <= 2 * BACK_STW);
     if (dma) {
         // MASKED: matched mode's per-bin u0 mask, a separate instance so that PETL mode carries none of it
-        if constexpr (ST == STORE_F32) {
-            if (mode == BACK_MATCHED)
-                hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true, true>), dim3(fp.nagroups, fp.P.Nu1, nv1),
-                                   dim3(nthr), lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow,
-                                   scale, rows);
-            else
-                hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr),
-                                   lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
-        }
+        if (mode == BACK_MATCHED)
+            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true, true>), dim3(fp.nagroups, fp.P.Nu1, nv1),
+                               dim3(nthr), lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow,
+                               scale, rows);
+        else
+            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr),
+                               lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     } else {
         hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
                            static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
