@@ -353,30 +353,45 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
             const int bi = is0 / SB;
             const int2* xt_b = ftab + (int64_t(bi) * Nv0 + iv0_base) * SB;
             const float* xp_b = fwp + (int64_t(bi) * Nv0 + iv0_base) * FWD_WPROW;
+            // views in pairs: both views' table rows are requested together, so a pair costs one wait
+            // for scalar loads instead of two (a wait on scalar loads is a full drain of the counter
+            // they share with the LDS reads)
 #pragma unroll
-            for (int j = 0; j < J; j++) {
-                const int iv0 = iv0_base + FWD_WAVES * j;
-                if (iv0 < Nv0) {
-                    const int2* xt = xt_b + FWD_WAVES * SB * j;
-                    const float* xp = xp_b + FWD_WAVES * FWD_WPROW * j;
-                    int off[SB];
-                    float wq[SB], wp[SB];
+            for (int jp = 0; jp < J; jp += 2) {
+                int off[2][SB];
+                float wq[2][SB], wp[2][SB];
+                int cmask[2];
 #pragma unroll
-                    for (int s = 0; s < SB; s++) {
-                        const int2 e = xt[s];
-                        off[s] = e.x;
-                        wq[s] = __int_as_float(e.y);
-                        wp[s] = xp[s];
+                for (int q = 0; q < 2; q++) {
+                    const int j = jp + q;
+                    const int iv0 = iv0_base + FWD_WAVES * j;
+                    cmask[q] = 0;
+                    if (j < J && iv0 < Nv0) {
+                        const int2* xt = xt_b + FWD_WAVES * SB * j;
+                        const float* xp = xp_b + FWD_WAVES * FWD_WPROW * j;
+#pragma unroll
+                        for (int s = 0; s < SB; s++) {
+                            const int2 e = xt[s];
+                            off[q][s] = e.x;
+                            wq[q][s] = __int_as_float(e.y);
+                            wp[q][s] = xp[s];
+                        }
+                        // bit c: some slice of the block reaches output chunk c
+                        cmask[q] = __float_as_int(xp[SB]);
                     }
-                    // bit c: some slice of the block reaches output chunk c
-                    const int cmask = __float_as_int(xp[SB]);
+                }
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const int j = jp + q;
+                    if (j >= J)
+                        continue;
                     const float* hp[SB];
 #pragma unroll
                     for (int s = 0; s < SB; s++)
-                        hp[s] = hb + off[s];
+                        hp[s] = hb + off[q][s];
 #pragma unroll
                     for (int c = 0; c < C; c++) {
-                        if (cmask & (1 << c)) {
+                        if (cmask[q] & (1 << c)) {
 #pragma unroll
                             for (int r = 0; r < R; r++) {
                                 float v[SB];
@@ -385,8 +400,8 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                                     v[s] = hp[s][r * SB * FWD_HP + 64 * c];
 #pragma unroll
                                 for (int s = 0; s < SB; s++) {
-                                    W[r][j][c] += wp[s] * v[s];
-                                    Q[r][j][c] += wq[s] * v[s];
+                                    W[r][j][c] += wp[q][s] * v[s];
+                                    Q[r][j][c] += wq[q][s] * v[s];
                                 }
                             }
                         }
