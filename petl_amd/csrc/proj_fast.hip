@@ -674,9 +674,12 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
             // and LDS DMA retire through one in-order counter, and the accumulation must not wait for
             // the DMA; it then runs on registers and LDS only
             int2 eb[BACK_RB];
+            float scb[BACK_RB];  // the rows' view scales, requested together (one scalar-load wait a block)
 #pragma unroll
-            for (int r = 0; r < BACK_RB; r++)
+            for (int r = 0; r < BACK_RB; r++) {
                 eb[r] = xtab[int64_t(min(blk * BACK_RB + r, Nv0 - 1)) * xrow + is];
+                scb[r] = scale[int64_t(iv1) * Nv0 + min(blk * BACK_RB + r, Nv0 - 1)];
+            }
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left open (gfx9 encoding)
             if (blk + 1 < nblk) {
                 if (STAGED)
@@ -689,7 +692,7 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
                 const int iv0 = blk * BACK_RB + r;
                 if (iv0 < Nv0) {
                     const int m = -(eb[r].x - row_off) - 1;
-                    const float sc = scale[int64_t(iv1) * Nv0 + iv0];
+                    const float sc = scb[r];
                     const float wq = __int_as_float(eb[r].y) * sc, wp = (1.0f - __int_as_float(eb[r].y)) * sc;
                     const float* s = sb + r * A.srow + m;
                     float xp = s[0];
