@@ -35,8 +35,8 @@ struct FastPlan {
     int64_t fs_s = 0, fs_a = 0, fs_z = 0;  // element strides of the slice / row / z axes in f[z][y][x]
     // forward kernel shape
     int J = 0, C = 0, R = 1, ngroups = 0, SB = 0, hpad = 0, hrow = 0, xrow = 0;
-    // back kernel shape
-    int AG = 0, nagroups = 0, RB = 0, spad_lo = 0, spad_hi = 0, srow = 0, ZC = 0;
+    // back kernel shape (a-groups of BACK_AG columns; the z chunk of back_z is chosen per launch)
+    int nagroups = 0, spad_lo = 0, srow = 0;
     FastTables dev;
     void* dev_block = nullptr;  // one allocation holding all tables
     ~FastPlan();

@@ -6,20 +6,21 @@
 //
 // Forward (fwd_fast_kernel), one workgroup per (u1 row, v1 view row, group of v0 views):
 //   1. For a block of SB slices, interpolate the volume along z onto this u1 row,
-//      H[s][a] = wa * f[iz_a][s][a] + wb * f[iz_a+1][s][a], into LDS (zero-padded rows).
+//      H[s][a] = wa * f[iz_a][s][a] + wb * f[iz_a+1][s][a], into LDS (zero-padded rows; the v1 row's z
+//      table sits in LDS, the next block's loads are in flight during the current block).
 //   2. Each wave owns up to J v0 views; lane p of chunk c owns output bin u0 = 64c + p.  With the slice's
 //      u0 interpolation out[p] = wq*H[p+k] + wp*H[p+k+1], every lane reads ONE value v = H[p+k] per
-//      (view, slice) and accumulates Q[p] += wq*v, P[p] += wp*v; at the end out[p] = Q[p] + P[p+1].
-//      wq, wp, k are wave-uniform (scalar loads from the FP64-built table), so the inner step is one
-//      ds_read_b32 and two v_fmac_f32 per bin per slice.  Chunks that cannot touch the volume's
-//      footprint for the whole slice block are skipped with a uniform branch.
+//      (view, slice) and accumulates Q[p] += wq*v, W[p] += wp*v; at the end out[p] = Q[p] + W[p+1].
+//      wq, wp, k are wave-uniform (scalar loads from the FP64-built table, two views at a time), so the
+//      inner step is one ds_read_b32 and two v_fmac_f32 per bin per slice.  Chunks that cannot touch the
+//      volume's footprint for the whole slice block are skipped with a uniform branch.
 //
 // Back projection (two kernels, no atomics):
 //   back_h2_kernel, one workgroup per (u1 row, v1 view row, group of a): the exact transpose of step 2.
-//      Lane = slice; the workgroup streams the v0 rows g[v1][:, u1, :] (scaled, masked) through LDS and
-//      each lane accumulates its slice's column H2[s][a] = sum_v0 wp*S[a+m] + wq*S[a+m+1] in registers
-//      (sliding window: one LDS read and two FMAs per (view, voxel)).  The slab is written transposed
-//      through LDS to workspace[v1][u1][s][a].
+//      Lane = slice; the workgroup streams the v0 rows g[v1][:, u1, :] through LDS (by LDS DMA, one block
+//      of rows ahead) and each lane accumulates its slice's column H2[s][a] = sum_v0 wp*S[a+m] +
+//      wq*S[a+m+1] in registers (sliding window: one LDS read and two FMAs per (view, voxel)).  The slab
+//      is written transposed through LDS to workspace[v1][u1][s][a].
 //   back_z_kernel, one wave per (a chunk, slice, z chunk): the transpose of step 1, gathering the two
 //      u1 rows of every v1 for each voxel, f[iz][s][a] += sum_v1 wb*H2[iz+m1] + wa*H2[iz+m1+1].  In PETL
 //      mode the reference's voxel-centre u1 test (src/projectors.cu:284-289) is applied per voxel here.
@@ -957,10 +958,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->J = FWD_J;
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
     fp->SB = 8;
-    fp->AG = BACK_AG;
     fp->nagroups = (Na + BACK_AG - 1) / BACK_AG;
-    fp->RB = BACK_RB;
-    fp->ZC = 32;
     if (Ns > 256 || Na > 128)  // one lane per slice in back_h2_kernel; z-interpolated row length
         return nullptr;
     if (int64_t(V.Nx) * V.Ny * V.Nz >= (int64_t(1) << 31))  // the forward's 32-bit voxel offsets
