@@ -19,10 +19,14 @@ CASES = {
     "psi90": ((90.0,), 60.0, 32.0, 32.0, 2.0, (12, 12, 12), 2.0, 2.0, (0.0, 0.0, 0.0)),
     "pair_offsets": ((0.0, 90.0), 80.0, 48.0, 40.0, 2.0, (14, 10, 9), 2.0, 2.0, (2.0, -3.0, 1.0)),
     "panel32": ((0.0,), 100.0, 64.0, 64.0, 2.0, (32, 32, 32), 2.0, 2.0, (0.0, 0.0, 0.0)),
+    # odd crystal counts (15 x 11), an odd number of views (37 v0, 9 v1), odd volume sides, offsets on every axis
+    "ragged": ((0.0, 90.0), 72.0, 30.0, 22.0, 2.0, (13, 11, 7), 2.0, 2.0, (1.0, -2.0, 3.0)),
+    # one v1 row, two u1 rows, fewer z slices than the back projector's shortest z chunk (4)
+    "thin": ((0.0,), 20.0, 4.0, 6.0, 2.0, (5, 4, 3), 2.0, 2.0, (0.0, 0.0, 0.0)),
     "psi30_generic": ((30.0,), 60.0, 32.0, 32.0, 2.0, (12, 12, 12), 2.0, 2.0, (0.0, 0.0, 0.0)),
     "pitch_generic": ((0.0,), 60.0, 32.0, 32.0, 2.0, (16, 14, 12), 1.5, 2.5, (0.0, 0.0, 0.0)),
 }
-FAST_CASES = ["psi0", "psi90", "pair_offsets", "panel32"]
+FAST_CASES = ["psi0", "psi90", "pair_offsets", "panel32", "ragged", "thin"]
 
 
 def make_pair(oracle, case):
@@ -73,7 +77,8 @@ def test_forward_sparse_support_exact_zeros(oracle, case):
     rng = np.random.default_rng(9)
     f = np.zeros(vol.shape, np.float32)
     nz, ny, nx = vol.shape
-    f[nz // 3: nz // 3 + 3, ny // 4: ny // 4 + 4, nx // 2: nx // 2 + 3] = 1.0 + rng.random((3, 4, 3))
+    block = f[nz // 3: nz // 3 + 3, ny // 4: ny // 4 + 4, nx // 2: nx // 2 + 3]  # clipped at small volumes
+    block[...] = 1.0 + rng.random(block.shape)
     f[-1, -1, -1] = 5.0
     g = s.allocate_projections()
     s.project(g, f)
