@@ -116,6 +116,29 @@ def test_rwls_matches_oracle(oracle, case):
     assert rel(got, ref) <= tol, rel(got, ref)
 
 
+def test_rwls_ragged_geometry_matches_oracle(oracle):
+    """The default weighted loop with SQS and TV at odd crystal counts (15 x 11), odd view counts (37 v0,
+    9 v1) and an odd-sided volume off centre on every axis."""
+    psis, D, L, H, T = (0.0, 90.0), 72.0, 30.0, 22.0, 2.0
+    s = PETLserver()
+    s.print_warnings = False
+    for psi in psis:
+        s.add_planogram(psi, D, L, H, T)
+    s.set_volume(13, 11, 7, 2.0, 2.0, 1.0, -2.0, 3.0)
+    assert all(s.uses_fast_path(i) for i in range(2))
+    plans = [oracle.planogram(psi, D, L, H, T) for psi in psis]
+    vol = oracle.volume(13, 11, 7, 2.0, 2.0, 1.0, -2.0, 3.0)
+    rng = np.random.default_rng(72)
+    g = oracle.project(plans, vol, (0.5 + rng.random(vol.shape)).astype(np.float32))
+    tot = sum(float(x.sum()) for x in g)
+    y = [rng.poisson(x * (2e6 / tot)).astype(np.float32) for x in g]
+    kw, okw, tol = CASES["rwls_default_sqs_tv"]
+    got = s.RWLS(y, s.allocate_volume(), 6, **kw)
+    ref, done = oracle.rwls(plans, vol, y, 6, **okw)
+    assert done == 6
+    assert rel(got, ref) <= tol, rel(got, ref)
+
+
 def test_wls_weights_and_matched_mode(oracle):
     s, plans, vol, y = problem(oracle, psis=(0.0,))
     rng = np.random.default_rng(71)
