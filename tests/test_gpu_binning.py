@@ -123,3 +123,30 @@ def test_bin_events_gpu_matches_host_path(oracle):
     s.bin_events_gpu(torch.from_numpy(blue).to("cuda:0"), torch.from_numpy(pink).to("cuda:0"), g)
     for a, b in zip(g, host):
         np.testing.assert_array_equal(a.cpu().numpy(), b)
+
+
+@pytest.mark.parametrize("nearest", [False, True])
+def test_bin_events_ragged_geometry(oracle, nearest):
+    """Odd crystal counts (15 x 11, so L != H), odd view counts (37 v0, 9 v1), panels 72 mm apart:
+    bit-exact against the oracle."""
+    psis, D, L, H, T = (0.0, 90.0), 72.0, 30.0, 22.0, 2.0
+    s = PETLserver()
+    s.print_warnings = False
+    for psi in psis:
+        s.add_planogram(psi, D, L, H, T)
+    s.set_volume(13, 11, 7, 2.0, 2.0, 1.0, -2.0, 3.0)
+    plans = [oracle.planogram(psi, D, L, H, T) for psi in psis]
+    rng = np.random.default_rng(19)
+    n, R = 100000, D / 2
+    half = n // 2
+    a = np.column_stack([rng.uniform(-L / 2, L / 2, half), np.full(half, R), rng.uniform(0, H, half)])
+    b = np.column_stack([rng.uniform(-L / 2, L / 2, half), np.full(half, -R), rng.uniform(0, H, half)])
+    c = np.column_stack([np.full(n - half, R), rng.uniform(-L / 2, L / 2, n - half), rng.uniform(0, H, n - half)])
+    d = np.column_stack([np.full(n - half, -R), rng.uniform(-L / 2, L / 2, n - half), rng.uniform(0, H, n - half)])
+    blue = np.vstack([a, c]).astype(np.float32)
+    pink = np.vstack([b, d]).astype(np.float32)
+    got, acc = s.bin_events(blue, pink, nearest=nearest)
+    ref, racc = oracle.bin_events(plans, blue, pink, nearest=nearest, chest_wall=True)
+    assert acc == racc and min(acc) > 1000
+    for x, y in zip(got, ref):
+        np.testing.assert_array_equal(x, y)
