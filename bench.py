@@ -10,7 +10,9 @@ Prints one JSON line.  Besides the headline rate it carries:
   roofline      HBM roofline of the dominant kernel (the fast forward kernel), from HIP events around its
                 launches on the stream they run on; algorithmic bytes per launch = 4 * (voxels + bins);
                 traffic from the committed rocprofv3 counter table
-  valu_roofline the same kernel against FP32 vector arithmetic, the bound that actually limits it
+  valu_roofline the same kernel against FP32 vector arithmetic, the bound that actually limits it: a
+                dense-equivalent flop rate, and the executed VALU instruction issue rate from the
+                committed SQ counters
   lds_roofline  the same kernel's LDS read bandwidth (instruction count from the committed SQ counters)
   storage_forms the same pair with bfloat16 / binary16 volume and planograms (fp32 arithmetic)
   binning       list-mode binning of 16.8M device-resident coincidences into both planograms
@@ -225,6 +227,20 @@ def measured_counter(kernel, counter):
 
 # LDS bandwidth of MI355X: 128 bytes per clock per CU, 256 CUs, 2.4 GHz
 LDS_PEAK_GBS = 128 * 256 * 2.4
+# VALU issue of MI355X: 256 CUs x 4 SIMDs of 16 lanes at 2.4 GHz, a wave64 instruction every 4 clocks per SIMD
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def valu_issue(launch_ms):
+    """Executed VALU wave instructions per second of the fast forward kernel (count per launch from the
+    committed SQ_INSTS_VALU counter) against the issue peak."""
+    insts = measured_counter("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>", "SQ_INSTS_VALU")
+    if insts is None:
+        return None
+    rate = insts / (launch_ms * 1e-3)
+    return {"achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instructions/s",
+            "frac": rate / VALU_ISSUE_PEAK, "valu_instructions_per_launch": insts,
+            "source": "profiles/counters_panel128.json (SQ_INSTS_VALU, rocprofv3 --pmc, --pair-only)"}
 
 
 def main():
@@ -351,6 +367,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": flop_per_launch / (fwd_launch_ms * 1e-3) / 1e12 / 157.3,
             "flop_per_launch": flop_per_launch,
+            # flop_per_launch counts every (bin, slice) pair; the kernel skips slice chunks outside each
+            # view's support, so "achieved" is a dense-equivalent rate.  "issue" is what it executes.
+            "issue": valu_issue(fwd_launch_ms),
         },
     }
     # the same pair with the volume and planograms stored in bfloat16 / binary16 (fp32 arithmetic, fp32

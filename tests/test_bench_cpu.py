@@ -24,6 +24,10 @@ def test_committed_traffic_and_counters_cover_the_forward_kernel():
     lds = bench.lds_roofline(2.5)
     assert lds is not None and 0.0 < lds["frac"] < 1.0
     assert abs(lds["achieved"] - insts * 256.0 / 2.5e-3 / 1e9) < 1e-6 * lds["achieved"]
+    valu = bench.measured_counter(name, "SQ_INSTS_VALU")
+    issue = bench.valu_issue(2.5)
+    assert issue is not None and 0.0 < issue["frac"] < 1.0
+    assert abs(issue["achieved"] - valu / 2.5e-3 / 1e9) < 1e-6 * issue["achieved"]
 
 
 def test_unknown_kernels_give_no_figures():
