@@ -270,22 +270,54 @@ def rdivide(denom, num):
     return divide(num, denom)
 
 
+def _loglik_terms(q, y, rows=8):
+    """FP64 sum of y log q - q over the bins with q > 0, by chunks of v1 rows (bounded temporaries)."""
+    tot = 0.0
+    for a in range(0, q.shape[0], rows):
+        qa = np.asarray(q[a:a + rows], np.float64)
+        ya = np.asarray(y[a:a + rows], np.float64)
+        pos = qa > 0
+        tot += float(np.sum(ya[pos] * np.log(qa[pos]) - qa[pos]))
+    return tot
+
+
 def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0, dtype=np.float32,
-         mode="petl", callback=None, nthreads=None):
-    """PETLserver.MLEM restated on the oracle projectors.  prior 'rd' follows the reference
-    (relative differences, OSL); prior 'tv' uses the p-Huber TV gradient in the same OSL slot."""
+         mode="petl", callback=None, nthreads=None, loglik=None, keep_projections=True):
+    """PETLserver.MLEM (src/petl_server.py:1534-1606) restated on the oracle projectors.  prior 'rd'
+    follows the reference (relative differences, OSL); prior 'tv' uses the p-Huber TV gradient in the same
+    OSL slot.  Planograms are processed one at a time (one planogram-sized temporary at a time), summing
+    their back projections in planogram order, which is the order backproject() sums them in.  loglik: a
+    list that receives, per iteration, the FP64 Poisson log-likelihood sum(y log Af - Af) over bins with
+    Af > 0 of the image the iteration starts from.  callback(n, f, Pd) runs after every iteration; Pd is
+    the list of that iteration's projections, or None when keep_projections is False."""
     if f is None or not np.any(f):
         f = np.ones(vol.shape, dtype=dtype)
     else:
         f = np.array(f, dtype=dtype)
         f[f < 0] = 0
-    ones = [np.ones(pl.shape, dtype=dtype) for pl in plans]
-    S = backproject(plans, vol, ones, mode=mode, dtype=dtype, nthreads=nthreads)
+    S = np.zeros(vol.shape, dtype=dtype)
+    for pl in plans:
+        S += backproject([pl], vol, [np.ones(pl.shape, dtype=dtype)], mode=mode, dtype=dtype, nthreads=nthreads)
     S[S == 0] = 1
     for n in range(num_iter):
-        Pd = project(plans, vol, f, dtype=dtype, nthreads=nthreads)
-        r = [rdivide(a, np.asarray(b, dtype=dtype)) for a, b in zip(Pd, g)]
-        d = backproject(plans, vol, r, mode=mode, dtype=dtype, nthreads=nthreads)
+        d = np.zeros(vol.shape, dtype=dtype)
+        Pd = [] if (callback is not None and keep_projections) else None
+        ll = 0.0
+        for pl, gi in zip(plans, g):
+            q = project([pl], vol, f, dtype=dtype, nthreads=nthreads)[0]
+            if loglik is not None:
+                ll += _loglik_terms(q, gi)
+            if Pd is not None:
+                Pd.append(q.copy())
+            # dataCube RDIV, in place: q <- y / q, 0 where q == 0 (src/data_cube.cpp:150-155)
+            nz = q != 0
+            q[nz] = np.asarray(np.asarray(gi)[nz], dtype=dtype) / q[nz]
+            q[~nz] = 0
+            del nz
+            d += backproject([pl], vol, [q], mode=mode, dtype=dtype, nthreads=nthreads)
+            del q
+        if loglik is not None:
+            loglik.append(ll)
         if beta <= 0.0:
             f = f * divide(d, S)
         else:
@@ -298,6 +330,39 @@ def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0
             f = f * divide(d, G)
         if callback is not None:
             callback(n, f, Pd)
+    return f
+
+
+def row_subset(g, first, stride):
+    """A copy of planogram g that keeps the v1 rows first, first + stride, ... and zeroes the others."""
+    out = np.zeros_like(g)
+    out[first::stride] = g[first::stride]
+    return out
+
+
+def osem_composed(plans, vol, g, num_iter, num_subsets, f0=None, dtype=np.float32, mode="petl", nthreads=None,
+                  callback=None):
+    """OSEM built from the full-planogram oracle projectors: subset b's forward is the full forward restricted
+    to its v1 rows, and its back projection is the back projection of a planogram that is zero outside its
+    rows (the projectors are linear, and a zero bin adds nothing).  Independent of the row-range entry points
+    osem() uses, and available in FP64 and in matched mode, where the EM identities hold exactly.
+    callback(sub, f_before, f_after, S_sub, ratio_list) runs after every subset update."""
+    f = np.ones(vol.shape, dtype) if f0 is None else np.array(f0, dtype=dtype)
+    S = []
+    for sub in range(num_subsets):
+        Ss = backproject(plans, vol, [row_subset(np.ones(p.shape, dtype), sub, num_subsets) for p in plans],
+                         mode=mode, dtype=dtype, nthreads=nthreads)
+        Ss[Ss == 0] = 1
+        S.append(Ss)
+    for _ in range(num_iter):
+        for sub in range(num_subsets):
+            q = project(plans, vol, f, dtype=dtype, nthreads=nthreads)
+            r = [row_subset(rdivide(qi, np.asarray(gi, dtype)), sub, num_subsets) for qi, gi in zip(q, g)]
+            d = backproject(plans, vol, r, mode=mode, dtype=dtype, nthreads=nthreads)
+            f_next = f * divide(d, S[sub])
+            if callback is not None:
+                callback(sub, f, f_next, S[sub], [row_subset(qi, sub, num_subsets) for qi in q])
+            f = f_next
     return f
 
 

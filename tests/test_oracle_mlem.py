@@ -5,7 +5,9 @@ algorithm, using the exactly-adjoint (matched) back projector where the invarian
 * count preservation: sum_j S_j f_j^(n+1) = sum_i y_i over bins with (A f^n)_i > 0, S = A^T 1;
 * the Poisson log-likelihood sum(y log Af - Af) is non-decreasing;
 * noise-free consistent data y = A f is a fixed point when started at f;
-* with the relative-difference prior the OSL update equals the hand-written update formula.
+* with the relative-difference prior (one-step-late): a constant image takes the plain step, a hot voxel
+  is pulled down and its neighbours up, and the 0.1 S clamp of the denominator acts where S + grad R
+  falls below it.
 """
 import numpy as np
 import pytest
@@ -68,15 +70,45 @@ def test_initial_estimate_rules(oracle):
     assert fn[0, 0, 0] == 0.0
 
 
-def test_osl_prior_update_formula(oracle):
+def test_osl_constant_image_is_the_plain_step(oracle):
+    """A constant image has a zero relative-difference gradient, so the OSL step is the beta = 0 step."""
     plans, vol, _, y = setup(oracle, psis=(0.0,))
-    f = np.random.default_rng(31).random(vol.shape) + 0.5
-    beta, delta = 0.05, 2.0
-    got = oracle.mlem(plans, vol, y, 1, f=f, delta=delta, beta=beta, dtype=np.float64)
+    f = np.full(vol.shape, 1.7)
+    a = oracle.mlem(plans, vol, y, 1, f=f, delta=2.0, beta=0.5, dtype=np.float64)
+    b = oracle.mlem(plans, vol, y, 1, f=f, beta=0.0, dtype=np.float64)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_osl_hot_voxel_is_pulled_down(oracle):
+    """The prior penalises differences: a hot voxel's update ends below its beta = 0 update and its face
+    neighbours end above theirs (the sign of the gradient in the OSL denominator)."""
+    plans, vol, _, y = setup(oracle, psis=(0.0,))
+    f = np.ones(vol.shape)
+    f[5, 5, 5] = 6.0
+    a = oracle.mlem(plans, vol, y, 1, f=f, delta=2.0, beta=0.05, dtype=np.float64)
+    b = oracle.mlem(plans, vol, y, 1, f=f, beta=0.0, dtype=np.float64)
+    assert a[5, 5, 5] < b[5, 5, 5]
+    for nb in ((4, 5, 5), (6, 5, 5), (5, 4, 5), (5, 6, 5), (5, 5, 4), (5, 5, 6)):
+        assert a[nb] > b[nb], nb
+    far = np.ones(vol.shape, bool)
+    far[3:8, 3:8, 3:8] = False
+    np.testing.assert_array_equal(a[far], b[far])  # the gradient is zero two voxels away
+
+
+def test_osl_denominator_clamp(oracle):
+    """With a beta large enough that S + grad R < 0.1 S next to a hot voxel, the denominator there is
+    0.1 S: those voxels take exactly ten times the beta = 0 update factor."""
+    plans, vol, _, y = setup(oracle, psis=(0.0,))
+    f = np.ones(vol.shape)
+    f[5, 5, 5] = 50.0
+    beta = 1.0e4
+    a = oracle.mlem(plans, vol, y, 1, f=f, delta=2.0, beta=beta, dtype=np.float64)
+    b = oracle.mlem(plans, vol, y, 1, f=f, beta=0.0, dtype=np.float64)
     S = oracle.backproject(plans, vol, [np.ones(p.shape) for p in plans], dtype=np.float64)
     S[S == 0] = 1
-    Pd = oracle.project(plans, vol, f, dtype=np.float64)
-    d = oracle.backproject(plans, vol, [np.where(a == 0, 0, yy / np.where(a == 0, 1, a)) for a, yy in zip(Pd, y)],
-                           dtype=np.float64)
-    G = np.maximum(0.1 * S, oracle.rd_gradient(f, delta, beta, dtype=np.float64) + S)
-    np.testing.assert_allclose(got, f * d / G, rtol=1e-12)
+    G = oracle.rd_gradient(f, 2.0, beta, dtype=np.float64)
+    clamped = S + G < 0.1 * S
+    assert clamped.sum() >= 6 and not clamped[5, 5, 5]
+    np.testing.assert_allclose(a[clamped], 10.0 * b[clamped], rtol=1e-12)
+    free = ~clamped & (G == 0)
+    np.testing.assert_array_equal(a[free], b[free])

@@ -242,3 +242,96 @@ def test_rwls_default_weights(system):
     for wi, gi in zip(W, g):
         np.testing.assert_allclose(wi, 1.0 / np.maximum(orc.gaussian_filter(gi, 3.0), 7.0), rtol=0)
         assert np.all(wi <= 1.0 / 7.0)
+
+
+# ------------------------------------------------------------------------------------------------
+# the TV terms of the RWLS loop in the Huber quadratic regime
+# ------------------------------------------------------------------------------------------------
+def _shift_operators(shape):
+    """The 26 neighbour shifts of a [N1][N2][N3] grid as dense matrices S_o (row c picks the voxel at
+    c + o with every index clamped into the grid) with weights 1 / |o| (1, 1/sqrt 2, 1/sqrt 3)."""
+    n = int(np.prod(shape))
+    idx = np.arange(n).reshape(shape)
+    ops = []
+    for o in np.ndindex(3, 3, 3):
+        o = np.array(o) - 1
+        if not o.any():
+            continue
+        src = [np.clip(np.arange(shape[a]) + o[a], 0, shape[a] - 1) for a in range(3)]
+        nb = idx[np.ix_(*src)].ravel()
+        S = np.zeros((n, n))
+        S[np.arange(n), nb] = 1.0
+        ops.append((1.0 / np.sqrt(float(o @ o)), S))
+    return ops
+
+
+def test_tv_quadratic_regime_is_weighted_laplacian():
+    """With delta above every neighbour difference h'(t) = t and h''(t) = 1, so the TV gradient is
+    beta * sum_o w_o (I - S_o) f and the quadratic form is beta * sum_o w_o |(I - S_o) d|^2, whatever f."""
+    shape = (3, 4, 5)
+    f, d = RNG.random(shape), RNG.standard_normal(shape)
+    beta, delta = 0.75, 10.0
+    n = f.size
+    L = sum(w * (np.eye(n) - S) for w, S in _shift_operators(shape))
+    # the reference keeps the diagonal weights as float constants (relative rounding <= 6e-8)
+    want = beta * L @ f.ravel()
+    np.testing.assert_allclose(orc.tv_gradient(f, delta, beta, dtype=np.float64).ravel(), want,
+                               rtol=2e-7, atol=2e-7 * np.abs(want).max())
+    q = sum(w * float(np.sum(((np.eye(n) - S) @ d.ravel()) ** 2)) for w, S in _shift_operators(shape))
+    assert orc.tv_quadform(f, d, delta, beta) == pytest.approx(beta * q, rel=2e-7)
+
+
+def test_rwls_tv_quadratic_regime_iterates(system):
+    """RWLS with the TV prior in the quadratic regime, against the same Polak-Ribiere iteration written
+    with dense matrices: gradient A^T W (A f - g) + beta L f and step <d, grad> / (<Ad, Ad>_W + beta
+    sum_o w_o |(I - S_o) d|^2), with L and S_o built from the neighbour definition above.  Without the prior
+    the loop is pinned against scipy's CG; this pins how the loop adds the prior's gradient (at f, with a
+    plus sign) and its curvature along d (the reference's step uses h'' with the pair counted from both
+    voxels, twice the curvature of the gradient's own quadratic, so this is not CG on one quadratic)."""
+    plans, vol, A = system
+    rng = np.random.default_rng(8)
+    nvox = A.shape[1]
+    seen = A.sum(axis=0) > 0
+    x_true = np.where(seen, 4.0 + rng.random(nvox), 0.0)
+    y = A @ x_true
+    w = rng.uniform(0.2, 2.0, A.shape[0])
+    beta, delta = 0.25, 1.0e3
+    ops = _shift_operators(vol.shape)
+    L = sum(wo * (np.eye(nvox) - S) for wo, S in ops)
+    Dq = sum(wo * (np.eye(nvox) - S).T @ (np.eye(nvox) - S) for wo, S in ops)
+    q = A.T @ (w * (A @ np.ones(nvox)))
+    M = np.where(q == 0, 1.0, 1.0 / np.where(q == 0, 1.0, q))
+    # the loop of src/petl_server.py:1786-1830 on dense matrices (no iterate is clipped here)
+    f = np.zeros(nvox)
+    r = A @ f - y
+    d = np.zeros(nvox)
+    grad_old = np.zeros(nvox)
+    gg_old = 0.0
+    want = []
+    for n in range(4):
+        grad = A.T @ (w * r) + beta * L @ f
+        u = M * grad
+        if n == 0:
+            d = u.copy()
+        else:
+            gamma = (u @ grad - u @ grad_old) / gg_old
+            d = u + gamma * d
+            if d @ grad <= 0.0:
+                d = u.copy()
+        gg_old = u @ grad
+        grad_old = grad.copy()
+        Ad = A @ d
+        step = (d @ grad) / (Ad @ (w * Ad) + beta * d @ Dq @ d)
+        f = f - step * d
+        assert np.all(f >= 0)
+        r = A @ f - y
+        want.append(f.copy())
+    got = []
+    orc.rwls(plans, vol, _split(plans, y), 4, delta=delta, beta=beta, W=_split(plans, w), sqs=True, nonneg=True,
+             mode="matched", callback=lambda n, fi: got.append(fi.ravel().copy()))
+    assert len(got) == 4
+    for a, e in zip(got, want):
+        np.testing.assert_allclose(a, e, rtol=1e-6, atol=1e-6 * np.abs(e).max())
+    # the prior changes the iterates (the check above is not the beta = 0 path)
+    f0, _ = orc.rwls(plans, vol, _split(plans, y), 4, W=_split(plans, w), sqs=True, mode="matched")
+    assert np.max(np.abs(f0.ravel() - want[-1])) > 1e-3 * np.abs(want[-1]).max()
