@@ -180,6 +180,14 @@ PETL_EXTERN bool project_gpu_ex(void** d_g, const void* d_f, int volume_storage,
                                 float f_scale, float g_scale, void* stream, bool sync);
 /* backproject_gpu with the planograms in `storage`; the volume d_f is fp32 (overwritten). */
 PETL_EXTERN bool backproject_gpu_ex(void** d_g, float* d_f, int storage, float g_scale, void* stream, bool sync);
+/* Batches for differentiable callers (no reference counterpart; the reference projects one volume per
+ * call): d_f holds `batch` contiguous fp32 volumes and d_g[i] `batch` contiguous copies of planogram i.
+ * project_gpu_batch overwrites every planogram; backproject_gpu_batch overwrites every volume with the sum
+ * of its planograms' back projections in `mode` (0 PETL, 1 matched = exact transpose of the forward,
+ * -1 = the model's mode), without changing the model's mode.  The fast kernels take a whole batch in one
+ * launch per planogram (the back projector's workspace holds up to 4 GiB of batch elements). */
+PETL_EXTERN bool project_gpu_batch(float** d_g, float* d_f, int batch, void* stream, bool sync);
+PETL_EXTERN bool backproject_gpu_batch(float** d_g, float* d_f, int batch, int mode, void* stream, bool sync);
 
 /* Device-resident MLEM / OSEM (the loop of src/petl_server.py:1534-1606 with every step on the GPU).
  * g: host planograms (data); f: host volume, used as the initial estimate (all zeros -> ones, negatives
@@ -248,15 +256,20 @@ PETL_EXTERN void* petl_service_create(double hbm_budget_bytes, int max_slots);
 PETL_EXTERN int petl_service_submit(void* service, const petl_recon_request* request);
 /* Device bytes a request would hold (its admission footprint), or 0 when it is invalid. */
 PETL_EXTERN double petl_service_request_bytes(const petl_recon_request* request);
+/* Checks the admission footprint: out[0] = the bytes petl_service_request_bytes counts, out[1] = the bytes
+ * the request's device buffers hold once allocated (the buffers are allocated and freed again).  Added for
+ * tests; no reference counterpart.  Returns false when the request is invalid or allocation fails. */
+PETL_EXTERN bool petl_service_request_footprint(const petl_recon_request* request, double* out);
 /* Blocks until request `id` is done and copies its image (numZ x numY x numX) to f.  times (optional,
  * 3 doubles): seconds from submission to admission, from admission to completion, and in total.
  * Returns false for an unknown id or a failed request. */
 PETL_EXTERN bool petl_service_wait(void* service, int id, float* f, double* times);
-/* stats (11 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
+/* stats (12 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
  * requests currently queued, the host seconds spent so far admitting requests (allocation, data upload
- * with sensitivities, graph capture) and freeing resources, and how many requests reused the pooled
+ * with sensitivities, graph capture) and freeing resources, how many requests reused the pooled
  * resources (model, buffers, captured iteration) of a finished request with the same geometry and MLEM
- * parameters.  Pooled memory counts against the budget until a new request needs it. */
+ * parameters, and how many of those captured the iteration again because their binary16 data or image
+ * scale differed.  Pooled memory counts against the budget until a new request needs it. */
 PETL_EXTERN bool petl_service_stats(void* service, double* stats);
 /* Finishes every queued request, then frees the service. */
 PETL_EXTERN void petl_service_destroy(void* service);

@@ -281,24 +281,32 @@ def _loglik_terms(q, y, rows=8):
     return tot
 
 
+def sensitivity_image(plans, vol, mode="petl", dtype=np.float32, nthreads=None):
+    """S = sum_i A_i^T 1, zeros replaced by ones (src/petl_server.py:1257-1273, 1580-1585)."""
+    S = np.zeros(vol.shape, dtype=dtype)
+    for pl in plans:
+        S += backproject([pl], vol, [np.ones(pl.shape, dtype=dtype)], mode=mode, dtype=dtype, nthreads=nthreads)
+    S[S == 0] = 1
+    return S
+
+
 def mlem(plans, vol, g, num_iter, f=None, delta=2.0, beta=0.0, prior="rd", p=1.0, dtype=np.float32,
-         mode="petl", callback=None, nthreads=None, loglik=None, keep_projections=True):
+         mode="petl", callback=None, nthreads=None, loglik=None, keep_projections=True, sens=None):
     """PETLserver.MLEM (src/petl_server.py:1534-1606) restated on the oracle projectors.  prior 'rd'
     follows the reference (relative differences, OSL); prior 'tv' uses the p-Huber TV gradient in the same
     OSL slot.  Planograms are processed one at a time (one planogram-sized temporary at a time), summing
     their back projections in planogram order, which is the order backproject() sums them in.  loglik: a
     list that receives, per iteration, the FP64 Poisson log-likelihood sum(y log Af - Af) over bins with
     Af > 0 of the image the iteration starts from.  callback(n, f, Pd) runs after every iteration; Pd is
-    the list of that iteration's projections, or None when keep_projections is False."""
+    the list of that iteration's projections, or None when keep_projections is False.  sens: the
+    sensitivity sensitivity_image() returns for the same planograms and mode, to skip forming it again
+    (a long run can then be resumed one iteration at a time)."""
     if f is None or not np.any(f):
         f = np.ones(vol.shape, dtype=dtype)
     else:
         f = np.array(f, dtype=dtype)
         f[f < 0] = 0
-    S = np.zeros(vol.shape, dtype=dtype)
-    for pl in plans:
-        S += backproject([pl], vol, [np.ones(pl.shape, dtype=dtype)], mode=mode, dtype=dtype, nthreads=nthreads)
-    S[S == 0] = 1
+    S = sensitivity_image(plans, vol, mode, dtype, nthreads) if sens is None else np.asarray(sens, dtype)
     for n in range(num_iter):
         d = np.zeros(vol.shape, dtype=dtype)
         Pd = [] if (callback is not None and keep_projections) else None

@@ -3,7 +3,8 @@
 `project(server, f)` is differentiable in `f`: its forward is the HIP forward projector (the fast
 separable kernels where the geometry allows), its backward the matched back projector, i.e. the exact
 transpose of that forward, so gradients of any loss of the planograms are exact up to fp32 rounding.
-Both run on torch's current HIP stream on device-resident tensors; nothing is copied to the host.
+Both run on torch's current HIP stream on device-resident tensors; nothing is copied to the host, and a
+batch costs one forward and one back projector launch per planogram.
 
 Shapes: `f` is (nz, ny, nx) or a batch (B, nz, ny, nx) of float32 volumes on the GPU; the result is a
 tuple with one tensor per planogram of shape (N_v1, N_v0, N_u1, N_u0) or (B, N_v1, N_v0, N_u1, N_u0).
@@ -40,28 +41,21 @@ def _stream(device):
 
 
 def _forward(server, f):
-    """f: (B, nz, ny, nx) contiguous -> list of (B, *shape_i) planogram tensors."""
+    """f: (B, nz, ny, nx) contiguous -> list of (B, *shape_i) planogram tensors, one launch per planogram
+    for the whole batch on the fast kernels."""
     shapes = _shapes(server)
     g = [torch.empty((f.shape[0],) + sh, dtype=torch.float32, device=f.device) for sh in shapes]
-    st = _stream(f.device)
-    for b in range(f.shape[0]):
-        server.project_gpu([gi[b] for gi in g], f[b], stream=st, sync=False)
+    server.project_gpu_batch(g, f, stream=_stream(f.device), sync=False)
     return g
 
 
 def backproject_matched(server, g, batch):
-    """Sum over planograms of A_i^T g_i with the matched back projector, per batch element.
-    g: list of (B, *shape_i) device tensors.  Returns (B, nz, ny, nx)."""
+    """Sum over planograms of A_i^T g_i with the matched back projector, per batch element (the mode is an
+    argument of the call; the server's own mode is left alone).  g: list of (B, *shape_i) device tensors.
+    Returns (B, nz, ny, nx)."""
     dev = g[0].device
     out = torch.empty((batch,) + _vol_shape(server), dtype=torch.float32, device=dev)
-    st = _stream(dev)
-    mode = server.get_backprojector_mode()
-    server.set_backprojector_mode("matched")
-    try:
-        for b in range(batch):
-            server.backproject_gpu([gi[b] for gi in g], out[b], stream=st, sync=False)
-    finally:
-        server.set_backprojector_mode(mode)
+    server.backproject_gpu_batch(g, out, mode="matched", stream=_stream(dev), sync=False)
     return out
 
 

@@ -42,6 +42,13 @@ public:
     // enqueues the copy of the image (and log-likelihoods) to host memory on s
     void fetch(float* f, double* loglik, hipStream_t s);
     const MlemParams& params() const { return prm_; }
+    // the binary16 scales of the data and of the image chosen by the last setup(); a captured iteration
+    // holds them as kernel arguments, so it can be replayed for another request only when they are equal
+    float data_scale() const { return s_y_; }
+    float image_scale() const { return s_f_; }
+    // device bytes the job's buffers hold now (planogram and volume buffers, back projector workspace,
+    // staging; not the kernel tables)
+    size_t allocated_bytes() const;
 
 private:
     void setup_storage(float* const* g, const float* f0, hipStream_t s);
@@ -55,7 +62,7 @@ private:
 };
 
 // host <-> device helpers shared by the exports
-void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs);
+void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs, size_t elem_bytes = sizeof(float));
 std::vector<float*> dev_ptrs(std::vector<DevBuf>& bufs);
 void h2d(void* d, const void* h, size_t n, hipStream_t s);
 void d2h(void* h, const void* d, size_t n, hipStream_t s);

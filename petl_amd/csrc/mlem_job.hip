@@ -7,11 +7,11 @@
 
 namespace petl {
 
-void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs)
+void ensure_plan_bufs(Model& m, std::vector<DevBuf>& bufs, size_t elem_bytes)
 {
     bufs.resize(m.plans.size());
     for (size_t i = 0; i < m.plans.size(); i++)
-        bufs[i].ensure(size_t(m.plans[i].elems()) * sizeof(float));
+        bufs[i].ensure(size_t(m.plans[i].elems()) * elem_bytes);
 }
 
 std::vector<float*> dev_ptrs(std::vector<DevBuf>& bufs)
@@ -68,6 +68,9 @@ MlemJob::MlemJob(Model& m, const MlemParams& prm) : m_(m), prm_(prm)
         throw hip_error("unsupported storage formats (planograms and image equal, or an fp32 image)");
 }
 
+// The staging buffer of 16-bit uploads: fp32 chunks of the data (2^24 values), then the initial image.
+static size_t stage_bytes(int64_t V) { return size_t(std::max<int64_t>(int64_t(1) << 24, V)) * sizeof(float); }
+
 size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
 {
     const int nsub = clamp_subsets(m, prm.nsub);
@@ -75,23 +78,33 @@ size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
     for (const auto& pl : m.plans)
         G += size_t(pl.elems());
     const VolGeom V = m.vgeom();
-    for (const auto& pl : m.plans) {
-        // the fast back projector's workspace, the largest of the planograms (see Model::prepare)
-        if (m.path == 0) {
-            const PlanGeom P = pl.geom();
-            const bool sep = (V.Tx == P.Tu0 && V.Ty == P.Tu0 && V.Tz == P.Tu1);
-            if (sep) {
-                const size_t ns = size_t(std::max(V.Nx, V.Ny));
-                back_ws = std::max(back_ws, size_t(P.Nv1) * P.Nu1 * ns * ns * sizeof(float));
-            }
-        }
-    }
+    for (const auto& pl : m.plans)  // the fast back projector's workspace, the largest of the planograms
+        if (m.path == 0)
+            back_ws = std::max(back_ws, fast_workspace_estimate(pl.geom(), V));
+    // the general forward's transposed volume copy
+    size_t vol_t = 0;
+    for (const auto& pl : m.plans)
+        if (m.path != 0 || fast_workspace_estimate(pl.geom(), V) == 0)
+            vol_t = size_t(m.vol_elems()) * sizeof(float);
     const size_t vol = size_t(m.vol_elems()) * sizeof(float);
     const size_t es = storage_bytes(prm.storage), ei = storage_bytes(prm.image_storage);
-    size_t plan_bufs = 2 + (prm.response ? 1 : 0) + (prm.loglik ? 1 : 0);  // y, aux (+resp) (+q)
-    // image and its update buffer in storage format; back projection and sensitivities fp32
-    return G * es * plan_bufs + size_t(m.vol_elems()) * ei * 2 + vol * (2 + size_t(nsub)) + back_ws +
-           (size_t(1) << 20);
+    const bool st16 = prm.storage != STORE_F32 || prm.image_storage != STORE_F32;
+    // y and aux in storage format; response and the log-likelihood's projection fp32
+    size_t plan_bytes = G * es * 2 + G * sizeof(float) * ((prm.response ? 1 : 0) + (prm.loglik ? 1 : 0));
+    // image and its update buffer in storage format; back projection and sensitivities fp32; fp32
+    // staging of 16-bit uploads; 1 MiB for the kernel tables and reduction scratch
+    return plan_bytes + size_t(m.vol_elems()) * ei * 2 + vol * (1 + size_t(nsub)) + back_ws + vol_t +
+           (st16 ? stage_bytes(m.vol_elems()) : 0) + (size_t(1) << 20);
+}
+
+size_t MlemJob::allocated_bytes() const
+{
+    const DeviceState& D = m_.dev;
+    size_t n = D.f.bytes + D.f2.bytes + D.d.bytes + D.S.bytes + D.h2.bytes + D.stage.bytes + D.red.bytes + D.fT.bytes;
+    for (const auto* v : {&D.y, &D.aux, &D.resp, &D.g})
+        for (const DevBuf& b : *v)
+            n += b.bytes;
+    return n;
 }
 
 void MlemJob::allocate()
@@ -104,13 +117,15 @@ void MlemJob::allocate()
             if (!m.uses_fast(int(i)))
                 throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal "
                                 "pitch)");
-    // planogram and image buffers hold storage-format elements; sized for fp32 they fit any format
-    ensure_plan_bufs(m, D.y);
-    ensure_plan_bufs(m, D.aux);
+    // planogram and image buffers hold storage-format elements
+    ensure_plan_bufs(m, D.y, storage_bytes(prm_.storage));
+    ensure_plan_bufs(m, D.aux, storage_bytes(prm_.storage));
     if (prm_.response)
         ensure_plan_bufs(m, D.resp);
-    D.f.ensure(V_ * sizeof(float));
-    D.f2.ensure(V_ * sizeof(float));
+    D.f.ensure(V_ * storage_bytes(prm_.image_storage));
+    D.f2.ensure(V_ * storage_bytes(prm_.image_storage));
+    if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32)
+        D.stage.ensure(stage_bytes(V_));
     D.d.ensure(V_ * sizeof(float));
     D.S.ensure(size_t(prm_.nsub) * V_ * sizeof(float));
     if (prm_.loglik) {
@@ -188,7 +203,7 @@ void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
     s_y_ = (st == STORE_F16) ? pow2_scale_for(ymax, 16384.0f) : 1.0f;
     // data: fp32 chunks through a device staging buffer, converted on the device
     const int64_t chunk = int64_t(1) << 24;
-    D.stage.ensure(size_t(chunk) * sizeof(float));
+    D.stage.ensure(stage_bytes(V_));
     for (size_t i = 0; i < nP; i++) {
         const int64_t n = m.plans[i].elems();
         for (int64_t o = 0; o < n; o += chunk) {
@@ -219,9 +234,8 @@ void MlemJob::setup_storage(float* const* g, const float* f0, hipStream_t s)
         const double mean = ssum > 0.0 ? ysum / ssum : 1.0;
         s_f_ = pow2_scale_for(std::max(float(1024.0 * mean), f0max), 32768.0f);
     }
-    // the initial image (volume-sized); the staging buffer may grow, so the data uploads must be done
+    // the initial image (volume-sized), through the same staging buffer once the data uploads are done
     PETL_HIP_CHECK(hipStreamSynchronize(s));
-    D.stage.ensure(size_t(V_) * sizeof(float));
     h2d(D.stage.p, f0, size_t(V_) * sizeof(float), s);
     launch_convert_st(sti, D.f.p, D.stage.f(), V_, 1.0f / s_f_, s);
     PETL_HIP_CHECK(hipStreamSynchronize(s));

@@ -1023,6 +1023,34 @@ bool backproject_gpu(float** d_g, float* d_f, void* stream, bool sync)
     });
 }
 
+bool project_gpu_batch(float** d_g, float* d_f, int batch, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || batch < 1 || !m->allDefined())
+        return false;
+    return guarded("project_gpu_batch", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        m->forward_batch(d_g, d_f, batch, s);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
+bool backproject_gpu_batch(float** d_g, float* d_f, int batch, int mode, void* stream, bool sync)
+{
+    Model* m = M();
+    if (!d_g || !d_f || batch < 1 || mode < -1 || mode > 1 || !m->allDefined())
+        return false;
+    return guarded("backproject_gpu_batch", [&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        m->back_batch(d_g, d_f, batch, mode < 0 ? m->back_mode : mode, s);
+        if (sync)
+            PETL_HIP_CHECK(hipStreamSynchronize(s));
+        return true;
+    });
+}
+
 bool project_gpu_ex(void** d_g, const void* d_f, int volume_storage, int planogram_storage, float f_scale,
                     float g_scale, void* stream, bool sync)
 {

@@ -7,6 +7,7 @@
 #include "petl_common.h"
 #include "ray_trace.h"
 #include "proj_fast.h"
+#include "proj_general.h"
 
 namespace petl {
 
@@ -46,10 +47,12 @@ struct DeviceState {
     hipStream_t stream = nullptr;
     uint64_t built_version = 0;                   // geometry version the fast plans were made for
     int built_path = -1;
-    std::vector<std::unique_ptr<FastPlan>> fast;  // per planogram; null = generic kernels
+    std::vector<std::unique_ptr<FastPlan>> fast;  // per planogram; null = general kernels
+    std::vector<std::unique_ptr<GenPlan>> gen;    // per planogram without a fast plan
     DevBuf f, f2, S, d;                           // volume-sized
     std::vector<DevBuf> g, y, aux, resp;          // planogram-sized
     DevBuf h2;                                    // fast back projector workspace
+    DevBuf fT;                                    // the volume with x and y swapped (general forward)
     DevBuf red;                                   // reduction scratch
     BinScratch bin;                               // list-mode binning
     DevBuf events, counter;                       // staged coincidences, accepted-event counters
@@ -84,6 +87,9 @@ struct Model {
     hipStream_t stream();
     void prepare();  // (re)build per-planogram kernel plans when geometry changed
     bool uses_fast(int i);
+    // the general forward of planogram i (transposing f into dev.fT first when `transpose` is set)
+    void gen_forward_one(size_t i, float* g, const float* f, bool transpose, int epilogue, const float* y,
+                         const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets);
     // g_i = A_i f for all planograms (device pointers), with an optional epilogue (ratio, or residual with
     // the weights wts; the y, resp, wts lists may be null).  v1 subset (first, stride) restricts the views;
     // (0,1) = all.
@@ -98,6 +104,11 @@ struct Model {
     int64_t sumsq_parts(int i);
     // f = sum_i A_i^T g_i (overwrites f)
     void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
+    // Batches of `batch` volumes f (contiguous, vol_elems() each) and planograms g[i] (contiguous, batch x
+    // planogram i): one launch per planogram on the fast kernels, a loop over the batch on the generic ones.
+    // back_batch overwrites f with the sum over planograms, in `mode`.
+    void forward_batch(float* const* g, const float* f, int batch, hipStream_t s);
+    void back_batch(float* const* g, float* f, int batch, int mode, hipStream_t s);
     // The same with the volume in storage format sf and the planograms in sg (STORE_*; fast kernels only,
     // the generic kernels are fp32).  For STORE_F16, x = stored * scale with the scales given; back writes
     // an fp32 volume.

@@ -173,6 +173,7 @@ void Model::prepare()
     if (dev.built_version == version && dev.built_path == path && dev.fast.size() == plans.size())
         return;
     dev.fast.clear();
+    dev.gen.clear();
     const VolGeom V = vgeom();
     size_t ws = 0;
     for (const auto& p : plans) {
@@ -181,6 +182,7 @@ void Model::prepare()
             fp = make_fast_plan(p.geom(), V);
         if (fp)
             ws = std::max(ws, fast_back_workspace_bytes(*fp));
+        dev.gen.push_back(fp ? nullptr : make_gen_plan(p.geom(), V));
         dev.fast.push_back(std::move(fp));
     }
     if (ws)
@@ -195,19 +197,32 @@ bool Model::uses_fast(int i)
     return i >= 0 && i < int(dev.fast.size()) && dev.fast[i] != nullptr;
 }
 
+void Model::gen_forward_one(size_t i, float* g, const float* f, bool transpose, int epilogue, const float* y,
+                            const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets)
+{
+    const GenPlan& gp = *dev.gen[i];
+    if (gp.any_xs && transpose) {
+        dev.fT.ensure(size_t(vol_elems()) * sizeof(float));
+        launch_transpose_xy(f, dev.fT.f(), vgeom(), s);
+    }
+    gen_forward(gp, g, f, gp.any_xs ? dev.fT.f() : nullptr, epilogue, y, resp, wts, s, subset, nsubsets);
+}
+
 void Model::forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
                     int subset, int nsubsets, float* const* wts)
 {
     prepare();
-    const VolGeom V = vgeom();
+    bool transposed = false;  // one transposed copy of f serves every general planogram of the call
     for (size_t i = 0; i < plans.size(); i++) {
         const float* yi = y ? y[i] : nullptr;
         const float* ri = resp ? resp[i] : nullptr;
         const float* wi = wts ? wts[i] : nullptr;
-        if (dev.fast[i])
+        if (dev.fast[i]) {
             fast_forward(*dev.fast[i], g[i], f, epilogue, yi, ri, s, subset, nsubsets, wi);
-        else
-            launch_fwd_generic(g[i], f, plans[i].geom(), V, epilogue, yi, ri, s, subset, nsubsets, wi);
+        } else {
+            gen_forward_one(i, g[i], f, !transposed, epilogue, yi, ri, wi, s, subset, nsubsets);
+            transposed = transposed || dev.gen[i]->any_xs;
+        }
     }
 }
 
@@ -239,7 +254,46 @@ void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset,
         if (dev.fast[i])
             fast_back(*dev.fast[i], g[i], f, mode, dev.h2.f(), s, subset, nsubsets);
         else
-            launch_back_generic(g[i], f, plans[i].geom(), V, mode, s, subset, nsubsets);
+            gen_back(*dev.gen[i], g[i], f, mode, s, subset, nsubsets);
+    }
+}
+
+void Model::forward_batch(float* const* g, const float* f, int batch, hipStream_t s)
+{
+    prepare();
+    const VolGeom V = vgeom();
+    const int64_t nv = vol_elems();
+    for (size_t i = 0; i < plans.size(); i++) {
+        if (dev.fast[i]) {
+            fast_forward_batch(*dev.fast[i], g[i], f, batch, s);
+            continue;
+        }
+        const int64_t G = plans[i].elems();
+        for (int b = 0; b < batch; b++)
+            gen_forward_one(i, g[i] + b * G, f + b * nv, true, EPI_NONE, nullptr, nullptr, nullptr, s, 0, 1);
+    }
+}
+
+// the batched back projector's workspace holds this many bytes at most (whole batch elements, at least one)
+constexpr size_t BATCH_WS_CAP = size_t(4) << 30;
+
+void Model::back_batch(float* const* g, float* f, int batch, int mode, hipStream_t s)
+{
+    prepare();
+    const VolGeom V = vgeom();
+    const int64_t nv = vol_elems();
+    launch_fill(f, 0.0f, nv * batch, s);
+    for (size_t i = 0; i < plans.size(); i++) {
+        const int64_t G = plans[i].elems();
+        if (dev.fast[i]) {
+            const size_t ws = fast_back_workspace_bytes(*dev.fast[i]);
+            const int k = int(std::max<size_t>(1, std::min<size_t>(size_t(batch), BATCH_WS_CAP / ws)));
+            dev.h2.ensure(ws * size_t(k));
+            fast_back_batch(*dev.fast[i], g[i], f, mode, dev.h2.f(), k, batch, s);
+            continue;
+        }
+        for (int b = 0; b < batch; b++)
+            gen_back(*dev.gen[i], g[i] + b * G, f + b * nv, mode, s);
     }
 }
 
@@ -247,13 +301,15 @@ void Model::forward_st(int sf, int sg, void* const* g, const void* f, int epilog
                        hipStream_t s, int subset, int nsubsets, float f_scale, float y_scale, float g_scale)
 {
     prepare();
+    bool transposed = false;
     for (size_t i = 0; i < plans.size(); i++) {
         if (!dev.fast[i] && (sf != STORE_F32 || sg != STORE_F32))
             throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
         if (!dev.fast[i]) {
-            launch_fwd_generic(static_cast<float*>(g[i]), static_cast<const float*>(f), plans[i].geom(), vgeom(),
-                               epilogue, y ? static_cast<const float*>(y[i]) : nullptr, resp ? resp[i] : nullptr, s,
-                               subset, nsubsets);
+            gen_forward_one(i, static_cast<float*>(g[i]), static_cast<const float*>(f), !transposed, epilogue,
+                            y ? static_cast<const float*>(y[i]) : nullptr, resp ? resp[i] : nullptr, nullptr, s,
+                            subset, nsubsets);
+            transposed = transposed || dev.gen[i]->any_xs;
             continue;
         }
         fast_forward_st(*dev.fast[i], sf, sg, g[i], f, epilogue, y ? y[i] : nullptr, resp ? resp[i] : nullptr, s, subset,
@@ -269,8 +325,7 @@ void Model::back_st(int st, void* const* g, float* f, int mode, hipStream_t s, i
         if (!dev.fast[i] && st != STORE_F32)
             throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
         if (!dev.fast[i]) {
-            launch_back_generic(static_cast<const float*>(g[i]), f, plans[i].geom(), vgeom(), mode, s, subset,
-                                nsubsets);
+            gen_back(*dev.gen[i], static_cast<const float*>(g[i]), f, mode, s, subset, nsubsets);
             continue;
         }
         fast_back_st(*dev.fast[i], st, g[i], f, mode, dev.h2.f(), s, subset, nsubsets, g_scale);

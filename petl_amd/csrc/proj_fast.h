@@ -36,12 +36,15 @@ struct FastPlan {
     // forward kernel shape
     int J = 0, C = 0, R = 1, ngroups = 0, SB = 0, hpad = 0, hrow = 0, xrow = 0;
     // back kernel shape (a-groups of BACK_AG columns; the z chunk of back_z is chosen per launch)
-    int nagroups = 0, spad_lo = 0, srow = 0;
+    int nagroups = 0, nsgroups = 0, spad_lo = 0, srow = 0;
     FastTables dev;
     void* dev_block = nullptr;  // one allocation holding all tables
     ~FastPlan();
 };
 
+// Bytes of the back projector workspace a fast plan of this geometry holds (0 when the geometry is not
+// one the fast kernels take); a host-only check that allocates nothing.
+size_t fast_workspace_estimate(const PlanGeom& P, const VolGeom& V);
 // Returns nullptr when the geometry is not eligible for the fast kernels.
 std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V);
 
@@ -62,11 +65,18 @@ void fast_forward_st(const FastPlan& fp, int sf, int sg, void* g, const void* f,
 int64_t fast_forward_groups(const FastPlan& fp, int v1_first = 0, int v1_stride = 1);
 void fast_forward_sumsq(const FastPlan& fp, const float* f, const float* resp, const float* wts, double* part,
                         hipStream_t s, int v1_first = 0, int v1_stride = 1);
+// g = A f for `batch` volumes f[b] (contiguous, V elements each) into planograms g[b] (contiguous, one
+// planogram each): one launch (a batch axis in the grid) per 65535 / N_v1 elements.
+void fast_forward_batch(const FastPlan& fp, float* g, const float* f, int batch, hipStream_t s);
 // Bytes of scratch fast_back needs (the intermediate u0-gathered slabs).
 size_t fast_back_workspace_bytes(const FastPlan& fp);
 // f += A^T g (mode BACK_PETL or BACK_MATCHED), one planogram.
 void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, hipStream_t s,
                int v1_first = 0, int v1_stride = 1);
+// f[b] += A^T g[b] for `batch` planograms and volumes (contiguous), in launches of up to ws_elements
+// elements (the workspace holds ws_elements x fast_back_workspace_bytes).
+void fast_back_batch(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, int ws_elements,
+                     int batch, hipStream_t s);
 // The same with the planogram in storage format st (g_scale as above); f stays fp32.
 void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode, float* workspace, hipStream_t s,
                   int v1_first, int v1_stride, float g_scale);

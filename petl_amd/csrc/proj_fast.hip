@@ -47,9 +47,13 @@ constexpr int FWD_THREADS = 512;
 constexpr int FWD_WPROW = 16;  // floats per (block, view) row of the forward's wp table: SB wp, mask, pad
 constexpr int FWD_WAVES = FWD_THREADS / 64;
 constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
-constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows
+constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows (Na <= 128)
+constexpr int FWD_HP_WIDE = 449;  // the same for the wide instances (128 < Na <= 256)
+// row pitch of the forward instance for in-plane axes of up to NAM voxels
+constexpr int fwd_hp(int NAM) { return NAM <= 128 ? FWD_HP : FWD_HP_WIDE; }
 constexpr int BACK_RB = 8;    // v0 rows per LDS block in the back projector
 constexpr int BACK_AG = 128;  // a-values per lane in the back projector
+constexpr int BACK_SG = 256;  // slices per back projector workgroup (one per lane)
 constexpr int BACK_TW = 32;   // transpose tile width of the back epilogue
 constexpr int BACK_STW = 256; // dwords per staged row of 16-bit planogram bins (Nu0 <= 511)
 
@@ -81,6 +85,11 @@ struct FwdArgs {
     float f_scale, y_scale, g_inv_scale;  // storage scales (16-bit formats; 1 for fp32)
     const float* wts;                     // EPI_RESID / sum-of-squares weights (nullable)
     double* part;                         // sum-of-squares instance: one partial per workgroup
+    // batch: blockIdx.z = b * nv1 + k runs v1 row v1_first + k * v1_stride of element b, whose volume and
+    // planogram start b * f_bstride and b * g_bstride elements after f and g (nv1 = gridDim.z, b = 0
+    // without a batch)
+    int nv1;
+    int64_t f_bstride, g_bstride;
 };
 
 // compile-time epilogue of a forward instance: the runtime epilogue argument (none / ratio), the
@@ -92,7 +101,7 @@ constexpr int EK_RUNTIME = 0, EK_RESID = 1, EK_SUMSQ = 2;
 // forward
 // =============================================================================================
 // z interpolation of R u1 rows x SB slices, software-pipelined: the global loads of block b+1 are issued
-// before the block-b accumulation and written to LDS after it.  LDS layout per buffer: [R][SB][FWD_HP].
+// before the block-b accumulation and written to LDS after it.  LDS layout per buffer: [R][SB][HP].
 // The loaded volume values stay in their storage format until zstore, so the loads remain in flight.
 template <int MAXE, int ST>
 struct ZLoads {
@@ -102,7 +111,7 @@ struct ZLoads {
 
 // The thread's elements of a block, the same for every block: slice sl within the block, the column's
 // offset in the volume (ia * fs_a), the z-row base iu1_0 + r - 1 of its u1 row, whether that row is live,
-// and the LDS destination (r * SB + sl) * FWD_HP + ia, or -1 past the block's R * SB * Na values (those
+// and the LDS destination (r * SB + sl) * HP + ia, or -1 past the block's R * SB * Na values (those
 // elements load clamped addresses and are never stored).  Offsets are 32-bit: the fast path takes
 // volumes of fewer than 2^31 voxels.
 template <int MAXE>
@@ -111,7 +120,7 @@ struct ZMap {
     bool live[MAXE];
 };
 
-template <int SB, int R, int MAXE>
+template <int SB, int R, int MAXE, int HP>
 __device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int iu1_0, int nrows, int tid)
 {
     const int per_row = SB * A.Na;
@@ -134,7 +143,7 @@ __device__ __forceinline__ void zmap(ZMap<MAXE>& M, const FwdArgs& A, int iu1_0,
         M.aoff[e] = ia * int(A.fs_a);
         M.zr[e] = iu1_0 + r - 1;
         M.live[e] = r < nrows;
-        M.dst[e] = (idx0 < n) ? (r * SB + sl) * FWD_HP + ia : -1;
+        M.dst[e] = (idx0 < n) ? (r * SB + sl) * HP + ia : -1;
     }
 }
 
@@ -211,7 +220,7 @@ __device__ __forceinline__ void epi_load(EpiIn<C>& in, const FwdArgs& A, int epi
 // the volume and SG of the planogram and the data y.  EK: the compile-time epilogue (EK_*); the residual
 // and sum-of-squares forms are separate instances so that the projection kernels keep no weight pointer
 // live.
-template <int J, int C, int SB, int R, int SF, int SG, int EK = EK_RUNTIME>
+template <int J, int C, int SB, int R, int SF, int SG, int EK = EK_RUNTIME, int NAM = 128>
 __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
                                                                const typename Stor<SF>::T* __restrict__ f,
                                                                FwdArgs A, const int2* __restrict__ ftab,
@@ -223,6 +232,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
 {
     using TS = typename Stor<SG>::T;
     constexpr bool RES = EK == EK_RESID, SQ = EK == EK_SUMSQ;
+    constexpr int HP = fwd_hp(NAM);
     static_assert((2 * R * SB) % 4 == 0, "the z table after the row buffers must be 16-byte aligned");
     extern __shared__ __align__(16) float hbuf[];
     __shared__ int s_lo, s_hi;
@@ -232,7 +242,10 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // one workgroup per (group of 56 views, u1 row, v1); a renumbering that gives each XCD a contiguous
     // range of u1 rows (so resident workgroups share a band of the volume in L2) measured 30 % slower
-    const int grp = blockIdx.x, iu1_0 = blockIdx.y * R, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const int bz = blockIdx.z, bel = bz / A.nv1;
+    const int grp = blockIdx.x, iu1_0 = blockIdx.y * R, iv1 = A.v1_first + (bz - bel * A.nv1) * A.v1_stride;
+    g += bel * A.g_bstride;
+    f += bel * A.f_bstride;
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz;
@@ -293,7 +306,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         s_lo = Ns;
         s_hi = -1;
     }
-    const int hbuf_elems = 2 * R * SB * FWD_HP;
+    const int hbuf_elems = 2 * R * SB * HP;
     // LDS after the two row buffers: this v1 row's z table (16-byte aligned: hbuf_elems is a multiple of 4)
     int4* s_z = reinterpret_cast<int4*>(hbuf + hbuf_elems);
     for (int i = tid; i < hbuf_elems; i += FWD_THREADS)
@@ -335,9 +348,9 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
         // blocks start at multiples of SB so the table can hold each slice's LDS row offset
         const int is_first = (is_lo / SB) * SB;
         const int nblk = (is_hi - is_first) / SB + 1;
-        constexpr int MAXE = (R * SB * 128 + FWD_THREADS - 1) / FWD_THREADS;  // Na <= 128
+        constexpr int MAXE = (R * SB * NAM + FWD_THREADS - 1) / FWD_THREADS;  // Na <= NAM
         ZMap<MAXE> M;
-        zmap<SB, R>(M, A, iu1_0, nrows, tid);
+        zmap<SB, R, MAXE, HP>(M, A, iu1_0, nrows, tid);
         ZLoads<MAXE, SF> L;
         zload<SB, R>(L, M, A, f, s_z, is_first);
         zstore(L, M, hbuf + A.hpad, A.f_scale);
@@ -347,8 +360,8 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
             if (more)
                 zload<SB, R>(L, M, A, f, s_z, is_first + (blk + 1) * SB);
             const int is0 = is_first + blk * SB;
-            const float* hb = hbuf + (blk & 1) * R * SB * FWD_HP + A.hpad + lane;
-            // this block's table rows: [block][view] x {SB entries {(is % SB) * FWD_HP + k, wq}} and
+            const float* hb = hbuf + (blk & 1) * R * SB * HP + A.hpad + lane;
+            // this block's table rows: [block][view] x {SB entries {(is % SB) * HP + k, wq}} and
             // {SB x wp, chunk mask, pad}; the wave's views sit at a fixed stride, so every view's scalar
             // loads are immediate offsets from these two addresses
             const int bi = is0 / SB;
@@ -398,7 +411,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                                 float v[SB];
 #pragma unroll
                                 for (int s = 0; s < SB; s++)
-                                    v[s] = hp[s][r * SB * FWD_HP + 64 * c];
+                                    v[s] = hp[s][r * SB * HP + 64 * c];
 #pragma unroll
                                 for (int s = 0; s < SB; s++) {
                                     W[r][j][c] += wp[q][s] * v[s];
@@ -410,7 +423,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
                 }
             }
             if (more)
-                zstore(L, M, hbuf + ((blk + 1) & 1) * R * SB * FWD_HP + A.hpad, A.f_scale);
+                zstore(L, M, hbuf + ((blk + 1) & 1) * R * SB * HP + A.hpad, A.f_scale);
             __syncthreads();
         }
     }
@@ -505,8 +518,15 @@ struct BackArgs {
     int mode;
     int psi90;
     int v1_first, v1_stride;
-    int sb;  // slice block of the forward table layout
+    int sb;    // slice block of the forward table layout
+    int hrow;  // the forward's LDS row pitch, which the table's offsets include
     float g_scale;  // storage scale of the planogram (16-bit formats; 1 for fp32)
+    // batch: back_h2's blockIdx.z = b * nv1 + k (v1 row v1_first + k * v1_stride of element b) and
+    // back_z's blockIdx.z = b * nzc + z chunk; element b's planogram, workspace and volume start b *
+    // g_bstride, b * h_bstride and b * f_bstride elements after g, h2 and f
+    int nv1, nzc;
+    int64_t g_bstride, h_bstride, f_bstride;
+    int nagroups;  // a groups per slice group (blockIdx.x of back_h2 = slice group x nagroups + a group)
 };
 
 // DMA: the next block of v0 rows goes from HBM straight into LDS (global_load_lds, one 64-dword row
@@ -525,14 +545,20 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     extern __shared__ float sbuf[];
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int ag = blockIdx.x, iu1 = blockIdx.y, iv1 = A.v1_first + blockIdx.z * A.v1_stride;
+    const int bz = blockIdx.z, bel = bz / A.nv1;
+    // blockIdx.x = slice group x a group: a workgroup takes BACK_SG slices (one per lane) of one a group
+    const int sgr = blockIdx.x / A.nagroups, ag = blockIdx.x - sgr * A.nagroups;
+    const int iu1 = blockIdx.y, iv1 = A.v1_first + (bz - bel * A.nv1) * A.v1_stride;
+    g += bel * A.g_bstride;
+    h2 += bel * A.h_bstride;
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1, Ns = A.Ns, Na = A.Na;
     const int2 rr = rows[iv1];
     if (iu1 < rr.x || iu1 > rr.y)
         return;
     const int a0 = ag * AG;
-    const int is = min(tid, Ns - 1);
+    const int s0 = sgr * BACK_SG, ns = min(BACK_SG, Ns - s0);  // this workgroup's slices [s0, s0 + ns)
+    const int is = min(s0 + tid, Ns - 1);
 
     float acc[AG];
 #pragma unroll
@@ -657,19 +683,23 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     // not alias the row buffer the accumulation reads
     constexpr bool STAGED = DMA && ST != STORE_F32;
     __shared__ __align__(16) uint32_t s_stage[STAGED ? BACK_RB * BACK_STW : 1];
+    // Every wave reads rows that other waves' LDS DMA wrote: each wave waits for its own DMA (vmcnt(0))
+    // before the barrier, so that after the barrier every wave's DMA has landed.
     if (STAGED) {
         dma16(0, s_stage);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
         convert(0, 0, s_stage);
     } else if (DMA) {
         dma(0, 0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     } else {
         load(0, 0);
     }
     __syncthreads();
     for (int blk = 0; blk < nblk; blk++) {
         const float* sb = sbuf + (blk & 1) * BACK_RB * A.srow + A.spad_lo + a0;
-        const int row_off = (is % A.sb) * FWD_HP;
+        const int row_off = (is % A.sb) * A.hrow;
         if (DMA) {
             // this block's table entries first, waited for before the next block's DMA is issued: loads
             // and LDS DMA retire through one in-order counter, and the accumulation must not wait for
@@ -706,6 +736,8 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
                     }
                 }
             }
+            // the next block's DMA (issued above) must have landed for every wave after the barrier
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             __syncthreads();
             if (STAGED && blk + 1 < nblk) {
                 convert(blk + 1, (blk + 1) & 1, s_stage);
@@ -742,16 +774,16 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
 
     // transposed write: workspace[iv1][iu1][s][a]
     float* tile = sbuf;
-    float* out = h2 + (int64_t(iv1) * Nu1 + iu1) * Ns * int64_t(Na);
+    float* out = h2 + ((int64_t(iv1) * Nu1 + iu1) * Ns + s0) * int64_t(Na);
 #pragma unroll
     for (int q = 0; q < AG / BACK_TW; q++) {
-        if (tid < Ns) {
+        if (tid < ns) {
 #pragma unroll
             for (int t = 0; t < BACK_TW; t++)
                 tile[tid * (BACK_TW + 1) + t] = acc[q * BACK_TW + t];
         }
         __syncthreads();
-        for (int idx = tid; idx < Ns * BACK_TW; idx += nthr) {
+        for (int idx = tid; idx < ns * BACK_TW; idx += nthr) {
             const int s = idx / BACK_TW, t = idx - s * BACK_TW;
             const int a = a0 + q * BACK_TW + t;
             if (a < Na)
@@ -772,7 +804,10 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
     const int lane = threadIdx.x;
     const int ia = blockIdx.x * 64 + lane;
     const int is = blockIdx.y;
-    const int z0 = blockIdx.z * ZC;
+    const int bel = blockIdx.z / A.nzc;
+    const int z0 = (blockIdx.z - bel * A.nzc) * ZC;
+    h2 += bel * A.h_bstride;
+    f += bel * A.f_bstride;
     const PlanGeom& P = A.P;
     const VolGeom& V = A.V;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz, Nu1 = P.Nu1;
@@ -854,19 +889,19 @@ using fwd_fn = void (*)(typename Stor<SG>::T*, const typename Stor<SF>::T*, FwdA
 
 // FWD_J views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
 // two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower)
-template <int SF, int SG>
-fwd_fn<SF, SG> pick_fwd(int C, int ek = EK_RUNTIME)
+template <int SF, int SG, int NAM>
+fwd_fn<SF, SG> pick_fwd_na(int C, int ek)
 {
 #define PETL_FWD_BY_C(EKV)                                                \
     switch (C) {                                                          \
-    case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG, EKV>;          \
-    case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG, EKV>;          \
-    case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG, EKV>;          \
-    case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG, EKV>;          \
-    case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG, EKV>;          \
-    case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG, EKV>;          \
-    case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG, EKV>;          \
-    case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG, EKV>;          \
+    case 1: return fwd_fast_kernel<FWD_J, 1, 8, 1, SF, SG, EKV, NAM>;     \
+    case 2: return fwd_fast_kernel<FWD_J, 2, 8, 1, SF, SG, EKV, NAM>;     \
+    case 3: return fwd_fast_kernel<FWD_J, 3, 8, 1, SF, SG, EKV, NAM>;     \
+    case 4: return fwd_fast_kernel<FWD_J, 4, 8, 1, SF, SG, EKV, NAM>;     \
+    case 5: return fwd_fast_kernel<FWD_J, 5, 8, 1, SF, SG, EKV, NAM>;     \
+    case 6: return fwd_fast_kernel<FWD_J, 6, 8, 1, SF, SG, EKV, NAM>;     \
+    case 7: return fwd_fast_kernel<FWD_J, 7, 8, 1, SF, SG, EKV, NAM>;     \
+    case 8: return fwd_fast_kernel<FWD_J, 8, 8, 1, SF, SG, EKV, NAM>;     \
     default: return nullptr;                                              \
     }
     if constexpr (SF == STORE_F32 && SG == STORE_F32) {
@@ -879,6 +914,15 @@ fwd_fn<SF, SG> pick_fwd(int C, int ek = EK_RUNTIME)
     }
     PETL_FWD_BY_C(EK_RUNTIME)
 #undef PETL_FWD_BY_C
+}
+
+// FWD_J views per wave, one u1 row per workgroup, slice blocks of 8 (the fastest shape measured at 128^3;
+// two rows per workgroup with 4 or 2 views per wave ran 17-37 % slower); in-plane axes of up to 128
+// voxels, or the wide instances (longer LDS rows, more z-stage elements per thread) up to 256
+template <int SF, int SG>
+fwd_fn<SF, SG> pick_fwd(int C, int ek, bool wide)
+{
+    return wide ? pick_fwd_na<SF, SG, 256>(C, ek) : pick_fwd_na<SF, SG, 128>(C, ek);
 }
 
 inline float fbits(int i)
@@ -908,17 +952,20 @@ FastPlan::~FastPlan()
         (void)hipFree(dev_block);
 }
 
-std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
+// Host check of the geometry conditions of the fast kernels (the LDS fits checked in make_fast_plan aside):
+// psi = 0 or 90 degrees, voxel pitch = crystal pitch, every view on the same slice branch, at most 8 chunks
+// of 64 u0 bins, in-plane axes of at most 256 and slice axes of at most 1024 voxels.  psi90 receives 1 for
+// x slices.
+static bool fast_geometry(const PlanGeom& P, const VolGeom& V, int& psi90)
 {
-    int psi90;
     if (P.sin_psi == 0.0f && P.cos_psi == 1.0f)
         psi90 = 0;
     else if (P.sin_psi == 1.0f && std::fabs(P.cos_psi) < 1.0e-6f)
         psi90 = 1;
     else
-        return nullptr;
+        return false;
     if (!(V.Tx == P.Tu0 && V.Ty == P.Tu0 && V.Tz == P.Tu1))
-        return nullptr;
+        return false;
     // every view must take the same slice branch as this kernel assumes (src/projectors.cu:77)
     for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
         const float v0 = coordh(iv0, P.Tv0, P.v0_0);
@@ -927,9 +974,26 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         const float rx = -(t1 + P.sin_psi), ry = -(t2 - P.cos_psi);
         const bool xs = std::fabs(rx) >= std::fabs(ry);
         if (xs != (psi90 == 1))
-            return nullptr;
+            return false;
     }
+    const int Ns = psi90 ? V.Nx : V.Ny, Na = psi90 ? V.Ny : V.Nx;
+    return (P.Nu0 + 1 + 63) / 64 <= 8 && Ns <= 1024 && Na <= 256 &&
+           int64_t(V.Nx) * V.Ny * V.Nz < (int64_t(1) << 31);
+}
 
+size_t fast_workspace_estimate(const PlanGeom& P, const VolGeom& V)
+{
+    int psi90;
+    if (!fast_geometry(P, V, psi90))
+        return 0;
+    return size_t(P.Nv1) * P.Nu1 * size_t(V.Nx) * V.Ny * sizeof(float);
+}
+
+std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
+{
+    int psi90;
+    if (!fast_geometry(P, V, psi90))
+        return nullptr;
     auto fp = std::make_unique<FastPlan>();
     fp->P = P;
     fp->V = V;
@@ -959,8 +1023,10 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     fp->ngroups = (P.Nv0 + FWD_WAVES * fp->J - 1) / (FWD_WAVES * fp->J);
     fp->SB = 8;
     fp->nagroups = (Na + BACK_AG - 1) / BACK_AG;
-    if (Ns > 256 || Na > 128)  // one lane per slice in back_h2_kernel; z-interpolated row length
+    fp->nsgroups = (Ns + BACK_SG - 1) / BACK_SG;
+    if (Na > 256)  // the forward's z-interpolated rows hold the whole in-plane axis
         return nullptr;
+    fp->hrow = (Na <= 128) ? FWD_HP : FWD_HP_WIDE;
     if (int64_t(V.Nx) * V.Ny * V.Nz >= (int64_t(1) << 31))  // the forward's 32-bit voxel offsets
         return nullptr;
 
@@ -1008,7 +1074,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
         }
         // the forward reads entry .x as the slice's LDS offset within an SB-aligned block
         for (int is = 0; is < xrow; is++)
-            xtab[size_t(iv0) * xrow + is].x += (is % fp->SB) * FWD_HP;
+            xtab[size_t(iv0) * xrow + is].x += (is % fp->SB) * fp->hrow;
     }
     for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
         const double v1 = coordh(iv1, P.Tv1, P.v1_0);
@@ -1078,7 +1144,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
                 ftab[row * SBk + sl] = e;
                 fwp[row * FWD_WPROW + sl] = xwp[size_t(iv0) * xrow + is];
                 if (is < Ns) {
-                    const int k = e.x - sl * FWD_HP;  // lane p reads H[s][p + 64 c + k]
+                    const int k = e.x - sl * fp->hrow;  // lane p reads H[s][p + 64 c + k]
                     for (int c = 0; c < fp->C; c++)
                         if (64 * c + k <= Na - 1 && 64 * c + 63 + k >= 0)
                             mask |= 1 << c;
@@ -1087,8 +1153,7 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
             fwp[row * FWD_WPROW + SBk] = fbits(mask);
         }
     fp->hpad = 64 + kspread + 1;
-    fp->hrow = FWD_HP;
-    if (fp->hpad + Na + 63 + kspread >= FWD_HP)  // reads of an active chunk must stay inside the LDS row
+    if (fp->hpad + Na + 63 + kspread >= fp->hrow)  // reads of an active chunk must stay inside the LDS row
         return nullptr;
     // back: S rows read at indices [m, m + NaPad] for m in [mmin, mmax]
     const int na_pad = fp->nagroups * BACK_AG;
@@ -1136,7 +1201,7 @@ template <int SF, int SG>
 static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
                            const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
                            float y_scale, float g_scale, const float* wts = nullptr, int ek = -1,
-                           double* part = nullptr)
+                           double* part = nullptr, int batch = 1, int64_t f_bstride = 0, int64_t g_bstride = 0)
 {
     using TG = typename Stor<SG>::T;
     using TF = typename Stor<SF>::T;
@@ -1162,11 +1227,16 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     if (ek < 0)
         ek = (epilogue == EPI_RESID) ? EK_RESID : EK_RUNTIME;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, ek);
+    A.nv1 = nv1;
+    A.f_bstride = f_bstride;
+    A.g_bstride = g_bstride;
+    if (int64_t(nv1) * batch > 65535)
+        throw hip_error("fast forward: batch x v1 rows exceeds the grid");
+    fwd_fn<SF, SG> fn = pick_fwd<SF, SG>(fp.C, ek, fp.Na > 128);
     if (!fn || fp.J != FWD_J || fp.R != 1)
         throw hip_error("fast forward: no kernel instance for this shape");
-    const size_t lds = size_t(2) * fp.R * fp.SB * FWD_HP * sizeof(float) + size_t(fp.Ns) * sizeof(int4);
-    hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1), dim3(FWD_THREADS), lds, s,
+    const size_t lds = size_t(2) * fp.R * fp.SB * fp.hrow * sizeof(float) + size_t(fp.Ns) * sizeof(int4);
+    hipLaunchKernelGGL(fn, dim3(fp.ngroups, (fp.P.Nu1 + fp.R - 1) / fp.R, nv1 * batch), dim3(FWD_THREADS), lds, s,
                        static_cast<TG*>(g), static_cast<const TF*>(f), A, fp.dev.ftab, fp.dev.fwp, fp.dev.ztab,
                        fp.dev.fscale, epilogue, static_cast<const TG*>(y), resp);
     PETL_LAUNCH_CHECK();
@@ -1177,6 +1247,18 @@ void fast_forward(const FastPlan& fp, float* g, const float* f, int epilogue, co
 {
     fast_forward_t<STORE_F32, STORE_F32>(fp, g, f, epilogue, y, resp, s, v1_first, v1_stride, 1.0f, 1.0f, 1.0f,
                                          wts);
+}
+
+void fast_forward_batch(const FastPlan& fp, float* g, const float* f, int batch, hipStream_t s)
+{
+    const int64_t V = int64_t(fp.V.Nx) * fp.V.Ny * fp.V.Nz, G = plan_elems(fp.P);
+    const int nv1 = fp.P.Nv1;
+    const int per = std::max(1, 65535 / nv1);  // batch elements per launch (grid z = elements x v1 rows)
+    for (int b0 = 0; b0 < batch; b0 += per) {
+        const int nb = std::min(per, batch - b0);
+        fast_forward_t<STORE_F32, STORE_F32>(fp, g + b0 * G, f + b0 * V, EPI_NONE, nullptr, nullptr, s, 0, 1, 1.0f,
+                                             1.0f, 1.0f, nullptr, -1, nullptr, nb, V, G);
+    }
 }
 
 int64_t fast_forward_groups(const FastPlan& fp, int v1_first, int v1_stride)
@@ -1218,7 +1300,7 @@ size_t fast_back_workspace_bytes(const FastPlan& fp)
 
 template <int ST>
 static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, float* workspace, hipStream_t s,
-                        int v1_first, int v1_stride, float g_scale)
+                        int v1_first, int v1_stride, float g_scale, int batch = 1)
 {
     using TS = typename Stor<ST>::T;
     BackArgs A;
@@ -1239,10 +1321,23 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     A.v1_stride = v1_stride;
     A.g_scale = g_scale;
     const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
+    const int nzc = (fp.Nz + zc - 1) / zc;
+    A.nv1 = nv1;
+    A.nzc = nzc;
+    A.g_bstride = plan_elems(fp.P);
+    A.h_bstride = int64_t(fast_back_workspace_bytes(fp) / sizeof(float));
+    A.f_bstride = int64_t(fp.V.Nx) * fp.V.Ny * fp.V.Nz;
+    if (int64_t(nv1) * batch > 65535 || int64_t(nzc) * batch > 65535)
+        throw hip_error("fast back: batch x v1 rows exceeds the grid");
     const int2* rows = fp.dev.rows + (mode == BACK_PETL ? fp.P.Nv1 : 0);
     const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
-    const int nthr = ((fp.Ns + 63) / 64) * 64;
-    const size_t lds = std::max(size_t(2) * BACK_RB * fp.srow, size_t(fp.Ns) * (BACK_TW + 1)) * sizeof(float);
+    A.hrow = fp.hrow;
+    A.nagroups = fp.nagroups;
+    const int nthr = ((std::min(fp.Ns, BACK_SG) + 63) / 64) * 64;
+    const size_t lds =
+        std::max(size_t(2) * BACK_RB * fp.srow, size_t(std::min(fp.Ns, BACK_SG)) * (BACK_TW + 1)) * sizeof(float);
+    const int gx = fp.nagroups * fp.nsgroups;
     // the 16-bit DMA form moves row dwords: rows of an even number of bins (and so 4-byte aligned); its
     // matched-mode instance needs registers for two waves per SIMD only, so that mode keeps the register form
     const bool dma = (ST == STORE_F32) ||
@@ -1250,19 +1345,18 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
     if (dma) {
         // MASKED: matched mode's per-bin u0 mask, a separate instance so that PETL mode carries none of it
         if (mode == BACK_MATCHED)
-            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true, true>), dim3(fp.nagroups, fp.P.Nu1, nv1),
+            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true, true>), dim3(gx, fp.P.Nu1, nv1 * batch),
                                dim3(nthr), lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow,
                                scale, rows);
         else
-            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr),
+            hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST, true>), dim3(gx, fp.P.Nu1, nv1 * batch), dim3(nthr),
                                lds, s, static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     } else {
-        hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(fp.nagroups, fp.P.Nu1, nv1), dim3(nthr), lds, s,
+        hipLaunchKernelGGL((back_h2_kernel<BACK_AG, ST>), dim3(gx, fp.P.Nu1, nv1 * batch), dim3(nthr), lds, s,
                            static_cast<const TS*>(g), workspace, A, fp.dev.xtab, fp.xrow, scale, rows);
     }
     PETL_LAUNCH_CHECK();
-    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
-    const dim3 zgrid((fp.Na + 63) / 64, fp.Ns, (fp.Nz + zc - 1) / zc);
+    const dim3 zgrid((fp.Na + 63) / 64, fp.Ns, nzc * batch);
     if (zc == 32)
         hipLaunchKernelGGL(back_z_kernel<32>, zgrid, dim3(64), 0, s, workspace, f, A, fp.dev.ztab, rows);
     else if (zc == 16)
@@ -1278,6 +1372,17 @@ void fast_back(const FastPlan& fp, const float* g, float* f, int mode, float* wo
                int v1_stride)
 {
     fast_back_t<STORE_F32>(fp, g, f, mode, workspace, s, v1_first, v1_stride, 1.0f);
+}
+
+void fast_back_batch(const FastPlan& fp, const float* g, float* f, int mode, float* workspace, int ws_elements,
+                     int batch, hipStream_t s)
+{
+    const int64_t V = int64_t(fp.V.Nx) * fp.V.Ny * fp.V.Nz, G = plan_elems(fp.P);
+    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
+    const int nzc = (fp.Nz + zc - 1) / zc;
+    const int per = std::max(1, std::min({ws_elements, 65535 / fp.P.Nv1, 65535 / nzc}));
+    for (int b0 = 0; b0 < batch; b0 += per)
+        fast_back_t<STORE_F32>(fp, g + b0 * G, f + b0 * V, mode, workspace, s, 0, 1, 1.0f, std::min(per, batch - b0));
 }
 
 void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode, float* workspace, hipStream_t s,

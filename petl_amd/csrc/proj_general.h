@@ -1,0 +1,69 @@
+// General SF projector for any panel angle psi and any voxel / crystal pitch (the geometries the
+// separable fast kernels do not take: oblique panels, voxels of another size than the crystals).
+//
+// Model.  A view (v1, v0) is a set of parallel lines of response.  The reference steps through the
+// volume in planes ("slices") normal to the axis the line direction r is closest to
+// (src/projectors.cu:77: x planes when |r_x| >= |r_y|, else y planes).  In slice s the line of bin
+// (u0, u1) crosses the plane at in-plane coordinate a_c (x for y planes, y for x planes) and height z_c;
+// the bin's footprint there is an interval of width T_u0 |p_a| along a (p_a = cos psi for y planes,
+// sin psi for x planes) and T_u1 along z, and a voxel contributes the product of the overlap lengths of
+// its box with those intervals (src/projectors.cu:100-177 evaluates the same overlaps as
+// min(upper) - max(lower) in bin units).  Both crossings are affine in the bin indices and the slice
+// index, so in voxel-index units
+//
+//     xi   = xi0[v0][s] + Au * iu0                      (a crossing; Au, xi0 depend on v0 only)
+//     zeta = Z0 + Bs * s + Bu * iu0 + Bz * iu1          (z crossing; Z0, Bs, Bu per view)
+//
+// with every coefficient formed in FP64 on the host.  The bin value is scale * sum f * ov(xi, hb, i) *
+// ov(zeta, hz, k), ov(c, h, i) the overlap of [c - h, c + h] with [i - 1/2, i + 1/2], and scale the
+// slice length over the footprint normalisation and the obliquity 1 / sqrt(1 + v0^2 + v1^2).
+//
+// The forward and the matched back projector evaluate the crossings and overlaps with the same
+// functions on the same table entries, and a tap counts exactly when both overlaps are positive, so the
+// matched back projector is the exact transpose of the forward.  The PETL-mode back projector follows the
+// reference's voxel-driven semantics instead (src/projectors.cu:240-377): per voxel and view row v1 the
+// voxel-centre u1 test and u1 interval, per v0 the projection of the voxel's mid-line along the dominant
+// axis onto u0, exact bilinear weights.
+#pragma once
+#include <memory>
+#include "petl_common.h"
+
+namespace petl {
+
+struct GenView {
+    float Au;     // d xi / d iu0
+    float hb;     // half width of a bin's footprint along a, in voxels
+    float Z0, Bs, Bu;
+    float scale;  // forward / matched per-bin factor
+    float lpetl;  // PETL back: 1 / max(|w_y|, |w_x|) / sqrt(1 + v0^2 + v1^2)
+    int xs;       // 1: x planes (a = y, slices along x); 0: y planes (a = x)
+};
+
+struct GenPlan {
+    PlanGeom P;
+    VolGeom V;
+    int Nsmax = 0;         // max(Nx, Ny): row length of xi0
+    float Bz = 0, hz = 0;  // d zeta / d iu1; half height of a bin's footprint in voxels
+    bool any_xs = false;   // some view takes x planes (reads the transposed volume)
+    GenView* views = nullptr;   // [Nv1][Nv0]
+    float* xi0 = nullptr;       // [Nv0][Nsmax]
+    float4* pcoef = nullptr;    // [Nv0] PETL back: {Cx, Cy, C0, h0}: u0 centre = x Cx + y Cy + C0 (bins)
+    void* block = nullptr;
+    ~GenPlan();
+};
+
+std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V);
+
+// fT = f with x and y swapped ([z][x][y]); the forward reads x-plane views from it so that the footprint
+// axis is the contiguous one.
+void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_t s);
+
+// g = A f (+ epilogue: EPI_RATIO g = resp y / (resp A f), EPI_RESID g = (resp A f - y) w), v1 rows
+// v1_first, v1_first + v1_stride, ...; fT: launch_transpose_xy(f) when gp.any_xs, else unused.
+void gen_forward(const GenPlan& gp, float* g, const float* f, const float* fT, int epilogue, const float* y,
+                 const float* resp, const float* wts, hipStream_t s, int v1_first = 0, int v1_stride = 1);
+// f += A^T g in mode BACK_PETL or BACK_MATCHED (v1 rows as above).
+void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first = 0,
+              int v1_stride = 1);
+
+}  // namespace petl

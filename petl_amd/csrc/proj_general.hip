@@ -1,0 +1,461 @@
+// General SF projector (any psi, any pitch): per-view FP64-built crossing tables, interval-overlap
+// weights shared by the forward and its exact transpose, and the reference's voxel-driven PETL-mode back
+// projector.  See proj_general.h for the model.
+//
+// Kernels:
+//   gen_fwd_kernel          a workgroup is one view (v1, v0) x a tile of 64 u0 bins (lanes) x 4 u1 rows
+//                           (waves); each thread walks the slices for its bin, reading the volume through
+//                           the layout whose footprint axis is contiguous (f for y planes, the transposed
+//                           copy for x planes), so the lanes of a wave read neighbouring voxels;
+//   gen_back_matched_kernel one thread per voxel gathers, per view, the bins whose footprint it meets
+//                           (candidate bin windows from the inverted crossings, widened by one bin, every
+//                           tap re-evaluated with the forward's own functions): no atomics;
+//   gen_back_petl_kernel    one thread per (x, y) column and ZC heights: per view row v1 the voxel-centre
+//                           u1 test and u1 interval per height, per v0 the u0 interval of the column
+//                           (independent of the height), exact bilinear weights.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+#include "proj_general.h"
+#include "proj_kernels.h"
+
+namespace petl {
+
+namespace {
+
+struct GenArgs {
+    PlanGeom P;
+    VolGeom V;
+    int Nsmax;
+    float Bz, hz;
+    const GenView* views;
+    const float* xi0;
+    const float4* pcoef;
+    int v1_first, v1_stride, v1_chunk;
+};
+
+__device__ __forceinline__ float coord(int i, float T, float s)
+{
+#pragma clang fp contract(off)
+    return float(i) * T + s;
+}
+
+// Forward masks of the reference (src/projectors.cu:30-43), evaluated in float.
+__device__ __forceinline__ bool mask_u0(const PlanGeom& P, float u0, float v0)
+{
+#pragma clang fp contract(off)
+    return fabsf(u0) - 0.5f * P.Tu0 > P.L + P.R * fabsf(v0);
+}
+__device__ __forceinline__ bool mask_u1(const PlanGeom& P, float u1, float v1)
+{
+#pragma clang fp contract(off)
+    return fabsf(u1) - 0.5f * P.Tu1 > P.H - P.R * fabsf(v1);
+}
+
+// overlap length of [c - h, c + h] with the unit cell [i - 1/2, i + 1/2] (negative when they are apart)
+__device__ __forceinline__ float overlap(float c, float h, float i)
+{
+    return fminf(c + h, i + 0.5f) - fmaxf(c - h, i - 0.5f);
+}
+
+// The crossings of a bin's line with slice s, in voxel-index units.  The forward and the matched back
+// projector call exactly these, so they compute identical weights.
+__device__ __forceinline__ float cross_a(const GenView& w, float xs0, int iu0) { return fmaf(w.Au, float(iu0), xs0); }
+__device__ __forceinline__ float cross_z_base(const GenView& w, int iu0, int s)
+{
+    return fmaf(w.Bu, float(iu0), fmaf(w.Bs, float(s), w.Z0));
+}
+__device__ __forceinline__ float cross_z(float Bz, float zb, int iu1) { return fmaf(Bz, float(iu1), zb); }
+
+// ---------------------------------------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) gen_fwd_kernel(float* __restrict__ g, const float* __restrict__ f,
+                                                      const float* __restrict__ fT, GenArgs A, int epilogue,
+                                                      const float* __restrict__ y, const float* __restrict__ resp,
+                                                      const float* __restrict__ wts)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int nt0 = (P.Nu0 + 63) >> 6;
+    const int t1 = blockIdx.x / nt0, t0 = blockIdx.x - t1 * nt0;
+    const int iu0 = t0 * 64 + (threadIdx.x & 63), iu1 = t1 * 4 + (threadIdx.x >> 6);
+    const int iv0 = blockIdx.y, iv1 = A.v1_first + (A.v1_chunk + int(blockIdx.z)) * A.v1_stride;
+    if (iu0 >= P.Nu0 || iu1 >= P.Nu1)
+        return;  // no barrier follows
+    const int64_t gi = ((int64_t(iv1) * P.Nv0 + iv0) * P.Nu1 + iu1) * P.Nu0 + iu0;
+    const float v0 = coord(iv0, P.Tv0, P.v0_0), v1 = coord(iv1, P.Tv1, P.v1_0);
+    float out = 0.0f;
+    if (!mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0) && !mask_u1(P, coord(iu1, P.Tu1, P.u1_0), v1)) {
+        const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
+        const int Na = w.xs ? V.Ny : V.Nx, Ns = w.xs ? V.Nx : V.Ny;
+        const float* vol = w.xs ? fT : f;
+        const int64_t sk = int64_t(V.Nx) * V.Ny;
+        const float* xr = A.xi0 + int64_t(iv0) * A.Nsmax;
+        const float hb = w.hb, hz = A.hz;
+        float acc = 0.0f;
+        for (int s = 0; s < Ns; s++) {
+            const float xi = cross_a(w, xr[s], iu0);
+            const int ilo = max(0, int(floorf(xi - hb - 0.5f))), ihi = min(Na - 1, int(ceilf(xi + hb + 0.5f)));
+            if (ilo > ihi)
+                continue;
+            const float ze = cross_z(A.Bz, cross_z_base(w, iu0, s), iu1);
+            const int klo = max(0, int(floorf(ze - hz - 0.5f))), khi = min(V.Nz - 1, int(ceilf(ze + hz + 0.5f)));
+            const float* vs = vol + int64_t(s) * Na;
+            for (int k = klo; k <= khi; k++) {
+                const float wz = overlap(ze, hz, float(k));
+                if (!(wz > 0.0f))
+                    continue;
+                const float* vk = vs + int64_t(k) * sk;
+                for (int i = ilo; i <= ihi; i++) {
+                    const float wa = overlap(xi, hb, float(i));
+                    if (wa > 0.0f)
+                        acc += vk[i] * (wa * wz);
+                }
+            }
+        }
+        out = acc * w.scale;
+    }
+    {
+#pragma clang fp contract(off)
+        if (resp)
+            out = out * resp[gi];
+        if (epilogue == EPI_RATIO) {
+            // dataCube RDIV (src/data_cube.cpp:150-155): y / Pf, 0 where Pf == 0
+            out = (out == 0.0f) ? 0.0f : y[gi] / out;
+            if (resp)
+                out = out * resp[gi];
+        } else if (epilogue == EPI_RESID) {
+            // Pf - g, weighted in place (src/petl_server.py:1790-1792)
+            out = out - y[gi];
+            if (wts)
+                out = out * wts[gi];
+        }
+    }
+    g[gi] = out;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// matched back projection: the exact transpose of gen_fwd_kernel, gathered per voxel.  f += A^T g.
+// ---------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) gen_back_matched_kernel(const float* __restrict__ g, float* __restrict__ f,
+                                                               GenArgs A)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63), iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int iz = blockIdx.z;
+    if (ix >= V.Nx || iy >= V.Ny)
+        return;
+    const float hz = A.hz, Bz = A.Bz, fz = float(iz);
+    const int64_t row = int64_t(P.Nu1) * P.Nu0;
+    float sum = 0.0f;
+    for (int iv1 = A.v1_first; iv1 < P.Nv1; iv1 += A.v1_stride) {
+        const float v1 = coord(iv1, P.Tv1, P.v1_0);
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
+            const int i = w.xs ? iy : ix, s = w.xs ? ix : iy;
+            const float fi = float(i), hb = w.hb;
+            const float xs0 = A.xi0[int64_t(iv0) * A.Nsmax + s];
+            // bins whose a crossing can lie within hb + 1/2 of this voxel (inverted, one bin of margin)
+            float lo = (fi - hb - 0.5f - xs0) / w.Au, hi = (fi + hb + 0.5f - xs0) / w.Au;
+            if (lo > hi) {
+                const float t = lo;
+                lo = hi;
+                hi = t;
+            }
+            const int b0 = max(0, int(floorf(lo)) - 1), b1 = min(P.Nu0 - 1, int(ceilf(hi)) + 1);
+            if (b0 > b1)
+                continue;
+            const float v0 = coord(iv0, P.Tv0, P.v0_0);
+            const float* gv = g + (int64_t(iv1) * P.Nv0 + iv0) * row;
+            float acc = 0.0f;
+            for (int iu0 = b0; iu0 <= b1; iu0++) {
+                const float wa = overlap(cross_a(w, xs0, iu0), hb, fi);
+                if (!(wa > 0.0f) || mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0))
+                    continue;
+                const float zb = cross_z_base(w, iu0, s);
+                const int r0 = max(0, int(floorf((fz - hz - 0.5f - zb) / Bz)) - 1);
+                const int r1 = min(P.Nu1 - 1, int(ceilf((fz + hz + 0.5f - zb) / Bz)) + 1);
+                for (int iu1 = r0; iu1 <= r1; iu1++) {
+                    const float wz = overlap(cross_z(Bz, zb, iu1), hz, fz);
+                    if (!(wz > 0.0f) || mask_u1(P, coord(iu1, P.Tu1, P.u1_0), v1))
+                        continue;
+                    acc += gv[int64_t(iu1) * P.Nu0 + iu0] * (wa * wz);
+                }
+            }
+            sum += acc * w.scale;
+        }
+    }
+    f[(int64_t(iz) * V.Ny + iy) * V.Nx + ix] += sum;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// PETL-mode back projection (src/projectors.cu:240-377 semantics, exact bilinear weights).  f += B g.
+// ---------------------------------------------------------------------------------------------------------
+template <int ZC>
+__global__ void __launch_bounds__(256) gen_back_petl_kernel(const float* __restrict__ g, float* __restrict__ f,
+                                                            GenArgs A)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63), iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = blockIdx.z * ZC;
+    if (ix >= V.Nx || iy >= V.Ny)
+        return;
+    float tcol;
+    {
+#pragma clang fp contract(off)
+        // the voxel centre's signed distance along the panel normal direction (src/projectors.cu:284)
+        tcol = coord(iy, V.Ty, V.y0) * P.cos_psi - coord(ix, V.Tx, V.x0) * P.sin_psi;
+    }
+    const float x = coord(ix, V.Tx, V.x0), yv = coord(iy, V.Ty, V.y0);
+    const float hz1 = 0.5f * V.Tz / P.Tu1;
+    const int64_t row = int64_t(P.Nu1) * P.Nu0;
+    float acc[ZC];
+#pragma unroll
+    for (int j = 0; j < ZC; j++)
+        acc[j] = 0.0f;
+    for (int iv1 = A.v1_first; iv1 < P.Nv1; iv1 += A.v1_stride) {
+        const float v1 = coord(iv1, P.Tv1, P.v1_0);
+        float A1[ZC], B1[ZC];
+        int q0[ZC], q1[ZC];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < ZC; j++) {
+            float u1;
+            bool live;
+            {
+#pragma clang fp contract(off)
+                u1 = coord(z0 + j, V.Tz, V.z0) + v1 * tcol;
+                live = (z0 + j < V.Nz) && !(fabsf(u1) - 0.5f * P.Tu1 > P.H - P.R * fabsf(v1));
+            }
+            const float mid = (u1 - P.u1_0) / P.Tu1;
+            A1[j] = mid - hz1;
+            B1[j] = mid + hz1;
+            q0[j] = max(0, int(ceilf(A1[j] - 0.5f)));
+            q1[j] = live ? min(P.Nu1 - 1, int(floorf(B1[j] + 0.5f))) : -1;
+            any = any || (live && q0[j] <= q1[j]);
+        }
+        if (!any)
+            continue;
+        const GenView* wr = A.views + int64_t(iv1) * P.Nv0;
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const float4 pc = A.pcoef[iv0];
+            const float l = wr[iv0].lpetl;
+            const float c0 = fmaf(x, pc.x, fmaf(yv, pc.y, pc.z));
+            const float A0 = c0 - pc.w, B0 = c0 + pc.w;
+            const int p0 = max(0, int(ceilf(A0 - 0.5f))), p1 = min(P.Nu0 - 1, int(floorf(B0 + 0.5f)));
+            const float* gv = g + (int64_t(iv1) * P.Nv0 + iv0) * row;
+            for (int iu0 = p0; iu0 <= p1; iu0++) {
+                const float w0 = fmaxf(0.0f, fminf(float(iu0) + 0.5f, B0) - fmaxf(float(iu0) - 0.5f, A0));
+                if (!(w0 > 0.0f))
+                    continue;
+                const float wl = w0 * l;
+                const float* gc = gv + iu0;
+#pragma unroll
+                for (int j = 0; j < ZC; j++) {
+                    for (int iu1 = q0[j]; iu1 <= q1[j]; iu1++) {
+                        const float w1 = fmaxf(0.0f, fminf(float(iu1) + 0.5f, B1[j]) - fmaxf(float(iu1) - 0.5f, A1[j]));
+                        if (w1 > 0.0f)
+                            acc[j] += gc[int64_t(iu1) * P.Nu0] * (wl * w1);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < ZC; j++)
+        if (z0 + j < V.Nz)
+            f[(int64_t(z0 + j) * V.Ny + iy) * V.Nx + ix] += V.Tx * acc[j];
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// fT[z][x][y] = f[z][y][x], 32 x 32 tiles through LDS
+// ---------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) transpose_xy_kernel(const float* __restrict__ f, float* __restrict__ fT,
+                                                           int Nx, int Ny)
+{
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int64_t zo = int64_t(blockIdx.z) * Nx * Ny;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int x = bx + tx, y = by + r;
+        if (x < Nx && y < Ny)
+            tile[r][tx] = f[zo + int64_t(y) * Nx + x];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int y = by + tx, x = bx + r;
+        if (x < Nx && y < Ny)
+            fT[zo + int64_t(x) * Ny + y] = tile[tx][r];
+    }
+}
+
+inline float coord_host(int i, float T, float s)
+{
+    volatile float a = float(i) * T;
+    return a + s;
+}
+
+GenArgs args_of(const GenPlan& gp, int v1_first, int v1_stride)
+{
+    GenArgs A;
+    A.P = gp.P;
+    A.V = gp.V;
+    A.Nsmax = gp.Nsmax;
+    A.Bz = gp.Bz;
+    A.hz = gp.hz;
+    A.views = gp.views;
+    A.xi0 = gp.xi0;
+    A.pcoef = gp.pcoef;
+    A.v1_first = v1_first;
+    A.v1_stride = v1_stride;
+    A.v1_chunk = 0;
+    return A;
+}
+
+}  // namespace
+
+GenPlan::~GenPlan()
+{
+    if (block)
+        (void)hipFree(block);
+}
+
+std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
+{
+    auto gp = std::make_unique<GenPlan>();
+    gp->P = P;
+    gp->V = V;
+    gp->Nsmax = std::max(V.Nx, V.Ny);
+    gp->Bz = float(double(P.Tu1) / double(V.Tz));
+    gp->hz = float(0.5 * double(P.Tu1) / double(V.Tz));
+    const double cs = P.cos_psi, sn = P.sin_psi;  // the float values the reference works with
+    const double Tu0 = P.Tu0, Tu1 = P.Tu1, Tz = V.Tz;
+    std::vector<GenView> views(size_t(P.Nv1) * P.Nv0);
+    std::vector<float> xi0(size_t(P.Nv0) * gp->Nsmax, 0.0f);
+    std::vector<float4> pcoef(P.Nv0);
+    struct PerV0 {
+        int xs;
+        double pa, ps, rs, Ta, S0, Ts, dom_inv;
+    };
+    std::vector<PerV0> pv(P.Nv0);
+    for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+        const float v0f = coord_host(iv0, P.Tv0, P.v0_0);
+        // slice axis: the reference's float decision (src/projectors.cu:77)
+        volatile float t1 = v0f * P.cos_psi;
+        volatile float t2 = v0f * P.sin_psi;
+        const float rxf = -(t1 + P.sin_psi), ryf = -(t2 - P.cos_psi);
+        const int xs = std::fabs(rxf) >= std::fabs(ryf) ? 1 : 0;
+        const double v0 = v0f;
+        const double rx = -(v0 * cs + sn), ry = -(v0 * sn - cs);
+        PerV0& q = pv[iv0];
+        q.xs = xs;
+        double ra, a0, Na;
+        if (!xs) {
+            q.pa = cs, q.ps = sn, ra = rx, q.rs = ry, a0 = V.x0, q.Ta = V.Tx, q.S0 = V.y0, q.Ts = V.Ty, Na = V.Nx;
+        } else {
+            q.pa = sn, q.ps = cs, ra = ry, q.rs = rx, a0 = V.y0, q.Ta = V.Ty, q.S0 = V.x0, q.Ts = V.Tx, Na = V.Ny;
+        }
+        (void)Na;
+        const int Ns = xs ? V.Nx : V.Ny;
+        // a crossing of bin u0 with plane S:  a_c = u0 (p_a - p_s r_a / r_s) + S r_a / r_s
+        const double ka = q.pa - q.ps * ra / q.rs, ks = ra / q.rs;
+        for (int s = 0; s < Ns; s++) {
+            const double S = q.S0 + double(s) * q.Ts;
+            xi0[size_t(iv0) * gp->Nsmax + s] = float((ka * double(P.u0_0) + ks * S - a0) / q.Ta);
+        }
+        const float Au = float(ka * Tu0 / q.Ta), hb = float(0.5 * Tu0 * std::fabs(q.pa) / q.Ta);
+        // PETL back: the column's u0 centre x w_x + y w_y and the half width of the voxel mid-line's
+        // projection along the dominant of w_y, w_x (decided in float, src/projectors.cu:314-347)
+        volatile float a1 = v0f * P.cos_psi;
+        volatile float a2 = v0f * P.sin_psi;
+        const float wyf = a1 + P.sin_psi, wxf = P.cos_psi - a2;
+        const double wy = v0 * cs + sn, wx = cs - v0 * sn;
+        const bool ydom = std::fabs(wyf) > std::fabs(wxf);
+        const double h0 = ydom ? 0.5 * double(V.Ty) * std::fabs(wy) / Tu0 : 0.5 * double(V.Tx) * std::fabs(wx) / Tu0;
+        pcoef[iv0] = make_float4(float(wx / Tu0), float(wy / Tu0), float(-double(P.u0_0) / Tu0), float(h0));
+        q.dom_inv = 1.0 / std::max(std::fabs(wy), std::fabs(wx));
+        for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
+            GenView& w = views[size_t(iv1) * P.Nv0 + iv0];
+            w.xs = xs;
+            w.Au = Au;
+            w.hb = hb;
+        }
+        gp->any_xs = gp->any_xs || xs;
+    }
+    for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
+        const double v1 = coord_host(iv1, P.Tv1, P.v1_0);
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const PerV0& q = pv[iv0];
+            const double v0 = coord_host(iv0, P.Tv0, P.v0_0);
+            GenView& w = views[size_t(iv1) * P.Nv0 + iv0];
+            // z crossing: z_c = u1 + (r_z / r_s) (S - u0 p_s), r_z = -v1
+            const double kz = -v1 / q.rs;
+            w.Z0 = float((double(P.u1_0) - double(V.z0) + kz * q.S0 - kz * q.ps * double(P.u0_0)) / Tz);
+            w.Bs = float(kz * q.Ts / Tz);
+            w.Bu = float(-kz * q.ps * Tu0 / Tz);
+            const double obl = 1.0 / std::sqrt(1.0 + v0 * v0 + v1 * v1);
+            // slice length T_x / |p_a| (src/projectors.cu:84,138) over the footprint normalisations
+            // T_u0 |p_a| / T_a and T_u1 / T_z, times the obliquity
+            w.scale = float(double(V.Tx) * q.Ta * Tz * obl / (Tu0 * q.pa * q.pa * Tu1));
+            w.lpetl = float(q.dom_inv * obl);
+        }
+    }
+    const size_t nv = views.size() * sizeof(GenView), nx = xi0.size() * sizeof(float),
+                 np = pcoef.size() * sizeof(float4);
+    auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
+    char* blk = nullptr;
+    PETL_HIP_CHECK(hipMalloc(&blk, al(nv) + al(nx) + al(np)));
+    gp->block = blk;
+    gp->views = reinterpret_cast<GenView*>(blk);
+    gp->xi0 = reinterpret_cast<float*>(blk + al(nv));
+    gp->pcoef = reinterpret_cast<float4*>(blk + al(nv) + al(nx));
+    PETL_HIP_CHECK(hipMemcpy(gp->views, views.data(), nv, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(gp->xi0, xi0.data(), nx, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(gp->pcoef, pcoef.data(), np, hipMemcpyHostToDevice));
+    return gp;
+}
+
+void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_t s)
+{
+    hipLaunchKernelGGL(transpose_xy_kernel, dim3((V.Nx + 31) / 32, (V.Ny + 31) / 32, V.Nz), dim3(256), 0, s, f, fT,
+                       V.Nx, V.Ny);
+    PETL_LAUNCH_CHECK();
+}
+
+void gen_forward(const GenPlan& gp, float* g, const float* f, const float* fT, int epilogue, const float* y,
+                 const float* resp, const float* wts, hipStream_t s, int v1_first, int v1_stride)
+{
+    if (gp.any_xs && !fT)
+        throw hip_error("general forward: the transposed volume is missing");
+    GenArgs A = args_of(gp, v1_first, v1_stride);
+    const PlanGeom& P = gp.P;
+    const int tiles = ((P.Nu0 + 63) / 64) * ((P.Nu1 + 3) / 4);
+    const int nv1 = (P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    for (int c = 0; c < nv1; c += 65535) {
+        A.v1_chunk = c;
+        hipLaunchKernelGGL(gen_fwd_kernel, dim3(tiles, P.Nv0, std::min(65535, nv1 - c)), dim3(256), 0, s, g, f, fT, A,
+                           epilogue, y, resp, wts);
+        PETL_LAUNCH_CHECK();
+    }
+}
+
+void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first, int v1_stride)
+{
+    const GenArgs A = args_of(gp, v1_first, v1_stride);
+    const VolGeom& V = gp.V;
+    if (mode == BACK_MATCHED) {
+        hipLaunchKernelGGL(gen_back_matched_kernel, dim3((V.Nx + 63) / 64, (V.Ny + 3) / 4, V.Nz), dim3(256), 0, s, g,
+                           f, A);
+    } else {
+        constexpr int ZC = 4;
+        hipLaunchKernelGGL(gen_back_petl_kernel<ZC>, dim3((V.Nx + 63) / 64, (V.Ny + 3) / 4, (V.Nz + ZC - 1) / ZC),
+                           dim3(256), 0, s, g, f, A);
+    }
+    PETL_LAUNCH_CHECK();
+}
+
+}  // namespace petl

@@ -56,7 +56,26 @@ struct Slot {
     bool fetching = false;
     std::string sig;                       // geometry and MLEM parameters the resources were built for
     size_t bytes = 0;                      // device bytes the resources hold
+    float cap_sy = 0.0f, cap_sf = 0.0f;    // binary16 data / image scales the captured iteration holds
 };
+
+// Captures one iteration of the slot's job as a graph (the binary16 scales of the request set up last
+// become kernel arguments of the graph).
+void capture_iteration(Slot& s)
+{
+    if (s.exec)
+        PETL_HIP_CHECK(hipGraphExecDestroy(s.exec));
+    if (s.graph)
+        PETL_HIP_CHECK(hipGraphDestroy(s.graph));
+    s.exec = nullptr;
+    s.graph = nullptr;
+    PETL_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+    s.job->iterate(0, s.stream);
+    PETL_HIP_CHECK(hipStreamEndCapture(s.stream, &s.graph));
+    PETL_HIP_CHECK(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+    s.cap_sy = s.job->data_scale();
+    s.cap_sf = s.job->image_scale();
+}
 
 // Requests with equal signatures can run on the same model, buffers and captured iteration.
 std::string signature(const petl_recon_request& r)
@@ -217,6 +236,7 @@ public:
         out[8] = t_capture_;
         out[9] = t_release_;
         out[10] = double(pool_hits_);
+        out[11] = double(recaptures_);
     }
 
 private:
@@ -345,8 +365,16 @@ private:
                 std::fill(s.pinned, s.pinned + V, 1.0f);
                 s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
                 PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
+                const Clock::time_point t2 = Clock::now();
+                // binary16 requests pick their scales from their data: a graph captured with another
+                // request's scales is captured again
+                const bool recapture = s.job->data_scale() != s.cap_sy || s.job->image_scale() != s.cap_sf;
+                if (recapture)
+                    capture_iteration(s);
                 std::lock_guard<std::mutex> lk(mu_);
-                t_setup_ += seconds(t1, Clock::now());
+                t_setup_ += seconds(t1, t2);
+                t_capture_ += seconds(t2, Clock::now());
+                recaptures_ += recapture ? 1 : 0;
             } catch (const std::exception& e) {
                 fprintf(stderr, "PETL service: request %d failed at admission: %s\n", req->id, e.what());
                 (void)hipStreamSynchronize(s.stream);
@@ -383,10 +411,7 @@ private:
             s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
             PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
             const Clock::time_point t2 = Clock::now();
-            PETL_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-            s.job->iterate(0, s.stream);
-            PETL_HIP_CHECK(hipStreamEndCapture(s.stream, &s.graph));
-            PETL_HIP_CHECK(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+            capture_iteration(s);
             const Clock::time_point t3 = Clock::now();
             std::lock_guard<std::mutex> lk(mu_);
             t_alloc_ += seconds(t0, t1);
@@ -540,6 +565,7 @@ private:
     std::vector<Slot> pool_;  // resources of finished requests, oldest first
     size_t pooled_ = 0;       // device bytes they hold
     long pool_hits_ = 0;
+    long recaptures_ = 0;  // pooled iterations captured again for other binary16 scales
     std::thread admit_thread_, launch_thread_;
 };
 
@@ -585,6 +611,22 @@ double petl_service_request_bytes(const petl_recon_request* request)
         auto m = petl::make_model(*request);
         return m ? double(petl::MlemJob::device_bytes(*m, petl::params_of(*request))) : 0.0;
     }, 0.0);
+}
+
+bool petl_service_request_footprint(const petl_recon_request* request, double* out)
+{
+    if (!request || !out)
+        return false;
+    return petl::guarded_call("petl_service_request_footprint", [&] {
+        auto m = petl::make_model(*request);
+        if (!m)
+            return false;
+        petl::MlemJob job(*m, petl::params_of(*request));
+        out[0] = double(petl::MlemJob::device_bytes(*m, petl::params_of(*request)));
+        job.allocate();
+        out[1] = double(job.allocated_bytes());
+        return true;
+    }, false);
 }
 
 bool petl_service_wait(void* service, int id, float* f, double* times)

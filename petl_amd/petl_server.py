@@ -100,6 +100,10 @@ _SIGNATURES = {
                                        ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_bool]),
     "backproject_gpu_ex": (ctypes.c_bool, [ctypes.POINTER(ctypes.c_void_p), _c_float_p, ctypes.c_int, ctypes.c_float,
                                            ctypes.c_void_p, ctypes.c_bool]),
+    "project_gpu_batch": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _c_float_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_bool]),
+    "backproject_gpu_batch": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _c_float_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_bool]),
     "MLEM_gpu_storage": (ctypes.c_bool, [ctypes.POINTER(_c_float_p), _f32, ctypes.c_int, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]),
@@ -440,6 +444,27 @@ class PETLserver:
             ptrs = (ctypes.c_void_p * len(g))(*[x.data_ptr() for x in g])
             ok = self.libprojectors.backproject_gpu_ex(ptrs, fp, sg, float(g_scale), s, sync)
         if not ok:
+            raise TypeError('backproject failed')
+        return f
+
+    def project_gpu_batch(self, g, f, stream=None, sync=True):
+        """Forward projection of a batch: f is a (B, nz, ny, nx) float32 device tensor, g a list with one
+        (B, N_v1, N_v0, N_u1, N_u0) float32 device tensor per planogram (all contiguous)."""
+        self.set_model()
+        s = None if stream is None else ctypes.c_void_p(stream)
+        fp = ctypes.cast(ctypes.c_void_p(f.data_ptr()), _c_float_p)
+        if not self.libprojectors.project_gpu_batch(self._dev_ptrs(g), fp, int(f.shape[0]), s, sync):
+            raise TypeError('project failed')
+        return g
+
+    def backproject_gpu_batch(self, g, f, mode=None, stream=None, sync=True):
+        """Back projection of a batch (shapes as project_gpu_batch) into f, overwritten; mode 'petl' or
+        'matched' for this call only (None = the server's mode)."""
+        self.set_model()
+        s = None if stream is None else ctypes.c_void_p(stream)
+        m = -1 if mode is None else {"petl": 0, "matched": 1}[mode]
+        fp = ctypes.cast(ctypes.c_void_p(f.data_ptr()), _c_float_p)
+        if not self.libprojectors.backproject_gpu_batch(self._dev_ptrs(g), fp, int(f.shape[0]), m, s, sync):
             raise TypeError('backproject failed')
         return f
 

@@ -46,6 +46,8 @@ def _declare(lib):
     lib.petl_service_wait.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_float_p, ctypes.POINTER(ctypes.c_double)]
     lib.petl_service_stats.restype = ctypes.c_bool
     lib.petl_service_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    lib.petl_service_request_footprint.restype = ctypes.c_bool
+    lib.petl_service_request_footprint.argtypes = [ctypes.POINTER(ReconRequest), ctypes.POINTER(ctypes.c_double)]
     lib.petl_service_destroy.restype = None
     lib.petl_service_destroy.argtypes = [ctypes.c_void_p]
     return lib
@@ -102,6 +104,15 @@ class ReconService:
         r, _ = make_request(**kw)
         return self.lib.petl_service_request_bytes(ctypes.byref(r))
 
+    def request_footprint(self, **kw):
+        """(bytes the admission counts, bytes the request's buffers hold once allocated): allocates the
+        request's buffers on the device and frees them again."""
+        r, _ = make_request(**kw)
+        out = (ctypes.c_double * 2)()
+        if not self.lib.petl_service_request_footprint(ctypes.byref(r), out):
+            raise RuntimeError("petl_service_request_footprint failed")
+        return out[0], out[1]
+
     def submit(self, **kw):
         r, keep = make_request(**kw)
         rid = self.lib.petl_service_submit(self.h, ctypes.byref(r))
@@ -126,11 +137,11 @@ class ReconService:
         return f, (t[0], t[1], t[2])
 
     def stats(self):
-        s = (ctypes.c_double * 11)()
+        s = (ctypes.c_double * 12)()
         self.lib.petl_service_stats(self.h, s)
         return dict(completed=int(s[0]), failed=int(s[1]), peak_active=int(s[2]), peak_reserved_bytes=s[3],
                     budget_bytes=s[4], queued=int(s[5]), admit_alloc_s=s[6], admit_setup_s=s[7],
-                    admit_capture_s=s[8], release_s=s[9], pool_hits=int(s[10]))
+                    admit_capture_s=s[8], release_s=s[9], pool_hits=int(s[10]), recaptures=int(s[11]))
 
     def close(self):
         if self.h:

@@ -170,3 +170,30 @@ def test_gradient_ragged_geometry_matches_oracle(oracle):
         ref = oracle.backproject(plans, vol, r, mode="matched")
         got = f.grad[b].cpu().numpy()
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-4
+
+
+def test_batched_launch_equals_per_volume_calls(setup):
+    """The batch axis of the fast kernels' grids computes every element as its own call would, bit for bit,
+    for the forward and for both back projector modes; the backward's mode is an argument of the call and
+    the server's own mode is left alone."""
+    s, ft, _, _ = setup
+    shapes = [tuple(int(x) for x in s.get_planogramSize(i)) for i in range(2)]
+    gb = [torch.empty((B,) + sh, device=DEV) for sh in shapes]
+    s.project_gpu_batch(gb, ft)
+    gen = torch.Generator(device=DEV).manual_seed(71)
+    r = [torch.rand((B,) + sh, generator=gen, device=DEV) for sh in shapes]
+    s.set_backprojector_mode("petl")
+    for mode in ("matched", "petl"):
+        fb = torch.empty_like(ft)
+        s.backproject_gpu_batch(r, fb, mode=mode)
+        assert s.get_backprojector_mode() == "petl"
+        s.set_backprojector_mode(mode)
+        for b in range(B):
+            one = [torch.empty(sh, device=DEV) for sh in shapes]
+            s.project_gpu(one, ft[b].contiguous())
+            for x, y in zip(one, gb):
+                assert torch.equal(x, y[b])
+            fo = torch.empty_like(ft[b])
+            s.backproject_gpu([x[b].contiguous() for x in r], fo)
+            assert torch.equal(fo, fb[b])
+        s.set_backprojector_mode("petl")

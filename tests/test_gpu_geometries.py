@@ -1,0 +1,131 @@
+"""The reference's own geometries that are not the benchmark's, on the GPU against the fp32 oracle.
+
+* The PET/X preset (src/petx_server.py:8-11 with set_default_volume, src/parameters.cpp:205-248): a psi = 0
+  planogram of 251.52 x 156.72 mm panels 101 mm apart and a psi = 90 planogram of 125.12 x 156.72 mm panels
+  266.6 mm apart, 1.6 mm crystals, the 156 x 62 x 96 volume at 1.6 mm.  The psi = 0 planogram's in-plane
+  axis has 156 voxels, which the fast kernels take with their wide instances; both planograms also run on
+  the general kernels (forced).
+* The rotating-panel demo geometry (demo/d01_set_geometry.py:21-22): six planograms at psi = 0, 30, ..., 150
+  degrees, D = 180, L = 192, H = 144 mm, 2 mm crystals, v_m0 = v_m1 = tan 15 degrees, its default volume.
+  psi = 0 and 90 run the fast kernels, the four oblique planograms the general ones.
+
+The full projections take the oracle minutes, so it checks sampled v1 rows (the first, a middle and the
+last row, where the masks act), forward and PETL-mode back, to the float32 parity bar of 1e-5 relative L2;
+the adjoint identity of the matched back projector holds over the whole geometry to 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from petl_amd.petl_server import PETLserver  # noqa: E402
+from petl_amd.workloads import ellipsoid_phantom  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+PETX = [(0.0, 101.0, 251.52, 156.72, 1.6, None, None), (90.0, 266.6, 125.12, 156.72, 1.6, None, None)]
+T15 = math.tan(15.0 * math.pi / 180.0)
+ROTATING = [(n * 180.0 / 6, 180.0, 192.0, 144.0, 2.0, T15, T15) for n in range(6)]
+GEOMS = {"petx": PETX, "rotating": ROTATING}
+
+
+def make(oracle, name, path="auto"):
+    s = PETLserver()
+    s.print_warnings = False
+    for psi, D, L, H, T, vm0, vm1 in GEOMS[name]:
+        s.add_planogram(psi, D, L, H, T, vm0, vm1)
+    s.set_default_volume()
+    s.set_projector_path(path)
+    plans = [oracle.planogram(psi, D, L, H, T, vm0, vm1) for psi, D, L, H, T, vm0, vm1 in GEOMS[name]]
+    vol = oracle.volume(s.numX(), s.numY(), s.numZ(), s.voxelWidth(), s.voxelHeight())
+    shapes = [tuple(int(x) for x in s.get_planogramSize(i)) for i in range(len(plans))]
+    assert shapes == [p.shape for p in plans]
+    return s, plans, vol, shapes
+
+
+def rows_of(p):
+    return sorted({0, p.N_v1 // 2, p.N_v1 - 1})
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))
+
+
+def test_geometry_paths(oracle):
+    s, plans, vol, _ = make(oracle, "petx")
+    assert vol.shape == (96, 62, 156)
+    assert s.uses_fast_path(0) and s.uses_fast_path(1)
+    s, plans, vol, _ = make(oracle, "rotating")
+    assert [s.uses_fast_path(i) for i in range(6)] == [True, False, False, True, False, False]
+
+
+@pytest.fixture(scope="module")
+def references(oracle):
+    """Per geometry: the phantom, the oracle forward of the sampled rows, and for the PETL-mode back
+    projection the planograms that are random on the sampled rows and zero elsewhere with the oracle's
+    back projection of them."""
+    out = {}
+    for name in GEOMS:
+        _, plans, vol, _ = make(oracle, name)
+        nz, ny, nx = vol.shape
+        f = ellipsoid_phantom(nx, float(vol.voxelWidth), 8, seed=5, shape=(nz, ny, nx))
+        rng = np.random.default_rng(17)
+        fwd, gb = [], []
+        back = np.zeros(vol.shape, np.float32)
+        for p in plans:
+            ref = np.zeros(p.shape, np.float32)
+            host = np.zeros(p.shape, np.float32)
+            for r in rows_of(p):
+                oracle.project_rows(p, vol, f, ref, r, r + 1)
+                host[r] = rng.random(p.shape[1:], dtype=np.float32)
+                oracle.backproject_petl_rows(p, vol, host, back, r, r + 1)
+            fwd.append({r: ref[r].copy() for r in rows_of(p)})
+            gb.append(host)
+            del ref
+        out[name] = (f, fwd, gb, back)
+    return out
+
+
+@pytest.mark.parametrize("path", ["auto", "generic"])
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_sampled_rows_forward(oracle, references, name, path):
+    s, plans, vol, shapes = make(oracle, name, path)
+    f, fwd, _, _ = references[name]
+    g = [torch.empty(sh, device=DEV) for sh in shapes]
+    s.project_gpu(g, torch.from_numpy(f).to(DEV))
+    for i, (p, gi) in enumerate(zip(plans, g)):
+        for r, ref in fwd[i].items():
+            got = gi[r].cpu().numpy()
+            assert rel(got, ref) <= 1e-5, (i, r)
+            assert float(np.abs(got[ref == 0.0]).max(initial=0.0)) <= 1e-6 * float(ref.max())
+
+
+@pytest.mark.parametrize("path", ["auto", "generic"])
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_sampled_rows_petl_back(oracle, references, name, path):
+    s, plans, vol, shapes = make(oracle, name, path)
+    s.set_backprojector_mode("petl")
+    _, _, gb, back = references[name]
+    out = torch.empty(vol.shape, device=DEV)
+    s.backproject_gpu([torch.from_numpy(x).to(DEV) for x in gb], out)
+    assert rel(out.cpu().numpy(), back) <= 1e-5
+
+
+@pytest.mark.parametrize("path", ["auto", "generic"])
+@pytest.mark.parametrize("name", list(GEOMS))
+def test_adjoint_whole_geometry(oracle, name, path):
+    s, plans, vol, shapes = make(oracle, name, path)
+    s.set_backprojector_mode("matched")
+    gen = torch.Generator(device=DEV).manual_seed(18)
+    x = torch.rand(vol.shape, generator=gen, device=DEV)
+    y = [torch.rand(sh, generator=gen, device=DEV) for sh in shapes]
+    Ax = [torch.empty(sh, device=DEV) for sh in shapes]
+    s.project_gpu(Ax, x)
+    ATy = torch.empty(vol.shape, device=DEV)
+    s.backproject_gpu(y, ATy)
+    lhs = sum(float(torch.dot(a.reshape(-1).double(), b.reshape(-1).double())) for a, b in zip(Ax, y))
+    rhs = float(torch.dot(x.reshape(-1).double(), ATy.reshape(-1).double()))
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)

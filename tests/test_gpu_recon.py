@@ -134,14 +134,21 @@ def test_mlem_invariants_on_gpu(oracle):
     assert np.all(np.diff(lls) >= -1e-7 * np.abs(lls[1:]))
 
 
-def test_osem_generic_matches_oracle(oracle):
-    """Interleaved v1 subsets, per-subset sensitivity: the generic kernels share the oracle's float
-    arithmetic, so OSEM on Poisson data follows it to 1e-4."""
-    s, plans, vol, y = panel32(oracle, (0.0, 90.0))
+@pytest.mark.parametrize("nsub", [2, 4])
+def test_osem_generic_matches_oracle(oracle, nsub):
+    """Interleaved v1 subsets on the general kernels (forced on this geometry), from the support start of
+    the fast-kernel test below (from a uniform start the OSEM iterates of grazing edge voxels amplify
+    rounding, as described there)."""
+    s, plans, vol, _ = panel32(oracle, (0.0, 90.0))
     s.set_projector_path("generic")
-    got = s.OSEM(y, None, 2, 4)
-    ref = oracle.osem(plans, vol, y, 2, 4)
-    assert rel(got, ref) <= 1e-4
+    assert not any(s.uses_fast_path(i) for i in range(2))
+    zz, yy, xx = np.meshgrid(*(np.arange(32) - 15.5,) * 3, indexing="ij")
+    support = ((xx ** 2 + yy ** 2 <= 144.0) & (np.abs(zz) <= 12.0)).astype(np.float32)
+    f_true = (ellipsoid_phantom(32, 2.0, 6, seed=1) + np.float32(0.05)) * support
+    y = oracle.project(plans, vol, f_true)
+    got = s.OSEM(y, support.copy(), 2, nsub)
+    ref = oracle.osem(plans, vol, y, 2, nsub, f0=support)
+    assert rel(got, ref) <= 1e-5
 
 
 @pytest.mark.parametrize("nsub", [2, 4, 8])

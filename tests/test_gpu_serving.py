@@ -20,6 +20,8 @@ GEOS = {
     "p32": dict(psis=(0.0,), D=100.0, L=64.0, H=64.0, T=2.0, n=32, voxel=2.0),
     "p48": dict(psis=(0.0, 90.0), D=200.0, L=96.0, H=96.0, T=2.0, n=48, voxel=2.0),
     "p64": dict(psis=(0.0, 90.0), D=200.0, L=128.0, H=128.0, T=2.0, n=64, voxel=2.0),
+    "p96": dict(psis=(0.0, 90.0), D=200.0, L=192.0, H=192.0, T=2.0, n=96, voxel=2.0),
+    "p128": dict(psis=(0.0, 90.0), D=200.0, L=256.0, H=256.0, T=2.0, n=128, voxel=2.0),
 }
 
 
@@ -176,3 +178,59 @@ def test_bf16_storage_requests_match_mlem_storage():
         got, got16 = svc.wait(a)[0], svc.wait(b)[0]
     np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(got16, want16)
+
+
+def test_f16_pooled_request_with_other_scales():
+    """Two binary16 requests with the same geometry and parameters but counts 100x apart: the second reuses
+    the first one's pooled resources, picks its own data and image scales, and must still give the image
+    MLEM gives alone (the captured iteration holds the scales, so it is captured again)."""
+    y_lo = data_for("p48", 41, counts=1e5)
+    y_hi = data_for("p48", 42, counts=1e7)
+    want = [server_for(GEOS["p48"]).MLEM(y, None, 3, storage="f16") for y in (y_lo, y_hi)]
+    with ReconService(max_slots=4) as svc:
+        got = []
+        for y in (y_lo, y_hi):
+            rid = svc.submit(**kw("p48", y, 3, storage="f16"))
+            got.append(svc.wait(rid)[0])
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["pool_hits"] == 1 and st["recaptures"] == 1
+
+
+@pytest.mark.parametrize("storage,image", [("f32", "f32"), ("bf16", "f32"), ("bf16", "bf16"), ("f16", "f16")])
+def test_admission_footprint_covers_the_buffers(storage, image):
+    """The bytes admission counts for a request cover what its buffers hold once allocated, with no more
+    than the fixed allowance for kernel tables and reduction scratch on top."""
+    y = data_for("p64", 51)
+    with ReconService(max_slots=1) as svc:
+        counted, held = svc.request_footprint(**kw("p64", y, 2, storage=storage, image_storage=image))
+    assert held <= counted <= held + (1 << 20) + 0.02 * held
+
+
+def test_poisson_arrivals_over_all_panel_sizes():
+    """A seeded Poisson-arrival stream (20 requests/s) of requests with 48, 64, 96 and 128-crystal panels and
+    matched volumes, counts log-uniform in [1e5, 1e8], 2-5 iterations, some with the relative-difference
+    prior: requests join while others run, and every image is bit-identical to MLEM run alone."""
+    rng = np.random.default_rng(77)
+    names = ["p48", "p64", "p96", "p128"]
+    cases = []
+    for k in range(12):
+        name = names[k % 4] if k < 4 else names[int(rng.integers(4))]
+        counts = float(10.0 ** rng.uniform(5.0, 8.0))
+        extra = dict(prior="rd", beta=0.01, delta=2.0) if rng.random() < 0.3 else {}
+        cases.append((name, 60 + k, int(rng.integers(2, 6)), counts, extra))
+    gaps = rng.exponential(1.0 / 20.0, size=len(cases))
+    want = [alone(n, data_for(n, sd, c), it, **ex) for (n, sd, it, c, ex) in cases]
+    with ReconService(max_slots=8) as svc:
+        ids = []
+        for gap, (n, sd, it, c, ex) in zip(gaps, cases):
+            time.sleep(gap)
+            ids.append(svc.submit(**kw(n, data_for(n, sd, c), it, **ex)))
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for (n, *_), g, w in zip(cases, got, want):
+        assert g.shape == (GEOS[n]["n"],) * 3
+        np.testing.assert_array_equal(g, w)
+    assert st["completed"] == len(cases) and st["failed"] == 0
+    assert st["peak_active"] >= 2
