@@ -15,6 +15,8 @@ Prints one JSON line.  Besides the headline rate it carries:
                 committed SQ counters
   lds_roofline  the same kernel's LDS read bandwidth (instruction count from the committed SQ counters)
   storage_forms the same pair with bfloat16 / binary16 volume and planograms (fp32 arithmetic)
+  geometries    the same pair at the PET/X preset (fast kernels, and the general kernels forced) and at the
+                six-planogram rotating-panel demo geometry (four oblique planograms on the general kernels)
   binning       list-mode binning of 16.8M device-resident coincidences into both planograms
   batched_autograd  8 volumes of 64^3 through the autograd projector, forward + matched backward
   response      scintillator stopping power of every bin through 256 box modules (ray tracing)
@@ -54,8 +56,8 @@ def parse():
     ap.add_argument("--pair-only", action="store_true",
                     help="measure only the projection pair and its 16-bit storage forms (for profiling runs)")
     ap.add_argument("--sections", default="all",
-                    help="comma list of the measurements after the projection pair: binning, autograd, response, "
-                         "recon, small, serving, cpu (default all)")
+                    help="comma list of the measurements after the projection pair: geometries, binning, autograd, "
+                         "response, recon, small, serving, cpu (default all)")
     ap.add_argument("--serve-rates", default="5,20,50",
                     help="request arrival rates (req/s), comma separated: below, near and above the service's "
                          "capacity")
@@ -399,6 +401,53 @@ def main():
         }
         del g16, f16
     out["storage_forms"] = storage_forms
+
+    if want("geometries"):
+        # the reference's other geometries: the PET/X preset (psi 0 / 90 panels at 1.6 mm, its default
+        # 156 x 62 x 96 volume) and the six-planogram rotating-panel demo (psi = 0, 30, ..., 150; four oblique
+        # planograms on the general kernels); fwd + PETL back pairs/s on device buffers, and the PET/X preset
+        # with the general kernels forced
+        import math
+        t15 = math.tan(15.0 * math.pi / 180.0)
+        geoms = {
+            "petx": [(0.0, 101.0, 251.52, 156.72, 1.6, None, None), (90.0, 266.6, 125.12, 156.72, 1.6, None, None)],
+            "rotating6": [(k * 30.0, 180.0, 192.0, 144.0, 2.0, t15, t15) for k in range(6)],
+        }
+        res = {}
+        for gname, planos in geoms.items():
+            for path in (("auto", "general") if gname == "petx" else ("auto",)):
+                sg = PETLserver()
+                sg.print_warnings = False
+                for psi, D_, L_, H_, T_, m0, m1 in planos:
+                    sg.add_planogram(psi, D_, L_, H_, T_, m0, m1)
+                sg.set_default_volume()
+                sg.set_projector_path("generic" if path == "general" else "auto")
+                vs = (sg.numZ(), sg.numY(), sg.numX())
+                gsh = [tuple(int(x) for x in sg.get_planogramSize(i)) for i in range(sg.get_numPlanograms())]
+                fg = torch.from_numpy(ellipsoid_phantom(vs[2], sg.voxelWidth(), 8, seed=5, shape=vs)).to(dev)
+                gg = [torch.empty(sh, dtype=torch.float32, device=dev) for sh in gsh]
+                fo = torch.empty_like(fg)
+                for _ in range(2):
+                    sg.project_gpu(gg, fg, stream=stream, sync=False)
+                    sg.backproject_gpu(gg, fo, stream=stream, sync=False)
+                eg = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                reps = 5
+                torch.cuda.synchronize()
+                eg[0].record()
+                for _ in range(reps):
+                    sg.project_gpu(gg, fg, stream=stream, sync=False)
+                eg[1].record()
+                for _ in range(reps):
+                    sg.backproject_gpu(gg, fo, stream=stream, sync=False)
+                eg[2].record()
+                torch.cuda.synchronize()
+                fms, bms = eg[0].elapsed_time(eg[1]) / reps, eg[1].elapsed_time(eg[2]) / reps
+                res[f"{gname}_{path}"] = {
+                    "planograms": [list(x) for x in gsh], "volume": list(vs),
+                    "fast_kernels": [sg.uses_fast_path(i) for i in range(len(gsh))],
+                    "forward_call_ms": fms, "back_call_ms": bms, "pairs_per_s": 1e3 / (fms + bms)}
+                del fg, gg, fo
+        out["geometries"] = res
 
     if want("binning"):
         # list-mode binning: device-resident coincidences between the two panel pairs into both planograms

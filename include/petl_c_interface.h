@@ -251,6 +251,14 @@ typedef struct petl_recon_request {
 /* hbm_budget_bytes: device memory the service may hold for requests; max_slots: requests on the device
  * at once.  Returns NULL on failure. */
 PETL_EXTERN void* petl_service_create(double hbm_budget_bytes, int max_slots);
+/* petl_service_create with flags: PETL_SERVICE_BATCHED runs the requests that allow it (fp32 MLEM, one
+ * subset, at most two planograms on the fast kernels with up to 255 u0 bins and 128-voxel in-plane and
+ * slice axes, PETL-mode back projection) in batched persistent iterations: one captured graph of six
+ * persistent kernels advances every such request on the device by one iteration, whatever its size, with
+ * work items taken from device queues; the others run their own captured iteration as without the flag.
+ * No reference counterpart (the reference has no service). */
+#define PETL_SERVICE_BATCHED 1
+PETL_EXTERN void* petl_service_create_ex(double hbm_budget_bytes, int max_slots, int flags);
 /* Queues a request (the struct is copied; the data arrays are read when the request is admitted).
  * Returns its id (>= 0), or -1 when the request is invalid or can never fit the budget. */
 PETL_EXTERN int petl_service_submit(void* service, const petl_recon_request* request);
@@ -264,12 +272,13 @@ PETL_EXTERN bool petl_service_request_footprint(const petl_recon_request* reques
  * 3 doubles): seconds from submission to admission, from admission to completion, and in total.
  * Returns false for an unknown id or a failed request. */
 PETL_EXTERN bool petl_service_wait(void* service, int id, float* f, double* times);
-/* stats (12 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
+/* stats (14 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
  * requests currently queued, the host seconds spent so far admitting requests (allocation, data upload
  * with sensitivities, graph capture) and freeing resources, how many requests reused the pooled
  * resources (model, buffers, captured iteration) of a finished request with the same geometry and MLEM
  * parameters, and how many of those captured the iteration again because their binary16 data or image
- * scale differed.  Pooled memory counts against the budget until a new request needs it. */
+ * scale differed, how many requests ran in the batched iterations, and how many batched iterations were
+ * launched.  Pooled memory counts against the budget until a new request needs it. */
 PETL_EXTERN bool petl_service_stats(void* service, double* stats);
 /* Finishes every queued request, then frees the service. */
 PETL_EXTERN void petl_service_destroy(void* service);

@@ -5,6 +5,7 @@
 #include <vector>
 #include "petl_model.h"
 #include "storage.h"
+#include "mlem_batch.h"
 
 namespace petl {
 
@@ -46,6 +47,14 @@ public:
     // holds them as kernel arguments, so it can be replayed for another request only when they are equal
     float data_scale() const { return s_y_; }
     float image_scale() const { return s_f_; }
+    // whether the job can run in batched persistent iterations (MlemBatch): fp32 MLEM (one subset, no mask,
+    // response or log-likelihood) in PETL mode on fast plans that MlemBatch takes; call after allocate()
+    bool batch_eligible();
+    // iteration `it` of the job as a batched-iteration entry (after setup()).  With a prior the image
+    // alternates between two buffers (the update reads one and writes the other); batch_image(n) is the
+    // buffer that holds the image after n batched iterations.
+    BatchIterJob batch_job(int it);
+    const float* batch_image(int n);
     // device bytes the job's buffers hold now (planogram and volume buffers, back projector workspace,
     // staging; not the kernel tables)
     size_t allocated_bytes() const;

@@ -321,3 +321,52 @@ void MlemJob::fetch(float* f, double* loglik, hipStream_t s)
 }
 
 }  // namespace petl
+
+namespace petl {
+
+bool MlemJob::batch_eligible()
+{
+    if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32 || prm_.nsub != 1 || prm_.mask ||
+        prm_.response || prm_.loglik || m_.back_mode != BACK_PETL)
+        return false;
+    m_.prepare();
+    std::vector<const FastPlan*> plans;
+    for (size_t i = 0; i < m_.plans.size(); i++)
+        plans.push_back(m_.dev.fast[i].get());
+    return MlemBatch::eligible(plans);
+}
+
+static bool pingpong(const MlemParams& prm) { return prm.prior != 0 && prm.beta > 0.0f; }
+
+BatchIterJob MlemJob::batch_job(int it)
+{
+    DeviceState& D = m_.dev;
+    BatchIterJob j;
+    j.nplans = int(m_.plans.size());
+    for (int p = 0; p < j.nplans; p++) {
+        j.plan[p] = D.fast[p].get();
+        j.ratio[p] = D.aux[p].f();
+        j.data[p] = D.y[p].f();
+    }
+    const bool pp = pingpong(prm_);
+    j.f_in = (pp && it % 2 == 1) ? D.f2.f() : D.f.f();
+    j.f_out = pp ? ((it % 2 == 0) ? D.f2.f() : D.f.f()) : D.f.f();
+    j.d = D.d.f();
+    j.S = D.S.f();
+    j.h2 = D.h2.f();
+    j.prior = prm_.prior;
+    j.delta = prm_.delta;
+    j.beta = prm_.beta;
+    j.p = prm_.p;
+    j.Nx = m_.numX;
+    j.Ny = m_.numY;
+    j.Nz = m_.numZ;
+    return j;
+}
+
+const float* MlemJob::batch_image(int n)
+{
+    return (pingpong(prm_) && n % 2 == 1) ? m_.dev.f2.f() : m_.dev.f.f();
+}
+
+}  // namespace petl

@@ -33,6 +33,8 @@
 #include "proj_kernels.h"
 #include "proj_fast.h"
 #include "storage.h"
+#include "mlem_batch.h"
+#include "mlem_update_dev.h"
 
 namespace petl {
 
@@ -220,15 +222,14 @@ __device__ __forceinline__ void epi_load(EpiIn<C>& in, const FwdArgs& A, int epi
 // the volume and SG of the planogram and the data y.  EK: the compile-time epilogue (EK_*); the residual
 // and sum-of-squares forms are separate instances so that the projection kernels keep no weight pointer
 // live.
-template <int J, int C, int SB, int R, int SF, int SG, int EK = EK_RUNTIME, int NAM = 128>
-__global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
-                                                               const typename Stor<SF>::T* __restrict__ f,
-                                                               FwdArgs A, const int2* __restrict__ ftab,
-                                                               const float* __restrict__ fwp,
-                                                               const int4* __restrict__ ztab,
-                                                               const float* __restrict__ fscale, int epilogue,
-                                                               const typename Stor<SG>::T* __restrict__ yv,
-                                                               const float* __restrict__ resp)
+// The work of one workgroup: views grp * 56 ..., u1 rows iu1_0 ... iu1_0 + R - 1, view row iv1 of the
+// planogram g (already offset to its batch element); part_idx: the sum-of-squares partial it writes.
+template <int J, int C, int SB, int R, int SF, int SG, int EK, int NAM>
+__device__ __forceinline__ void fwd_body(typename Stor<SG>::T* __restrict__ g, const typename Stor<SF>::T* __restrict__ f,
+                                         const FwdArgs& A, const int2* __restrict__ ftab, const float* __restrict__ fwp,
+                                         const int4* __restrict__ ztab, const float* __restrict__ fscale, int epilogue,
+                                         const typename Stor<SG>::T* __restrict__ yv, const float* __restrict__ resp,
+                                         const int grp, const int iu1_0, const int iv1, const int64_t part_idx)
 {
     using TS = typename Stor<SG>::T;
     constexpr bool RES = EK == EK_RESID, SQ = EK == EK_SUMSQ;
@@ -240,12 +241,6 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // one workgroup per (group of 56 views, u1 row, v1); a renumbering that gives each XCD a contiguous
-    // range of u1 rows (so resident workgroups share a band of the volume in L2) measured 30 % slower
-    const int bz = blockIdx.z, bel = bz / A.nv1;
-    const int grp = blockIdx.x, iu1_0 = blockIdx.y * R, iv1 = A.v1_first + (bz - bel * A.nv1) * A.v1_stride;
-    g += bel * A.g_bstride;
-    f += bel * A.f_bstride;
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz;
@@ -297,7 +292,7 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
             }
         }
         if (SQ && tid == 0)
-            A.part[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = 0.0;
+            A.part[part_idx] = 0.0;
         return;
     }
 
@@ -501,9 +496,29 @@ __global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>
             double t = 0.0;
             for (int w = 0; w < FWD_WAVES; w++)
                 t += s_sq[w];
-            A.part[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+            A.part[part_idx] = t;
         }
     }
+}
+
+template <int J, int C, int SB, int R, int SF, int SG, int EK = EK_RUNTIME, int NAM = 128>
+__global__ void __launch_bounds__(FWD_THREADS) fwd_fast_kernel(typename Stor<SG>::T* __restrict__ g,
+                                                               const typename Stor<SF>::T* __restrict__ f,
+                                                               FwdArgs A, const int2* __restrict__ ftab,
+                                                               const float* __restrict__ fwp,
+                                                               const int4* __restrict__ ztab,
+                                                               const float* __restrict__ fscale, int epilogue,
+                                                               const typename Stor<SG>::T* __restrict__ yv,
+                                                               const float* __restrict__ resp)
+{
+    // one workgroup per (group of 56 views, u1 row, v1 row of a batch element); a renumbering that gives
+    // each XCD a contiguous range of u1 rows (so resident workgroups share a band of the volume in L2)
+    // measured 30 % slower
+    const int bz = blockIdx.z, bel = bz / A.nv1;
+    fwd_body<J, C, SB, R, SF, SG, EK, NAM>(g + bel * A.g_bstride, f + bel * A.f_bstride, A, ftab, fwp, ztab, fscale,
+                                           epilogue, yv, resp, blockIdx.x, blockIdx.y * R,
+                                           A.v1_first + (bz - bel * A.nv1) * A.v1_stride,
+                                           (int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
 }
 
 // =============================================================================================
@@ -535,22 +550,17 @@ struct BackArgs {
 // number of bins) land as raw dwords in a staging array and one pass per block widens them into the row
 // buffer.  Otherwise the block is read through registers and scaled (and, in matched mode, masked per
 // bin) on its way into LDS.
-template <int AG, int ST, bool DMA = false, bool MASKED = false>
-__global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
-                                                      float* __restrict__ h2, BackArgs A,
-                                                      const int2* __restrict__ xtab, int xrow,
-                                                      const float* __restrict__ scale,
-                                                      const int2* __restrict__ rows)
+// The work of one workgroup: a group ag of AG in-plane columns and slice group sgr (BACK_SG slices, one per
+// lane) of u1 row iu1 and view row iv1; g and h2 already offset to their batch element.
+template <int AG, int ST, bool DMA, bool MASKED>
+__device__ __forceinline__ void back_h2_body(const typename Stor<ST>::T* __restrict__ g, float* __restrict__ h2,
+                                             const BackArgs& A, const int2* __restrict__ xtab, int xrow,
+                                             const float* __restrict__ scale, const int2* __restrict__ rows,
+                                             const int ag, const int sgr, const int iu1, const int iv1)
 {
     extern __shared__ float sbuf[];
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int bz = blockIdx.z, bel = bz / A.nv1;
-    // blockIdx.x = slice group x a group: a workgroup takes BACK_SG slices (one per lane) of one a group
-    const int sgr = blockIdx.x / A.nagroups, ag = blockIdx.x - sgr * A.nagroups;
-    const int iu1 = blockIdx.y, iv1 = A.v1_first + (bz - bel * A.nv1) * A.v1_stride;
-    g += bel * A.g_bstride;
-    h2 += bel * A.h_bstride;
     const PlanGeom& P = A.P;
     const int Nv0 = P.Nv0, Nu0 = P.Nu0, Nu1 = P.Nu1, Ns = A.Ns, Na = A.Na;
     const int2 rr = rows[iv1];
@@ -793,21 +803,35 @@ __global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T
     }
 }
 
+template <int AG, int ST, bool DMA = false, bool MASKED = false>
+__global__ void __launch_bounds__(256) back_h2_kernel(const typename Stor<ST>::T* __restrict__ g,
+                                                      float* __restrict__ h2, BackArgs A,
+                                                      const int2* __restrict__ xtab, int xrow,
+                                                      const float* __restrict__ scale,
+                                                      const int2* __restrict__ rows)
+{
+    // blockIdx.x = slice group x a group; blockIdx.z = batch element x view row
+    const int bz = blockIdx.z, bel = bz / A.nv1;
+    const int sgr = blockIdx.x / A.nagroups, ag = blockIdx.x - sgr * A.nagroups;
+    back_h2_body<AG, ST, DMA, MASKED>(g + bel * A.g_bstride, h2 + bel * A.h_bstride, A, xtab, xrow, scale, rows, ag,
+                                      sgr, blockIdx.y, A.v1_first + (bz - bel * A.nv1) * A.v1_stride);
+}
+
 // =============================================================================================
 // back projection, stage 2: u1 gather over all v1 rows, per voxel
 // =============================================================================================
+// The work of one wave: in-plane columns 64 * ac ... of slice is, heights z0 ... z0 + ZC - 1; h2 and f
+// already offset to their batch element.  overwrite: f = sum instead of f += sum (the value is the same
+// when f holds zeros: 0 + x = x for the non-negative sums of a back projection of non-negative data, and
+// for any sum that is not -0).
 template <int ZC>
-__global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2, float* __restrict__ f, BackArgs A,
-                                                    const int4* __restrict__ ztab, const int2* __restrict__ rows)
+__device__ __forceinline__ void back_z_body(const float* __restrict__ h2, float* __restrict__ f, const BackArgs& A,
+                                            const int4* __restrict__ ztab, const int2* __restrict__ rows,
+                                            const int lane, const int ac, const int is, const int z0,
+                                            const bool overwrite)
 {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x;
-    const int ia = blockIdx.x * 64 + lane;
-    const int is = blockIdx.y;
-    const int bel = blockIdx.z / A.nzc;
-    const int z0 = (blockIdx.z - bel * A.nzc) * ZC;
-    h2 += bel * A.h_bstride;
-    f += bel * A.f_bstride;
+    const int ia = ac * 64 + lane;
     const PlanGeom& P = A.P;
     const VolGeom& V = A.V;
     const int Ns = A.Ns, Na = A.Na, Nz = A.Nz, Nu1 = P.Nu1;
@@ -860,9 +884,22 @@ __global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2
         float* fp = f + int64_t(is) * A.fs_s + int64_t(ia) * A.fs_a;
 #pragma unroll
         for (int i = 0; i < ZC; i++)
-            if (z0 + i < Nz)
-                fp[int64_t(z0 + i) * A.fs_z] += acc[i];
+            if (z0 + i < Nz) {
+                if (overwrite)
+                    fp[int64_t(z0 + i) * A.fs_z] = acc[i];
+                else
+                    fp[int64_t(z0 + i) * A.fs_z] += acc[i];
+            }
     }
+}
+
+template <int ZC>
+__global__ void __launch_bounds__(64) back_z_kernel(const float* __restrict__ h2, float* __restrict__ f, BackArgs A,
+                                                    const int4* __restrict__ ztab, const int2* __restrict__ rows)
+{
+    const int bel = blockIdx.z / A.nzc;
+    back_z_body<ZC>(h2 + bel * A.h_bstride, f + bel * A.f_bstride, A, ztab, rows, threadIdx.x, blockIdx.x, blockIdx.y,
+                    (blockIdx.z - bel * A.nzc) * ZC, false);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1197,15 +1234,10 @@ std::unique_ptr<FastPlan> make_fast_plan(const PlanGeom& P, const VolGeom& V)
     return fp;
 }
 
-template <int SF, int SG>
-static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
-                           const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
-                           float y_scale, float g_scale, const float* wts = nullptr, int ek = -1,
-                           double* part = nullptr, int batch = 1, int64_t f_bstride = 0, int64_t g_bstride = 0)
+// kernel arguments of a plan (storage scales 1, no batch, no weights)
+static FwdArgs fwd_args(const FastPlan& fp, int v1_first, int v1_stride)
 {
-    using TG = typename Stor<SG>::T;
-    using TF = typename Stor<SF>::T;
-    FwdArgs A;
+    FwdArgs A{};
     A.P = fp.P;
     A.Ns = fp.Ns;
     A.Na = fp.Na;
@@ -1219,6 +1251,53 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.v1_first = v1_first;
     A.v1_stride = v1_stride;
     A.xrow = fp.xrow;
+    A.f_scale = A.y_scale = A.g_inv_scale = 1.0f;
+    A.wts = nullptr;
+    A.part = nullptr;
+    A.nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    A.f_bstride = A.g_bstride = 0;
+    return A;
+}
+
+static BackArgs back_args(const FastPlan& fp, int mode, int v1_first, int v1_stride)
+{
+    BackArgs A{};
+    A.P = fp.P;
+    A.V = fp.V;
+    A.Ns = fp.Ns;
+    A.Na = fp.Na;
+    A.Nz = fp.Nz;
+    A.fs_s = fp.fs_s;
+    A.fs_a = fp.fs_a;
+    A.fs_z = fp.fs_z;
+    A.spad_lo = fp.spad_lo;
+    A.srow = fp.srow;
+    A.mode = mode;
+    A.psi90 = fp.psi90;
+    A.sb = fp.SB;
+    A.hrow = fp.hrow;
+    A.v1_first = v1_first;
+    A.v1_stride = v1_stride;
+    A.g_scale = 1.0f;
+    A.nagroups = fp.nagroups;
+    A.nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
+    A.nzc = (fp.Nz + zc - 1) / zc;
+    A.g_bstride = plan_elems(fp.P);
+    A.h_bstride = int64_t(fast_back_workspace_bytes(fp) / sizeof(float));
+    A.f_bstride = int64_t(fp.V.Nx) * fp.V.Ny * fp.V.Nz;
+    return A;
+}
+
+template <int SF, int SG>
+static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilogue, const void* y,
+                           const float* resp, hipStream_t s, int v1_first, int v1_stride, float f_scale,
+                           float y_scale, float g_scale, const float* wts = nullptr, int ek = -1,
+                           double* part = nullptr, int batch = 1, int64_t f_bstride = 0, int64_t g_bstride = 0)
+{
+    using TG = typename Stor<SG>::T;
+    using TF = typename Stor<SF>::T;
+    FwdArgs A = fwd_args(fp, v1_first, v1_stride);
     A.f_scale = (SF == STORE_F16) ? f_scale : 1.0f;
     A.y_scale = (SG == STORE_F16) ? y_scale : 1.0f;
     A.g_inv_scale = (SG == STORE_F16) ? 1.0f / g_scale : 1.0f;
@@ -1226,8 +1305,7 @@ static void fast_forward_t(const FastPlan& fp, void* g, const void* f, int epilo
     A.part = part;
     if (ek < 0)
         ek = (epilogue == EPI_RESID) ? EK_RESID : EK_RUNTIME;
-    const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    A.nv1 = nv1;
+    const int nv1 = A.nv1;
     A.f_bstride = f_bstride;
     A.g_bstride = g_bstride;
     if (int64_t(nv1) * batch > 65535)
@@ -1303,37 +1381,13 @@ static void fast_back_t(const FastPlan& fp, const void* g, float* f, int mode, f
                         int v1_first, int v1_stride, float g_scale, int batch = 1)
 {
     using TS = typename Stor<ST>::T;
-    BackArgs A;
-    A.P = fp.P;
-    A.V = fp.V;
-    A.Ns = fp.Ns;
-    A.Na = fp.Na;
-    A.Nz = fp.Nz;
-    A.fs_s = fp.fs_s;
-    A.fs_a = fp.fs_a;
-    A.fs_z = fp.fs_z;
-    A.spad_lo = fp.spad_lo;
-    A.srow = fp.srow;
-    A.mode = mode;
-    A.psi90 = fp.psi90;
-    A.sb = fp.SB;
-    A.v1_first = v1_first;
-    A.v1_stride = v1_stride;
+    BackArgs A = back_args(fp, mode, v1_first, v1_stride);
     A.g_scale = g_scale;
-    const int nv1 = (fp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
-    const int zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
-    const int nzc = (fp.Nz + zc - 1) / zc;
-    A.nv1 = nv1;
-    A.nzc = nzc;
-    A.g_bstride = plan_elems(fp.P);
-    A.h_bstride = int64_t(fast_back_workspace_bytes(fp) / sizeof(float));
-    A.f_bstride = int64_t(fp.V.Nx) * fp.V.Ny * fp.V.Nz;
+    const int nv1 = A.nv1, nzc = A.nzc, zc = back_z_chunk(fp.Na, fp.Ns, fp.Nz);
     if (int64_t(nv1) * batch > 65535 || int64_t(nzc) * batch > 65535)
         throw hip_error("fast back: batch x v1 rows exceeds the grid");
     const int2* rows = fp.dev.rows + (mode == BACK_PETL ? fp.P.Nv1 : 0);
     const float* scale = (mode == BACK_PETL) ? fp.dev.pscale : fp.dev.fscale;
-    A.hrow = fp.hrow;
-    A.nagroups = fp.nagroups;
     const int nthr = ((std::min(fp.Ns, BACK_SG) + 63) / 64) * 64;
     const size_t lds =
         std::max(size_t(2) * BACK_RB * fp.srow, size_t(std::min(fp.Ns, BACK_SG)) * (BACK_TW + 1)) * sizeof(float);
@@ -1394,6 +1448,355 @@ void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode,
     case STORE_F16: fast_back_t<STORE_F16>(fp, g, f, mode, workspace, s, v1_first, v1_stride, g_scale); break;
     default: throw hip_error("fast back: unknown storage format");
     }
+}
+
+
+// =============================================================================================
+// batched persistent MLEM iterations (mlem_batch.h)
+// =============================================================================================
+namespace {
+
+enum BatchPhase { PH_F = 0, PH_H0, PH_Z0, PH_H1, PH_Z1, PH_U, PH_COUNT };
+constexpr int BATCH_ZC = 8;                 // z rows per back_z work item (the sums do not depend on it)
+constexpr int BATCH_FWD_LDS = 2 * 8 * FWD_HP * 4 + 128 * 16;  // row buffers + z table of Ns <= 128
+constexpr int BATCH_BACK_LDS = 40 * 1024;   // back_h2 row buffers or transpose tile, at most
+constexpr int BATCH_GRID_FWD = 1024, BATCH_GRID_H2 = 2048, BATCH_GRID_Z = 2048, BATCH_GRID_U = 1024;
+
+struct BFwd {
+    FwdArgs A;
+    float* g;
+    const float* f;
+    const int2* ftab;
+    const float* fwp;
+    const int4* ztab;
+    const float* fscale;
+    const float* y;
+    int ngroups, nu1;
+};
+
+struct BBack {
+    BackArgs A;
+    const float* g;
+    float* h2;
+    float* f;
+    const int2* xtab;
+    const float* scale;
+    const int2* rows;
+    const int4* ztab;
+    int xrow, gx, nu1, nac, nzc, overwrite;
+};
+
+struct BUpd {
+    float* fo;
+    const float* fi;
+    const float* d;
+    const float* S;
+    int N1, N2, N3, prior, nk;  // nk = 64-voxel chunks per N3 row
+    float delta, beta;
+    HuberParams h;
+};
+
+// the per-iteration table: job lists and the prefix sums of their work items
+struct BatchTable {
+    int nf, nb[BATCH_MAX_PLANOGRAMS], nu;
+    int fs[BATCH_MAX_PLANOGRAMS * BATCH_MAX_REQUESTS + 1];
+    int hs[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS + 1];
+    int zs[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS + 1];
+    int us[BATCH_MAX_REQUESTS + 1];
+    BFwd fwd[BATCH_MAX_PLANOGRAMS * BATCH_MAX_REQUESTS];
+    BBack back[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS];
+    BUpd upd[BATCH_MAX_REQUESTS];
+};
+
+// the job whose prefix range [start[j], start[j + 1]) holds item (start[0] = 0, item < start[n])
+__device__ __forceinline__ int batch_job(const int* __restrict__ start, int n, int item)
+{
+    int j = 0;
+    while (j + 1 < n && item >= start[j + 1])
+        j++;
+    return j;
+}
+
+// next work item of a workgroup-level queue (every thread gets the same uniform value)
+__device__ __forceinline__ int batch_next_block(unsigned* ctr, int* s_item)
+{
+    if (threadIdx.x == 0)
+        *s_item = int(atomicAdd(ctr, 1u));
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*s_item);
+    __syncthreads();
+    return it;
+}
+
+// next work item of a wave-level queue
+__device__ __forceinline__ int batch_next_wave(unsigned* ctr)
+{
+    int it = 0;
+    if ((threadIdx.x & 63) == 0)
+        it = int(atomicAdd(ctr, 1u));
+    return __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+}
+
+// Each phase clears the queue counter of the phase before it (finished: the launches are ordered on one
+// stream), so that every counter is zero again when its phase next runs.
+__global__ void __launch_bounds__(FWD_THREADS) batch_fwd_kernel(const BatchTable* __restrict__ T, unsigned* ctr)
+{
+    __shared__ int s_item;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctr[PH_U] = 0;
+    const int nf = T->nf, total = T->fs[nf];
+    for (;;) {
+        const int item = batch_next_block(ctr + PH_F, &s_item);
+        if (item >= total)
+            break;
+        const int j = batch_job(T->fs, nf, item);
+        const BFwd& J = T->fwd[j];
+        int l = item - T->fs[j];
+        const int grp = l % J.ngroups;
+        l /= J.ngroups;
+        const int iu1 = l % J.nu1, iv1 = l / J.nu1;
+        fwd_body<FWD_J, 4, 8, 1, STORE_F32, STORE_F32, EK_RUNTIME, 128>(J.g, J.f, J.A, J.ftab, J.fwp, J.ztab, J.fscale,
+                                                                        EPI_RATIO, J.y, nullptr, grp, iu1, iv1, 0);
+        __syncthreads();  // the next item rewrites the LDS buffers
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_h2_kernel(const BatchTable* __restrict__ T, unsigned* ctr, int p)
+{
+    __shared__ int s_item;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctr[p == 0 ? PH_F : PH_Z0] = 0;
+    const int n = T->nb[p], total = T->hs[p][n];
+    for (;;) {
+        const int item = batch_next_block(ctr + (p == 0 ? PH_H0 : PH_H1), &s_item);
+        if (item >= total)
+            break;
+        const int j = batch_job(T->hs[p], n, item);
+        const BBack& J = T->back[p][j];
+        int l = item - T->hs[p][j];
+        const int gxi = l % J.gx;
+        l /= J.gx;
+        const int iu1 = l % J.nu1, iv1 = l / J.nu1;
+        const int sgr = gxi / J.A.nagroups, ag = gxi - sgr * J.A.nagroups;
+        back_h2_body<BACK_AG, STORE_F32, true, false>(J.g, J.h2, J.A, J.xtab, J.xrow, J.scale, J.rows, ag, sgr, iu1,
+                                                      iv1);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_z_kernel(const BatchTable* __restrict__ T, unsigned* ctr, int p)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctr[p == 0 ? PH_H0 : PH_H1] = 0;
+    const int lane = threadIdx.x & 63;
+    const int n = T->nb[p], total = T->zs[p][n];
+    for (;;) {
+        const int item = batch_next_wave(ctr + (p == 0 ? PH_Z0 : PH_Z1));
+        if (item >= total)
+            break;
+        const int j = batch_job(T->zs[p], n, item);
+        const BBack& J = T->back[p][j];
+        int l = item - T->zs[p][j];
+        const int ac = l % J.nac;
+        l /= J.nac;
+        const int is = l % J.A.Ns, zc = l / J.A.Ns;
+        back_z_body<BATCH_ZC>(J.h2, J.f, J.A, J.ztab, J.rows, lane, ac, is, zc * BATCH_ZC, J.overwrite != 0);
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_update_kernel(const BatchTable* __restrict__ T, unsigned* ctr)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctr[PH_Z1] = 0;
+    const int lane = threadIdx.x & 63;
+    const int n = T->nu, total = T->us[n];
+    for (;;) {
+        const int item = batch_next_wave(ctr + PH_U);
+        if (item >= total)
+            break;
+        const int j = batch_job(T->us, n, item);
+        const BUpd& U = T->upd[j];
+        int l = item - T->us[j];
+        const int kc = l % U.nk;
+        l /= U.nk;
+        const int jj = l % U.N2, ii = l / U.N2;
+        const int k = kc * 64 + lane;
+        if (k >= U.N3)
+            continue;
+        if (U.prior == 0)
+            mlem_update_at(U.fo, U.d, U.S, (int64_t(ii) * U.N2 + jj) * U.N3 + k);
+        else if (U.prior == 1)
+            mlem_update_prior_at<0>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
+        else
+            mlem_update_prior_at<1>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
+    }
+}
+
+size_t batch_back_lds(const FastPlan& fp)
+{
+    return std::max(size_t(2) * BACK_RB * fp.srow, size_t(std::min(fp.Ns, BACK_SG)) * (BACK_TW + 1)) * sizeof(float);
+}
+
+}  // namespace
+
+struct MlemBatch::Impl {
+    static constexpr int RING = 4;     // host tables in flight
+    BatchTable* dev = nullptr;
+    unsigned* ctr = nullptr;
+    BatchTable* host[RING] = {};
+    hipEvent_t copied[RING] = {};
+    int next = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void capture()
+    {
+        PETL_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(batch_fwd_kernel, dim3(BATCH_GRID_FWD), dim3(FWD_THREADS), BATCH_FWD_LDS, cap, dev, ctr);
+        for (int p = 0; p < BATCH_MAX_PLANOGRAMS; p++) {
+            hipLaunchKernelGGL(batch_h2_kernel, dim3(BATCH_GRID_H2), dim3(256), BATCH_BACK_LDS, cap, dev, ctr, p);
+            hipLaunchKernelGGL(batch_z_kernel, dim3(BATCH_GRID_Z), dim3(256), 0, cap, dev, ctr, p);
+        }
+        hipLaunchKernelGGL(batch_update_kernel, dim3(BATCH_GRID_U), dim3(256), 0, cap, dev, ctr);
+        PETL_HIP_CHECK(hipStreamEndCapture(cap, &graph));
+        PETL_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+};
+
+MlemBatch::MlemBatch() : impl_(new Impl)
+{
+    static_assert(PH_COUNT == 6, "six phases");
+    PETL_HIP_CHECK(hipMalloc(&impl_->dev, sizeof(BatchTable)));
+    PETL_HIP_CHECK(hipMalloc(&impl_->ctr, 256));
+    PETL_HIP_CHECK(hipMemset(impl_->ctr, 0, 256));  // every queue starts empty; each phase clears its predecessor
+    for (int i = 0; i < Impl::RING; i++) {
+        PETL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&impl_->host[i]), sizeof(BatchTable)));
+        PETL_HIP_CHECK(hipEventCreateWithFlags(&impl_->copied[i], hipEventDisableTiming));
+    }
+    PETL_HIP_CHECK(hipStreamCreateWithFlags(&impl_->cap, hipStreamNonBlocking));
+    PETL_HIP_CHECK(hipDeviceSynchronize());
+    impl_->capture();
+}
+
+MlemBatch::~MlemBatch()
+{
+    if (impl_->exec)
+        (void)hipGraphExecDestroy(impl_->exec);
+    if (impl_->graph)
+        (void)hipGraphDestroy(impl_->graph);
+    for (int i = 0; i < Impl::RING; i++) {
+        if (impl_->copied[i])
+            (void)hipEventSynchronize(impl_->copied[i]);
+        if (impl_->copied[i])
+            (void)hipEventDestroy(impl_->copied[i]);
+        if (impl_->host[i])
+            (void)hipHostFree(impl_->host[i]);
+    }
+    if (impl_->dev)
+        (void)hipFree(impl_->dev);
+    if (impl_->ctr)
+        (void)hipFree(impl_->ctr);
+    delete impl_;
+}
+
+bool MlemBatch::eligible(const std::vector<const FastPlan*>& plans)
+{
+    if (plans.empty() || plans.size() > size_t(BATCH_MAX_PLANOGRAMS))
+        return false;
+    for (const FastPlan* fp : plans) {
+        if (!fp || fp->C > 4 || fp->Na > 128 || fp->Ns > 128 || fp->J != FWD_J || fp->R != 1 || fp->SB != 8)
+            return false;
+        if (batch_back_lds(*fp) > size_t(BATCH_BACK_LDS))
+            return false;
+    }
+    return true;
+}
+
+void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
+{
+    if (jobs.size() > size_t(BATCH_MAX_REQUESTS))
+        throw hip_error("batched MLEM: too many requests in one iteration");
+    Impl& I = *impl_;
+    const int slot = I.next;
+    I.next = (I.next + 1) % Impl::RING;
+    PETL_HIP_CHECK(hipEventSynchronize(I.copied[slot]));  // the copy that last read this host table is done
+    BatchTable& T = *I.host[slot];
+    T.nf = 0;
+    T.nu = 0;
+    T.fs[0] = 0;
+    T.us[0] = 0;
+    for (int p = 0; p < BATCH_MAX_PLANOGRAMS; p++) {
+        T.nb[p] = 0;
+        T.hs[p][0] = 0;
+        T.zs[p][0] = 0;
+    }
+    for (const BatchIterJob& jb : jobs) {
+        std::vector<const FastPlan*> pl(jb.plan, jb.plan + jb.nplans);
+        if (!eligible(pl))
+            throw hip_error("batched MLEM: a request outside the batched form");
+        for (int p = 0; p < jb.nplans; p++) {
+            const FastPlan& fp = *jb.plan[p];
+            BFwd& F = T.fwd[T.nf];
+            F.A = fwd_args(fp, 0, 1);
+            F.g = jb.ratio[p];
+            F.f = jb.f_in;
+            F.ftab = fp.dev.ftab;
+            F.fwp = fp.dev.fwp;
+            F.ztab = fp.dev.ztab;
+            F.fscale = fp.dev.fscale;
+            F.y = jb.data[p];
+            F.ngroups = fp.ngroups;
+            F.nu1 = fp.P.Nu1;
+            T.fs[T.nf + 1] = T.fs[T.nf] + fp.ngroups * fp.P.Nu1 * fp.P.Nv1;
+            T.nf++;
+            const int nb = T.nb[p];
+            BBack& B = T.back[p][nb];
+            B.A = back_args(fp, BACK_PETL, 0, 1);
+            B.g = jb.ratio[p];
+            B.h2 = jb.h2;
+            B.f = jb.d;
+            B.xtab = fp.dev.xtab;
+            B.scale = fp.dev.pscale;
+            B.rows = fp.dev.rows + fp.P.Nv1;
+            B.ztab = fp.dev.ztab;
+            B.xrow = fp.xrow;
+            B.gx = fp.nagroups * fp.nsgroups;
+            B.nu1 = fp.P.Nu1;
+            B.nac = (fp.Na + 63) / 64;
+            B.nzc = (fp.Nz + BATCH_ZC - 1) / BATCH_ZC;
+            B.overwrite = (p == 0) ? 1 : 0;  // d = the first planogram's sum, then += the next ones
+            T.hs[p][nb + 1] = T.hs[p][nb] + B.gx * fp.P.Nu1 * fp.P.Nv1;
+            T.zs[p][nb + 1] = T.zs[p][nb] + B.nac * fp.Ns * B.nzc;
+            T.nb[p]++;
+        }
+        BUpd& U = T.upd[T.nu];
+        U.fo = jb.f_out;
+        U.fi = jb.f_in;
+        U.d = jb.d;
+        U.S = jb.S;
+        U.N1 = jb.Nz;
+        U.N2 = jb.Ny;
+        U.N3 = jb.Nx;
+        U.nk = (jb.Nx + 63) / 64;
+        U.prior = (jb.beta > 0.0f) ? jb.prior : 0;
+        U.delta = jb.delta;
+        U.beta = jb.beta;
+        U.h = HuberParams{};
+        if (U.prior == 2) {
+            float dl = jb.delta, bt = jb.beta;
+            U.h = huber_params(dl, bt, jb.p);
+            U.delta = dl;
+            U.beta = bt;
+        }
+        if (U.prior == 0 && U.fo != U.fi)
+            throw hip_error("batched MLEM: the update without a prior is in place");
+        T.us[T.nu + 1] = T.us[T.nu] + U.nk * U.N2 * U.N1;
+        T.nu++;
+    }
+    PETL_HIP_CHECK(hipMemcpyAsync(I.dev, &T, sizeof(BatchTable), hipMemcpyHostToDevice, s));
+    PETL_HIP_CHECK(hipEventRecord(I.copied[slot], s));
+    PETL_HIP_CHECK(hipGraphLaunch(I.exec, s));
 }
 
 }  // namespace petl

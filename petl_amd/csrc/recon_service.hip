@@ -57,6 +57,8 @@ struct Slot {
     std::string sig;                       // geometry and MLEM parameters the resources were built for
     size_t bytes = 0;                      // device bytes the resources hold
     float cap_sy = 0.0f, cap_sf = 0.0f;    // binary16 data / image scales the captured iteration holds
+    bool batched = false;                  // runs in the batched persistent iterations, not its own graph
+    hipEvent_t fetched = nullptr;          // batched: the image copy to `pinned` is done
 };
 
 // Captures one iteration of the slot's job as a graph (the binary16 scales of the request set up last
@@ -138,6 +140,8 @@ void destroy_slot(Slot& s)
     for (hipEvent_t e : s.ev)
         if (e)
             (void)hipEventDestroy(e);
+    if (s.fetched)
+        (void)hipEventDestroy(s.fetched);
     s.job.reset();
     s.model.reset();  // frees the request's device buffers and kernel tables
     if (s.stream)
@@ -149,8 +153,14 @@ void destroy_slot(Slot& s)
 
 class Service {
 public:
-    Service(double budget, int max_slots) : max_slots_(std::max(1, max_slots))
+    Service(double budget, int max_slots, bool batched) : max_slots_(std::max(1, max_slots))
     {
+        if (batched) {
+            batch_ = std::make_unique<MlemBatch>();
+            PETL_HIP_CHECK(hipStreamCreateWithFlags(&bstream_, hipStreamNonBlocking));
+            for (hipEvent_t& e : round_ev_)
+                PETL_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         if (budget <= 0.0) {
             size_t free_b = 0, total_b = 0;
             PETL_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
@@ -171,6 +181,13 @@ public:
         cv_launch_.notify_all();
         admit_thread_.join();
         launch_thread_.join();
+        if (bstream_)
+            (void)hipStreamSynchronize(bstream_);
+        batch_.reset();
+        for (hipEvent_t e : round_ev_)
+            if (e)
+                (void)hipEventDestroy(e);
+        // the batch stream is left to process teardown, like the models' streams
     }
 
     int submit(const petl_recon_request& r)
@@ -237,6 +254,8 @@ public:
         out[9] = t_release_;
         out[10] = double(pool_hits_);
         out[11] = double(recaptures_);
+        out[12] = double(batched_requests_);
+        out[13] = double(batch_rounds_);
     }
 
 private:
@@ -294,7 +313,7 @@ private:
     void release(Slot& s)
     {
         const size_t bytes = s.req->bytes;
-        const bool keep = s.exec != nullptr && !s.req->failed && int(pool_.size()) < max_slots_;
+        const bool keep = (s.exec != nullptr || s.batched) && !s.req->failed && int(pool_.size()) < max_slots_;
         const Clock::time_point t0 = Clock::now();
         if (!keep)
             destroy_slot(s);
@@ -368,13 +387,15 @@ private:
                 const Clock::time_point t2 = Clock::now();
                 // binary16 requests pick their scales from their data: a graph captured with another
                 // request's scales is captured again
-                const bool recapture = s.job->data_scale() != s.cap_sy || s.job->image_scale() != s.cap_sf;
+                const bool recapture =
+                    !s.batched && (s.job->data_scale() != s.cap_sy || s.job->image_scale() != s.cap_sf);
                 if (recapture)
                     capture_iteration(s);
                 std::lock_guard<std::mutex> lk(mu_);
                 t_setup_ += seconds(t1, t2);
                 t_capture_ += seconds(t2, Clock::now());
                 recaptures_ += recapture ? 1 : 0;
+                batched_requests_ += s.batched ? 1 : 0;
             } catch (const std::exception& e) {
                 fprintf(stderr, "PETL service: request %d failed at admission: %s\n", req->id, e.what());
                 (void)hipStreamSynchronize(s.stream);
@@ -411,12 +432,18 @@ private:
             s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
             PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
             const Clock::time_point t2 = Clock::now();
-            capture_iteration(s);
+            // requests the batched persistent iterations take join them; the others replay a graph of their own
+            s.batched = batch_ && s.job->batch_eligible();
+            if (s.batched)
+                PETL_HIP_CHECK(hipEventCreateWithFlags(&s.fetched, hipEventDisableTiming));
+            else
+                capture_iteration(s);
             const Clock::time_point t3 = Clock::now();
             std::lock_guard<std::mutex> lk(mu_);
             t_alloc_ += seconds(t0, t1);
             t_setup_ += seconds(t1, t2);
             t_capture_ += seconds(t2, t3);
+            batched_requests_ += s.batched ? 1 : 0;
         } catch (const std::exception& e) {
             fprintf(stderr, "PETL service: request %d failed at admission: %s\n", req->id, e.what());
             if (s.stream) {
@@ -457,6 +484,75 @@ private:
     }
 
     // ---------------------------------------------------------------- launch thread
+    // Batched slots: one round = one iteration of every batched slot that has iterations left (up to
+    // BATCH_MAX_REQUESTS), enqueued on the batch stream; at most two rounds in flight.  A slot whose last
+    // iteration is enqueued has its image copied after that round, on the same stream.  Sets retire[i] for
+    // the batched slots that are finished.
+    bool step_batched(std::vector<Slot>& active, std::vector<char>& retire)
+    {
+        bool progress = false;
+        while (!rounds_.empty()) {
+            const hipError_t q = hipEventQuery(round_ev_[rounds_.front() % ROUND_EVENTS]);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();
+                break;
+            }
+            PETL_HIP_CHECK(q);
+            rounds_.pop_front();
+            progress = true;
+        }
+        for (size_t i = 0; i < active.size(); i++) {
+            Slot& s = active[i];
+            if (!s.batched || !s.fetching)
+                continue;
+            const hipError_t q = hipEventQuery(s.fetched);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();
+                continue;
+            }
+            PETL_HIP_CHECK(q);
+            s.req->image.assign(s.pinned, s.pinned + s.model->vol_elems());
+            finish(s.req, false);
+            retire[i] = 1;
+            progress = true;
+        }
+        if (rounds_.size() >= 2)
+            return progress;
+        std::vector<BatchIterJob> jobs;
+        std::vector<Slot*> in;
+        for (Slot& s : active) {
+            if (!s.batched || s.fetching)
+                continue;
+            if (s.launched < s.req->r.numIter && int(jobs.size()) < BATCH_MAX_REQUESTS) {
+                jobs.push_back(s.job->batch_job(s.launched));
+                in.push_back(&s);
+            } else if (s.launched >= s.req->r.numIter) {
+                in.push_back(&s);  // no iterations (left): only the image copy
+            }
+        }
+        if (in.empty())
+            return progress;
+        if (!jobs.empty()) {
+            batch_->launch(jobs, bstream_);
+            const long k = round_count_++;
+            PETL_HIP_CHECK(hipEventRecord(round_ev_[k % ROUND_EVENTS], bstream_));
+            rounds_.push_back(k);
+            std::lock_guard<std::mutex> lk(mu_);
+            batch_rounds_++;
+        }
+        for (Slot* s : in) {
+            const int n = s->req->r.numIter;
+            if (s->launched < n)
+                s->launched++;
+            if (s->launched >= n) {
+                d2h(s->pinned, s->job->batch_image(n), size_t(s->model->vol_elems()) * sizeof(float), bstream_);
+                PETL_HIP_CHECK(hipEventRecord(s->fetched, bstream_));
+                s->fetching = true;
+            }
+        }
+        return true;
+    }
+
     // returns true when the slot made progress; sets `retire` when it is finished
     bool step(Slot& s, bool& retire)
     {
@@ -515,8 +611,34 @@ private:
             }
             bool progress = false;
             std::vector<Slot> done;
-            for (size_t i = 0; i < active.size();) {
+            std::vector<char> bretire(active.size(), 0);
+            if (batch_) {
+                try {
+                    progress |= step_batched(active, bretire);
+                } catch (const std::exception& e) {
+                    // a failed batched launch fails every batched request on the device
+                    fprintf(stderr, "PETL service: batched iteration failed: %s\n", e.what());
+                    (void)hipStreamSynchronize(bstream_);
+                    (void)hipGetLastError();
+                    rounds_.clear();
+                    for (size_t i = 0; i < active.size(); i++)
+                        if (active[i].batched && !bretire[i]) {
+                            finish(active[i].req, true);
+                            bretire[i] = 1;
+                        }
+                }
+            }
+            for (size_t i = 0, k = 0; i < active.size(); k++) {
                 bool retire = false;
+                if (active[i].batched) {
+                    if (bretire[k]) {
+                        done.push_back(std::move(active[i]));
+                        active.erase(active.begin() + long(i));
+                    } else {
+                        i++;
+                    }
+                    continue;
+                }
                 try {
                     progress |= step(active[i], retire);
                 } catch (const std::exception& e) {
@@ -566,6 +688,14 @@ private:
     size_t pooled_ = 0;       // device bytes they hold
     long pool_hits_ = 0;
     long recaptures_ = 0;  // pooled iterations captured again for other binary16 scales
+    long batched_requests_ = 0, batch_rounds_ = 0;
+    // batched persistent iterations (null when the service was created without them)
+    static constexpr int ROUND_EVENTS = 4;
+    std::unique_ptr<MlemBatch> batch_;
+    hipStream_t bstream_ = nullptr;
+    hipEvent_t round_ev_[ROUND_EVENTS] = {};
+    std::deque<long> rounds_;  // batched rounds in flight (launch thread only)
+    long round_count_ = 0;
     std::thread admit_thread_, launch_thread_;
 };
 
@@ -592,7 +722,15 @@ extern "C" {
 void* petl_service_create(double hbm_budget_bytes, int max_slots)
 {
     return petl::guarded_call("petl_service_create",
-                              [&]() -> void* { return new Service(hbm_budget_bytes, max_slots); }, nullptr);
+                              [&]() -> void* { return new Service(hbm_budget_bytes, max_slots, false); }, nullptr);
+}
+
+void* petl_service_create_ex(double hbm_budget_bytes, int max_slots, int flags)
+{
+    return petl::guarded_call(
+        "petl_service_create_ex",
+        [&]() -> void* { return new Service(hbm_budget_bytes, max_slots, (flags & PETL_SERVICE_BATCHED) != 0); },
+        nullptr);
 }
 
 int petl_service_submit(void* service, const petl_recon_request* request)

@@ -4,6 +4,7 @@
 #include "petl_common.h"
 #include "vec_kernels.h"
 #include "priors_dev.h"
+#include "mlem_update_dev.h"
 #include "storage.h"
 
 namespace petl {
@@ -42,11 +43,8 @@ __global__ void sens_fix_kernel(float* __restrict__ S, int64_t n)
 __global__ void mlem_update_kernel(float* __restrict__ f, const float* __restrict__ d, const float* __restrict__ S,
                                    int64_t n)
 {
-    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT) {
-        const float s = S[i];
-        const float q = (s == 0.0f) ? 0.0f : d[i] / s;
-        f[i] = f[i] * q;
-    }
+    for (int64_t i = int64_t(blockIdx.x) * VT + threadIdx.x; i < n; i += int64_t(gridDim.x) * VT)
+        mlem_update_at(f, d, S, i);
 }
 
 // index of the i-th bin of a row subset
@@ -75,19 +73,9 @@ __global__ void __launch_bounds__(64) mlem_update_prior_kernel(float* __restrict
                                                                const float* __restrict__ S, int N1, int N2, int N3,
                                                                float delta, float beta, HuberParams h)
 {
-#pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
-    if (k >= N3)
-        return;
-    const int64_t v = (int64_t(i) * N2 + j) * N3 + k;
-    const float s = S[v];
-    float G = prior_grad_at<KIND>(f, i, j, k, N1, N2, N3, delta, beta, h);
-    G = G + s;
-    G = fmaxf(0.1f * s, G);
-    const float q = (G == 0.0f) ? 0.0f : d[v] / G;
-    fo[v] = f[v] * q;
+    if (k < N3)
+        mlem_update_prior_at<KIND>(fo, f, d, S, blockIdx.z, blockIdx.y, k, N1, N2, N3, delta, beta, h);
 }
 
 // ---- storage-format forms (16-bit image / planograms, fp32 arithmetic) ----

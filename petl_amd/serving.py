@@ -38,6 +38,8 @@ _PRIORS = {None: 0, "none": 0, "rd": 1, "tv": 2}
 def _declare(lib):
     lib.petl_service_create.restype = ctypes.c_void_p
     lib.petl_service_create.argtypes = [ctypes.c_double, ctypes.c_int]
+    lib.petl_service_create_ex.restype = ctypes.c_void_p
+    lib.petl_service_create_ex.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
     lib.petl_service_submit.restype = ctypes.c_int
     lib.petl_service_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(ReconRequest)]
     lib.petl_service_request_bytes.restype = ctypes.c_double
@@ -91,9 +93,11 @@ def make_request(psis, D, L, H, T, n, voxel, y, num_iter, v_m0=None, v_m1=None, 
 
 
 class ReconService:
-    def __init__(self, hbm_budget_bytes=0.0, max_slots=8):
+    def __init__(self, hbm_budget_bytes=0.0, max_slots=8, batched=False):
+        """batched: requests that allow it (fp32 MLEM on the fast kernels, see petl_service_create_ex) run in
+        batched persistent iterations, one launch per phase for all of them."""
         self.lib = _declare(load_library())
-        self.h = self.lib.petl_service_create(float(hbm_budget_bytes), int(max_slots))
+        self.h = self.lib.petl_service_create_ex(float(hbm_budget_bytes), int(max_slots), 1 if batched else 0)
         if not self.h:
             raise RuntimeError("petl_service_create failed")
         self._keep = {}
@@ -137,11 +141,12 @@ class ReconService:
         return f, (t[0], t[1], t[2])
 
     def stats(self):
-        s = (ctypes.c_double * 12)()
+        s = (ctypes.c_double * 14)()
         self.lib.petl_service_stats(self.h, s)
         return dict(completed=int(s[0]), failed=int(s[1]), peak_active=int(s[2]), peak_reserved_bytes=s[3],
                     budget_bytes=s[4], queued=int(s[5]), admit_alloc_s=s[6], admit_setup_s=s[7],
-                    admit_capture_s=s[8], release_s=s[9], pool_hits=int(s[10]), recaptures=int(s[11]))
+                    admit_capture_s=s[8], release_s=s[9], pool_hits=int(s[10]), recaptures=int(s[11]),
+                    batched_requests=int(s[12]), batched_rounds=int(s[13]))
 
     def close(self):
         if self.h:

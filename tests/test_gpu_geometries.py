@@ -100,7 +100,10 @@ def test_sampled_rows_forward(oracle, references, name, path):
         for r, ref in fwd[i].items():
             got = gi[r].cpu().numpy()
             assert rel(got, ref) <= 1e-5, (i, r)
-            assert float(np.abs(got[ref == 0.0]).max(initial=0.0)) <= 1e-6 * float(ref.max())
+            # bins the oracle projects to exactly zero: a footprint edge whose overlap the oracle's float
+            # arithmetic rounds to zero (its overlaps carry absolute errors of float eps x the bin index, up
+            # to 244 here, ~1.5e-5 of a bin) keeps a tiny weight in the FP64-built tables
+            assert float(np.abs(got[ref == 0.0]).max(initial=0.0)) <= 1e-5 * float(ref.max())
 
 
 @pytest.mark.parametrize("path", ["auto", "generic"])

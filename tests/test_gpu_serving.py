@@ -234,3 +234,69 @@ def test_poisson_arrivals_over_all_panel_sizes():
         np.testing.assert_array_equal(g, w)
     assert st["completed"] == len(cases) and st["failed"] == 0
     assert st["peak_active"] >= 2
+
+
+# ---------------------------------------------------------------------------------------------------------
+# batched persistent iterations (ReconService(batched=True))
+# ---------------------------------------------------------------------------------------------------------
+BATCH_MIX = [
+    ("p48", 81, 3, {}),
+    ("p128", 82, 2, dict(prior="rd", beta=0.01, delta=2.0)),
+    ("p64", 83, 5, dict(prior="rd", beta=0.01, delta=2.0)),
+    ("p96", 84, 4, {}),
+    ("p48", 85, 6, dict(prior="tv", beta=0.005, delta=0.05, p=1.0)),
+    ("p64", 86, 1, {}),
+    ("p128", 87, 3, {}),
+]
+
+
+def test_batched_iterations_match_alone_bit_exact():
+    """Requests of four panel sizes, with and without priors and with odd and even iteration counts, advance
+    together in one launch per phase; every image equals MLEM alone bit for bit."""
+    want = [alone(n, data_for(n, sd), it, **ex) for (n, sd, it, ex) in BATCH_MIX]
+    with ReconService(max_slots=8, batched=True) as svc:
+        ids = [svc.submit(**kw(n, data_for(n, sd), it, **ex)) for (n, sd, it, ex) in BATCH_MIX]
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for (n, *_), g, w in zip(BATCH_MIX, got, want):
+        assert g.shape == (GEOS[n]["n"],) * 3
+        np.testing.assert_array_equal(g, w)
+    assert st["batched_requests"] == len(BATCH_MIX) and st["failed"] == 0
+    # fewer launches of the iteration than iterations run: the requests shared rounds
+    assert max(it for *_, it, _ in BATCH_MIX) <= st["batched_rounds"] < sum(it for *_, it, _ in BATCH_MIX)
+
+
+def test_batched_service_runs_other_requests_on_their_own_graphs():
+    """OSEM, binary16 storage and one-planogram geometries are outside the batched form: in a batched service
+    they replay their own captured iteration beside the batched ones, with the images MLEM gives alone."""
+    y48, y64, y32 = data_for("p48", 91), data_for("p64", 92), data_for("p32", 93)
+    want = [alone("p48", y48, 3), alone("p64", y64, 2, num_subsets=4),
+            server_for(GEOS["p48"]).MLEM(y48, None, 2, storage="bf16", image_storage="f32"), alone("p32", y32, 4)]
+    with ReconService(max_slots=8, batched=True) as svc:
+        ids = [svc.submit(**kw("p48", y48, 3)), svc.submit(**kw("p64", y64, 2, num_subsets=4)),
+               svc.submit(**kw("p48", y48, 2, storage="bf16", image_storage="f32")), svc.submit(**kw("p32", y32, 4))]
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["batched_requests"] == 2 and st["failed"] == 0  # p48 fp32 and the one-planogram p32
+
+
+def test_batched_poisson_arrivals_and_pool():
+    """A Poisson stream into the batched service: requests join the running rounds as they are admitted,
+    later ones reuse pooled resources, and every image equals MLEM alone."""
+    rng = np.random.default_rng(95)
+    names = ["p48", "p64", "p96", "p128"]
+    cases = [(names[k % 4], 100 + (k % 6), int(rng.integers(2, 6))) for k in range(10)]
+    gaps = rng.exponential(1.0 / 30.0, size=len(cases))
+    want = [alone(n, data_for(n, sd), it) for (n, sd, it) in cases]
+    with ReconService(max_slots=6, batched=True) as svc:
+        ids = []
+        for gap, (n, sd, it) in zip(gaps, cases):
+            time.sleep(gap)
+            ids.append(svc.submit(**kw(n, data_for(n, sd), it)))
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["batched_requests"] == len(cases) and st["pool_hits"] >= 1
