@@ -272,13 +272,13 @@ PETL_EXTERN bool petl_service_request_footprint(const petl_recon_request* reques
  * 3 doubles): seconds from submission to admission, from admission to completion, and in total.
  * Returns false for an unknown id or a failed request. */
 PETL_EXTERN bool petl_service_wait(void* service, int id, float* f, double* times);
-/* stats (14 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
+/* stats (15 doubles): completed, failed, peak requests on the device, peak reserved bytes, budget,
  * requests currently queued, the host seconds spent so far admitting requests (allocation, data upload
  * with sensitivities, graph capture) and freeing resources, how many requests reused the pooled
  * resources (model, buffers, captured iteration) of a finished request with the same geometry and MLEM
  * parameters, and how many of those captured the iteration again because their binary16 data or image
- * scale differed, how many requests ran in the batched iterations, and how many batched iterations were
- * launched.  Pooled memory counts against the budget until a new request needs it. */
+ * scale differed, how many requests ran in the batched iterations, how many batched iterations were
+ * launched, and the most requests one batched iteration advanced.  Pooled memory counts against the budget until a new request needs it. */
 PETL_EXTERN bool petl_service_stats(void* service, double* stats);
 /* Finishes every queued request, then frees the service. */
 PETL_EXTERN void petl_service_destroy(void* service);

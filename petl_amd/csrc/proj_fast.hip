@@ -45,9 +45,13 @@ namespace {
 #define PETL_FWD_J 7
 #endif
 constexpr int FWD_J = PETL_FWD_J;
-constexpr int FWD_THREADS = 512;
+// waves per forward workgroup (compile-time tuning constant); each wave takes FWD_J views
+#ifndef PETL_FWD_WAVES
+#define PETL_FWD_WAVES 8
+#endif
+constexpr int FWD_WAVES = PETL_FWD_WAVES;
+constexpr int FWD_THREADS = 64 * FWD_WAVES;
 constexpr int FWD_WPROW = 16;  // floats per (block, view) row of the forward's wp table: SB wp, mask, pad
-constexpr int FWD_WAVES = FWD_THREADS / 64;
 constexpr int FWD_SBMAX = 16; // largest slice block of the forward (table row padding)
 constexpr int FWD_HP = 321;   // LDS row pitch (floats) of the forward's z-interpolated rows (Na <= 128)
 constexpr int FWD_HP_WIDE = 449;  // the same for the wide instances (128 < Na <= 256)

@@ -256,6 +256,7 @@ public:
         out[11] = double(recaptures_);
         out[12] = double(batched_requests_);
         out[13] = double(batch_rounds_);
+        out[14] = double(batch_peak_);
     }
 
 private:
@@ -539,6 +540,7 @@ private:
             rounds_.push_back(k);
             std::lock_guard<std::mutex> lk(mu_);
             batch_rounds_++;
+            batch_peak_ = std::max(batch_peak_, long(jobs.size()));
         }
         for (Slot* s : in) {
             const int n = s->req->r.numIter;
@@ -688,7 +690,7 @@ private:
     size_t pooled_ = 0;       // device bytes they hold
     long pool_hits_ = 0;
     long recaptures_ = 0;  // pooled iterations captured again for other binary16 scales
-    long batched_requests_ = 0, batch_rounds_ = 0;
+    long batched_requests_ = 0, batch_rounds_ = 0, batch_peak_ = 0;
     // batched persistent iterations (null when the service was created without them)
     static constexpr int ROUND_EVENTS = 4;
     std::unique_ptr<MlemBatch> batch_;

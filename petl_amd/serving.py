@@ -141,12 +141,13 @@ class ReconService:
         return f, (t[0], t[1], t[2])
 
     def stats(self):
-        s = (ctypes.c_double * 14)()
+        s = (ctypes.c_double * 15)()
         self.lib.petl_service_stats(self.h, s)
         return dict(completed=int(s[0]), failed=int(s[1]), peak_active=int(s[2]), peak_reserved_bytes=s[3],
                     budget_bytes=s[4], queued=int(s[5]), admit_alloc_s=s[6], admit_setup_s=s[7],
                     admit_capture_s=s[8], release_s=s[9], pool_hits=int(s[10]), recaptures=int(s[11]),
-                    batched_requests=int(s[12]), batched_rounds=int(s[13]))
+                    batched_requests=int(s[12]), batched_rounds=int(s[13]),
+                    batched_peak=int(s[14]))
 
     def close(self):
         if self.h:

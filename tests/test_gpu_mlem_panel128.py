@@ -1,0 +1,70 @@
+"""Reference match at the full benchmark configuration against the FP64 oracle.
+
+Configuration: two planograms (psi 0 and 90 degrees) of 128 x 128 crystals at 2 mm, D = 200 mm, shape
+(101, 101, 128, 228) each; 128^3 volume at 2 mm; the seed-2 20-ellipsoid phantom; Poisson data at 1e8
+counts; 20 MLEM iterations with the relative-difference prior (beta 0.01, delta 2), PETL-mode back
+projection, from an image of ones (src/petl_server.py:1534-1606, src/relative_differences.cu:117-173).
+
+The FP64 oracle's run is the fixture tests/golden/mlem_panel128_rd.npz, written by
+tests/golden/make_mlem_panel128_golden.py (about two hours on 8 host cores).  The test regenerates the same
+data from the same seeds with the same code (the fp32 oracle forward is deterministic) and first checks
+that the per-row sums of the counts equal the fixture's.  It then holds, against the fixture's image on the
+every-fourth-voxel grid (32^3 voxels) after 20 iterations:
+
+* the fp32 form to 1e-4 relative L2, its per-iteration FP64 log-likelihood to 1e-6 relative;
+* the 16-bit storage forms with FP32 accumulation to 1e-3: bfloat16 planograms with an fp32 image, binary16
+  planograms with an fp32 image, and binary16 planograms and image.  A bfloat16 image is not held to 1e-3:
+  rounding an image to bfloat16's 8-bit significand alone leaves about 1.6e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "mlem_panel128_rd.npz")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(FIX), reason="fixture not generated (see module doc)")]
+
+
+@pytest.fixture(scope="module")
+def case():
+    from golden.make_mlem_panel128_golden import ITERS, BETA, DELTA, data, row_sums
+    from petl_amd.petl_server import PETLserver
+    from petl_amd.workloads import setup_server
+    fx = np.load(FIX)
+    _, _, _, y = data()
+    np.testing.assert_array_equal(row_sums(y), fx["y_row_sums"])
+    s = PETLserver()
+    s.print_warnings = False
+    setup_server(s, "panel128")
+    assert all(s.uses_fast_path(i) for i in range(2))
+    return s, y, fx, (ITERS, BETA, DELTA)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.timeout(1200)
+def test_fp32_matches_fp64_oracle(case):
+    s, y, fx, (iters, beta, delta) = case
+    f, ll = s.MLEM(y, None, iters, delta=delta, beta=beta, return_loglik=True)
+    err = rel(f[::4, ::4, ::4], fx["image_sub"])
+    print(f"fp32: image rel-L2 {err:.3e}")
+    assert err <= 1e-4
+    np.testing.assert_allclose(ll, fx["loglik"][:iters], rtol=1e-6)
+    assert rel(np.linalg.norm(f.ravel()), fx["image_norm"][-1]) <= 1e-4
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.parametrize("storage,image", [("bf16", "f32"), ("f16", "f32"), ("f16", "f16")])
+def test_16bit_storage_matches_fp64_oracle(case, storage, image):
+    s, y, fx, (iters, beta, delta) = case
+    f = s.MLEM(y, None, iters, delta=delta, beta=beta, storage=storage, image_storage=image)
+    err = rel(f[::4, ::4, ::4], fx["image_sub"])
+    print(f"{storage} planograms, {image} image: image rel-L2 {err:.3e}")
+    assert err <= 1e-3
