@@ -106,6 +106,7 @@ def lib():
         L.orc_event_coords.argtypes = [P, fp, fp, fp]
         L.orc_event_coords.restype = c_int
         L.orc_backproject_petl_subset_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int]
+        L.orc_backproject_petl_tex_f32.argtypes = [P, V, fp, fp, c_int, c_int, c_int, c_int]
         _lib = L
     return _lib
 
@@ -173,6 +174,21 @@ def project_rows(plan, vol, f, g, v1_begin, v1_end, nthreads=None):
 def backproject_petl_rows(plan, vol, g, f, v1_begin, v1_end, nthreads=None):
     """fp32 PETL-mode back projection of v1 rows [v1_begin, v1_end), accumulated into f."""
     lib().orc_backproject_petl_rows_f32(ctypes.byref(plan), ctypes.byref(vol), g, f, v1_begin, v1_end, _nt(nthreads))
+
+
+TEX_QUANT = {"exact": 0, "nearest": 1, "truncate": 2}
+
+
+def backproject_petl_tex(plans, vol, g, quant="nearest", v1_rows=None, nthreads=None):
+    """PETL-mode back projection in the reference's texture-fetch form (bins merged in pairs by bilinear
+    fetches, src/projectors.cu:351-370) with the interpolation fractions exact or quantised to 8
+    fractional bits ('nearest' or 'truncate'); v1_rows: (begin, end) or None for all.  fp32."""
+    f = np.zeros(vol.shape, np.float32)
+    for p, gi in zip(plans, g):
+        b, e = (0, p.N_v1) if v1_rows is None else v1_rows
+        lib().orc_backproject_petl_tex_f32(ctypes.byref(p), ctypes.byref(vol), np.ascontiguousarray(gi, np.float32),
+                                           f, TEX_QUANT[quant], b, e, _nt(nthreads))
+    return f
 
 
 def bin_taps(plan, vol, iv1, iv0, iu1, iu0, cap=1 << 16):

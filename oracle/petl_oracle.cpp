@@ -419,6 +419,99 @@ void backproject_petl_plan(const orc_plan& P, const orc_vol& V, const R* g, R* f
     }
 }
 
+// PETL-mode back projection as the reference evaluates it with hardware texture filtering
+// (src/projectors.cu:296-377, texture set-up src/cuda_utils.cu:1020-1044): bins are taken in pairs (iu0,
+// iu0 + 1) x (iu1, iu1 + 1) from the lower bound of each footprint range, and each pair is one bilinear
+// texture fetch at (iu0 + w02 / (w01 + w02) + 1/2, iu1 + w12 / (w11 + w12) + 1/2) scaled by the weight sums;
+// out-of-array bins read 0 (border addressing).  In exact arithmetic this equals backproject_petl_plan.
+// The texture unit holds each interpolation fraction in 9-bit fixed point with 8 fractional bits (the
+// public CUDA C Programming Guide, texture fetching appendix, linear filtering); quant: 0 exact fractions,
+// 1 fractions rounded to the nearest multiple of 1/256, 2 fractions truncated to a multiple of 1/256 (the
+// guide does not state the rounding).  Float arithmetic throughout, as the reference kernel.
+static float tex_frac(float a, int quant)
+{
+    if (quant == 1)
+        return std::nearbyint(a * 256.0f) / 256.0f;
+    if (quant == 2)
+        return std::floor(a * 256.0f) / 256.0f;
+    return a;
+}
+
+void backproject_petl_tex(const orc_plan& P, const orc_vol& V, const float* g, float* f, int nthreads, int quant,
+                          int v1_begin, int v1_end)
+{
+    const int Nx = V.numX, Ny = V.numY, Nz = V.numZ;
+    const float cs = P.cos_psi, sn = P.sin_psi;
+    const float Tx = V.voxelWidth, Ty = V.voxelWidth, Tz = V.voxelHeight;
+    const float Tu1_inv = 1.0f / P.T_u1, Tu0_inv = 1.0f / P.T_u0;
+    const int64_t row = int64_t(P.N_u1) * P.N_u0;
+    const int64_t slab = int64_t(P.N_v0) * row;
+    auto at = [&](const float* gv, int iu1, int iu0) -> float {
+        if (iu0 < 0 || iu0 >= P.N_u0 || iu1 < 0 || iu1 >= P.N_u1)
+            return 0.0f;
+        return gv[int64_t(iu1) * P.N_u0 + iu0];
+    };
+    auto ovl = [](int i, float A, float B) {
+        return std::max(0.0f, std::min(float(i) + 0.5f, B) - std::max(float(i) - 0.5f, A));
+    };
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int iz = 0; iz < Nz; iz++)
+        for (int iy = 0; iy < Ny; iy++)
+            for (int ix = 0; ix < Nx; ix++) {
+                const float x = coord_f(ix, V.voxelWidth, V.x0), y = coord_f(iy, V.voxelWidth, V.y0);
+                const float z = coord_f(iz, V.voxelHeight, V.z0);
+                const float t = y * cs - x * sn, xdt = x * cs + y * sn;
+                float val = 0.0f;
+                for (int iv1 = v1_begin; iv1 < v1_end; iv1++) {
+                    const float v1 = coord_f(iv1, P.T_v1, P.v1_0);
+                    const float u1 = z + v1 * t;
+                    if (std::fabs(u1) - 0.5f * P.T_u1 > P.H - P.R * std::fabs(v1))
+                        continue;
+                    const float mid = (u1 - P.u1_0) * Tu1_inv;
+                    const float A1 = mid - 0.5f * Tz * Tu1_inv, B1 = mid + 0.5f * Tz * Tu1_inv;
+                    const int r0 = int(std::ceil(A1 - 0.5f)), r1 = int(std::floor(B1 + 0.5f));
+                    const float xdy = 0.5f * Ty * sn, tdy = 0.5f * Ty * cs, xdx = 0.5f * Tx * cs, tdx = -0.5f * Tx * sn;
+                    for (int iv0 = 0; iv0 < P.N_v0; iv0++) {
+                        const float v0 = coord_f(iv0, P.T_v0, P.v0_0);
+                        const float wy = v0 * cs + sn, wx = cs - v0 * sn;
+                        const float l = 1.0f / std::max(std::fabs(wy), std::fabs(wx)) /
+                                        std::sqrt(1.0f + v0 * v0 + v1 * v1);
+                        float A0, B0;
+                        if (std::fabs(wy) > std::fabs(wx)) {
+                            const float a = (xdt - xdy + v0 * (t - tdy) - P.u0_0) * Tu0_inv;
+                            const float b = (xdt + xdy + v0 * (t + tdy) - P.u0_0) * Tu0_inv;
+                            A0 = wy >= 0.0f ? a : b;
+                            B0 = wy >= 0.0f ? b : a;
+                        } else {
+                            const float a = (xdt - xdx + v0 * (t - tdx) - P.u0_0) * Tu0_inv;
+                            const float b = (xdt + xdx + v0 * (t + tdx) - P.u0_0) * Tu0_inv;
+                            A0 = wx >= 0.0f ? a : b;
+                            B0 = wx >= 0.0f ? b : a;
+                        }
+                        const int q0 = int(std::ceil(A0 - 0.5f)), q1 = int(std::floor(B0 + 0.5f));
+                        const float* gv = g + int64_t(iv1) * slab + int64_t(iv0) * row;
+                        for (int iu0 = q0; iu0 <= q1; iu0 += 2) {
+                            const float w02 = ovl(iu0 + 1, A0, B0), ws0 = w02 + ovl(iu0, A0, B0);
+                            if (!(ws0 > 0.0f))
+                                continue;
+                            const float a0 = tex_frac(w02 / ws0, quant);
+                            for (int iu1 = r0; iu1 <= r1; iu1 += 2) {
+                                const float w12 = ovl(iu1 + 1, A1, B1), ws1 = w12 + ovl(iu1, A1, B1);
+                                if (!(ws1 > 0.0f))
+                                    continue;
+                                const float a1 = tex_frac(w12 / ws1, quant);
+                                const float tex = (1.0f - a0) * (1.0f - a1) * at(gv, iu1, iu0) +
+                                                  a0 * (1.0f - a1) * at(gv, iu1, iu0 + 1) +
+                                                  (1.0f - a0) * a1 * at(gv, iu1 + 1, iu0) + a0 * a1 * at(gv, iu1 + 1, iu0 + 1);
+                                val += tex * l * ws0 * ws1;
+                            }
+                        }
+                    }
+                }
+                f[(int64_t(iz) * Ny + iy) * Nx + ix] += val * Tx;
+            }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Relative-difference prior gradient: src/relative_differences.cu:21-28, 107-173, 481-532.
 // Arrays are [N1][N2][N3] (numpy (z,y,x)); neighbours are clamped.
@@ -1264,6 +1357,14 @@ void orc_project_rows_f32(const orc_plan* P, const orc_vol* V, const float* f, f
                           int nthreads)
 {
     project_plan<float>(*P, *V, f, g, nthreads, v1_begin, v1_end);
+}
+
+// The reference's texture-fetch form of the PETL-mode back projection (quant: 0 exact fractions, 1 / 2
+// fractions rounded / truncated to 8 fractional bits), v1 rows [v1_begin, v1_end), accumulated into f.
+void orc_backproject_petl_tex_f32(const orc_plan* P, const orc_vol* V, const float* g, float* f, int quant,
+                                  int v1_begin, int v1_end, int nthreads)
+{
+    backproject_petl_tex(*P, *V, g, f, nthreads, quant, v1_begin, v1_end);
 }
 
 void orc_backproject_petl_rows_f32(const orc_plan* P, const orc_vol* V, const float* g, float* f, int v1_begin,

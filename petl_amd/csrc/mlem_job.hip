@@ -112,6 +112,7 @@ void MlemJob::allocate()
     Model& m = m_;
     DeviceState& D = m.dev;
     m.prepare();
+    m.ensure_workspace();
     if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32)
         for (size_t i = 0; i < m.plans.size(); i++)
             if (!m.uses_fast(int(i)))

@@ -52,6 +52,7 @@ struct DeviceState {
     DevBuf f, f2, S, d;                           // volume-sized
     std::vector<DevBuf> g, y, aux, resp;          // planogram-sized
     DevBuf h2;                                    // fast back projector workspace
+    size_t h2_need = 0;                           // bytes it needs for one planogram (the largest)
     DevBuf fT;                                    // the volume with x and y swapped (general forward)
     DevBuf red;                                   // reduction scratch
     BinScratch bin;                               // list-mode binning
@@ -86,6 +87,10 @@ struct Model {
     // device helpers (petl_device.hip)
     hipStream_t stream();
     void prepare();  // (re)build per-planogram kernel plans when geometry changed
+    // whether the kernel plans match the current geometry (no prepare() needed)
+    bool prepared() const;
+    // allocates the fast back projector's workspace (one planogram at a time) if it is not there yet
+    void ensure_workspace();
     bool uses_fast(int i);
     // the general forward of planogram i (transposing f into dev.fT first when `transpose` is set)
     void gen_forward_one(size_t i, float* g, const float* f, bool transpose, int epilogue, const float* y,
@@ -103,7 +108,9 @@ struct Model {
     // FP64 partials forward_sumsq_parts needs for planogram i (0 for the generic kernels)
     int64_t sumsq_parts(int i);
     // f = sum_i A_i^T g_i (overwrites f)
-    void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1);
+    // workspace: the fast back projector's workspace (at least dev.h2_need bytes), or null for the model's own
+    void back(float* const* g, float* f, int mode, hipStream_t s, int subset = 0, int nsubsets = 1,
+              float* workspace = nullptr);
     // Batches of `batch` volumes f (contiguous, vol_elems() each) and planograms g[i] (contiguous, batch x
     // planogram i): one launch per planogram on the fast kernels, a loop over the batch on the generic ones.
     // back_batch overwrites f with the sum over planograms, in `mode`.

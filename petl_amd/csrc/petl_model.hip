@@ -170,7 +170,7 @@ hipStream_t Model::stream()
 
 void Model::prepare()
 {
-    if (dev.built_version == version && dev.built_path == path && dev.fast.size() == plans.size())
+    if (prepared())
         return;
     dev.fast.clear();
     dev.gen.clear();
@@ -185,10 +185,20 @@ void Model::prepare()
         dev.gen.push_back(fp ? nullptr : make_gen_plan(p.geom(), V));
         dev.fast.push_back(std::move(fp));
     }
-    if (ws)
-        dev.h2.ensure(ws);
+    dev.h2_need = ws;  // allocated by the first back projection that needs it
     dev.built_version = version;
     dev.built_path = path;
+}
+
+bool Model::prepared() const
+{
+    return dev.built_version == version && dev.built_path == path && dev.fast.size() == plans.size();
+}
+
+void Model::ensure_workspace()
+{
+    if (dev.h2_need)
+        dev.h2.ensure(dev.h2_need);
 }
 
 bool Model::uses_fast(int i)
@@ -245,14 +255,18 @@ int64_t Model::forward_sumsq_parts(int i, const float* f, const float* resp, con
     return n;
 }
 
-void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets)
+void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets, float* workspace)
 {
     prepare();
+    if (!workspace) {
+        ensure_workspace();
+        workspace = dev.h2.f();
+    }
     const VolGeom V = vgeom();
     launch_fill(f, 0.0f, vol_elems(), s);
     for (size_t i = 0; i < plans.size(); i++) {
         if (dev.fast[i])
-            fast_back(*dev.fast[i], g[i], f, mode, dev.h2.f(), s, subset, nsubsets);
+            fast_back(*dev.fast[i], g[i], f, mode, workspace, s, subset, nsubsets);
         else
             gen_back(*dev.gen[i], g[i], f, mode, s, subset, nsubsets);
     }
@@ -320,6 +334,7 @@ void Model::forward_st(int sf, int sg, void* const* g, const void* f, int epilog
 void Model::back_st(int st, void* const* g, float* f, int mode, hipStream_t s, int subset, int nsubsets, float g_scale)
 {
     prepare();
+    ensure_workspace();
     launch_fill(f, 0.0f, vol_elems(), s);
     for (size_t i = 0; i < plans.size(); i++) {
         if (!dev.fast[i] && st != STORE_F32)
