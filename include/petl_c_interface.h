@@ -180,6 +180,15 @@ PETL_EXTERN bool project_gpu_ex(void** d_g, const void* d_f, int volume_storage,
                                 float f_scale, float g_scale, void* stream, bool sync);
 /* backproject_gpu with the planograms in `storage`; the volume d_f is fp32 (overwritten). */
 PETL_EXTERN bool backproject_gpu_ex(void** d_g, float* d_f, int storage, float g_scale, void* stream, bool sync);
+/* For callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it); no reference counterpart.  prepare_gpu builds the current model's kernel tables
+ * (it allocates; call it once beforehand).  backproject_gpu_noalloc is backproject_gpu with the fast back
+ * projector's workspace in a fixed 128 MiB device array that each GPU holds until the process ends (set up on
+ * the first call on a device): it allocates nothing, and returns false when the tables are not built or the
+ * geometry's workspace (N_v1 x N_u1 x in-plane voxels x 4 bytes for the largest planogram) exceeds 128 MiB.
+ * Every call on one GPU shares that array: calls that overlap on one GPU race. */
+PETL_EXTERN bool prepare_gpu(void);
+PETL_EXTERN bool backproject_gpu_noalloc(float** d_g, float* d_f, void* stream, bool sync);
 /* Batches for differentiable callers (no reference counterpart; the reference projects one volume per
  * call): d_f holds `batch` contiguous fp32 volumes and d_g[i] `batch` contiguous copies of planogram i.
  * project_gpu_batch overwrites every planogram; backproject_gpu_batch overwrites every volume with the sum
