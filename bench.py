@@ -23,7 +23,8 @@ Prints one JSON line.  Besides the headline rate it carries:
   reconstruction  MLEM and RWLS per-iteration times and FBP, host data in and image out
   small_config  10 MLEM iterations at 32x32-crystal panels and a 32^3 volume
   serving       reconstructions served per second and p50/p99 latency for a Poisson arrival stream of
-                MLEM requests of mixed panel/volume sizes, through the continuous-batching service and,
+                MLEM requests of mixed panel/volume sizes, through the continuous-batching service (each
+                request its own captured iteration, or all of them in batched persistent iterations) and,
                 for comparison, one request at a time
   cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference projectors) timed on a bounded
                 sample of the same workload on this host's cores, and on the small configuration
@@ -145,7 +146,7 @@ def serving_datasets(torch, dev, seed):
     return out
 
 
-def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
+def serving_run(datasets, lam, nreq, slots, seed, storage="f32", batched=False):
     """Poisson arrivals at rate lam; each request = (panel size, dataset, iterations U{5..20}).  Returns
     recons/s over the run and latency percentiles (submission to completion)."""
     import threading
@@ -160,7 +161,7 @@ def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
         which = int(rng.integers(2))
         reqs.append((t, name, which, int(rng.integers(5, 21))))
     lat = [None] * nreq
-    with ReconService(max_slots=slots) as svc:
+    with ReconService(max_slots=slots, batched=batched) as svc:
         ids = [None] * nreq
         ready = [threading.Event() for _ in range(nreq)]
 
@@ -191,39 +192,59 @@ def serving_run(datasets, lam, nreq, slots, seed, storage="f32"):
             "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "peak_on_device": st["peak_active"],
             "failed": st["failed"],
             "host_s": {k: round(st[k], 4) for k in ("admit_alloc_s", "admit_setup_s", "admit_capture_s", "release_s")},
-            "pool_hits": st["pool_hits"]}
+            "pool_hits": st["pool_hits"],
+            **({"batched_rounds": st["batched_rounds"], "batched_peak_requests": st["batched_peak"]} if batched else {})}
+
+
+FWD_KERNEL = "void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0, 128>"  # the fp32 forward instance of the benchmark
+
+
+def kernel_source_sha():
+    """SHA-256 of the fast kernels' source, which the committed counter tables record: a table made from
+    other kernel code is reported as not current."""
+    import hashlib
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "petl_amd", "csrc", "proj_fast.hip")
+    with open(path, "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()
+
+
+def profile_table(name):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return None
+
+
+def table_current(name):
+    t = profile_table(name)
+    return t is not None and t.get("source_sha256") == kernel_source_sha()
 
 
 def measured_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed counter summary (copy-calibrated FETCH_SIZE +
     WRITE_SIZE), or None when it has no entry for the kernel."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_panel128.json")
-    try:
-        with open(path) as fh:
-            entry = json.load(fh)["kernels"].get(kernel)
-    except (OSError, ValueError, KeyError):
-        return None
+    t = profile_table("traffic_panel128.json")
+    entry = None if t is None else t.get("kernels", {}).get(kernel)
     return None if entry is None else float(entry["traffic_bytes"])
 
 
 def lds_roofline(launch_ms):
-    insts = measured_counter("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>", "SQ_INSTS_LDS")
+    insts = measured_counter(FWD_KERNEL, "SQ_INSTS_LDS")
     if insts is None:
         return None
     achieved = insts * 256.0 / (launch_ms * 1e-3) / 1e9
     return {"kernel": "fwd_fast_kernel", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / LDS_PEAK_GBS, "lds_instructions_per_launch": insts,
-            "source": "profiles/counters_panel128.json (SQ_INSTS_LDS, rocprofv3 --pmc, --pair-only)"}
+            "source": "profiles/counters_panel128.json (SQ_INSTS_LDS, rocprofv3 --pmc, --pair-only)",
+            "table_current": table_current("counters_panel128.json")}
 
 
 def measured_counter(kernel, counter):
     """Mean per dispatch of an SQ counter of `kernel` from the committed counter summary, or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "counters_panel128.json")
-    try:
-        with open(path) as fh:
-            entry = json.load(fh)["kernels"].get(kernel)
-    except (OSError, ValueError, KeyError):
-        return None
+    t = profile_table("counters_panel128.json")
+    entry = None if t is None else t.get("kernels", {}).get(kernel)
     return None if entry is None or counter not in entry else float(entry[counter])
 
 
@@ -236,13 +257,14 @@ VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 def valu_issue(launch_ms):
     """Executed VALU wave instructions per second of the fast forward kernel (count per launch from the
     committed SQ_INSTS_VALU counter) against the issue peak."""
-    insts = measured_counter("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>", "SQ_INSTS_VALU")
+    insts = measured_counter(FWD_KERNEL, "SQ_INSTS_VALU")
     if insts is None:
         return None
     rate = insts / (launch_ms * 1e-3)
     return {"achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instructions/s",
             "frac": rate / VALU_ISSUE_PEAK, "valu_instructions_per_launch": insts,
-            "source": "profiles/counters_panel128.json (SQ_INSTS_VALU, rocprofv3 --pmc, --pair-only)"}
+            "source": "profiles/counters_panel128.json (SQ_INSTS_VALU, rocprofv3 --pmc, --pair-only)",
+            "table_current": table_current("counters_panel128.json")}
 
 
 def main():
@@ -351,7 +373,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>"),
+            "traffic": measured_traffic(FWD_KERNEL),
+            "traffic_table_current": table_current("traffic_panel128.json"),
             "traffic_source": "profiles/traffic_panel128.json (table in profiles/kernels_panel128.md): rocprofv3 "
                               "--pmc FETCH_SIZE and WRITE_SIZE of this kernel (one run each, --pair-only), scaled "
                               "by the same counters on a 256 MiB copy",
@@ -615,14 +638,18 @@ def main():
         for rate in [float(x) for x in a.serve_rates.split(",") if x]:
             mixed = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501)
             mixed16 = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, storage="bf16")
+            persistent = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, batched=True)
             single = serving_run(data, rate, a.serve_requests, 1, seed=501)
             runs.append({"arrival_rate_req_s": rate, "continuous_batching": dict(mixed, slots=a.serve_slots),
+                         "batched_persistent": dict(persistent, slots=a.serve_slots),
                          "continuous_batching_bf16_planograms": dict(mixed16, slots=a.serve_slots),
                          "one_at_a_time": dict(single, slots=1)})
         out["serving"] = {
             "workload": "Poisson arrivals; panels uniform over 48/64/96/128 crystals (2 orientations, D = 200 mm, "
                         "matched n^3 volumes at 2 mm); data = Poisson at counts log-uniform in [1e5, 1e8]; "
-                        "MLEM iterations uniform in 5..20; fp32 (and bf16 planograms with an fp32 image)",
+                        "MLEM iterations uniform in 5..20; fp32 (and bf16 planograms with an fp32 image); "
+                        "continuous_batching: each request replays its own captured iteration on its own stream; "
+                        "batched_persistent: one captured graph of persistent kernels advances all requests",
             "requests": a.serve_requests,
             "runs": runs,
         }

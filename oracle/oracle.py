@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "libpetl_oracle.so")
+_LIB = os.environ.get("PETL_ORACLE_LIBRARY") or os.path.join(_HERE, "libpetl_oracle.so")
 _REF = os.path.join(_HERE, "_ref", "libpetl_ref_geom.so")
 
 

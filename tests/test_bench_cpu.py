@@ -15,7 +15,7 @@ def load_bench():
 
 def test_committed_traffic_and_counters_cover_the_forward_kernel():
     bench = load_bench()
-    name = "void fwd_fast_kernel<7, 4, 8, 1, 0, 0, 0>"
+    name = bench.FWD_KERNEL
     traffic = bench.measured_traffic(name)
     # at least the bins written (1.2 GB per launch), well under 10x the algorithmic bytes
     assert traffic is not None and 1.0e9 < traffic < 1.2e10
@@ -28,6 +28,13 @@ def test_committed_traffic_and_counters_cover_the_forward_kernel():
     issue = bench.valu_issue(2.5)
     assert issue is not None and 0.0 < issue["frac"] < 1.0
     assert abs(issue["achieved"] - valu / 2.5e-3 / 1e9) < 1e-6 * issue["achieved"]
+
+
+def test_committed_tables_are_from_the_current_kernels():
+    """The tables record the SHA-256 of the fast kernels' source they were measured with."""
+    bench = load_bench()
+    assert bench.table_current("traffic_panel128.json")
+    assert bench.table_current("counters_panel128.json")
 
 
 def test_unknown_kernels_give_no_figures():
