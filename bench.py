@@ -24,7 +24,7 @@ Prints one JSON line.  Besides the headline rate it carries:
   small_config  10 MLEM iterations at 32x32-crystal panels and a 32^3 volume
   serving       reconstructions served per second and p50/p99 latency for a Poisson arrival stream of
                 MLEM requests of mixed panel/volume sizes, through the continuous-batching service (each
-                request its own captured iteration, or all of them in batched persistent iterations) and,
+                request its own captured iteration, or all of them in batched iterations) and,
                 for comparison, one request at a time
   cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference projectors) timed on a bounded
                 sample of the same workload on this host's cores, and on the small configuration
@@ -638,10 +638,10 @@ def main():
         for rate in [float(x) for x in a.serve_rates.split(",") if x]:
             mixed = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501)
             mixed16 = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, storage="bf16")
-            persistent = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, batched=True)
+            batched = serving_run(data, rate, a.serve_requests, a.serve_slots, seed=501, batched=True)
             single = serving_run(data, rate, a.serve_requests, 1, seed=501)
             runs.append({"arrival_rate_req_s": rate, "continuous_batching": dict(mixed, slots=a.serve_slots),
-                         "batched_persistent": dict(persistent, slots=a.serve_slots),
+                         "batched_iterations": dict(batched, slots=a.serve_slots),
                          "continuous_batching_bf16_planograms": dict(mixed16, slots=a.serve_slots),
                          "one_at_a_time": dict(single, slots=1)})
         out["serving"] = {
@@ -649,7 +649,7 @@ def main():
                         "matched n^3 volumes at 2 mm); data = Poisson at counts log-uniform in [1e5, 1e8]; "
                         "MLEM iterations uniform in 5..20; fp32 (and bf16 planograms with an fp32 image); "
                         "continuous_batching: each request replays its own captured iteration on its own stream; "
-                        "batched_persistent: one captured graph of persistent kernels advances all requests",
+                        "batched_iterations: one launch per phase advances all requests on the device",
             "requests": a.serve_requests,
             "runs": runs,
         }

@@ -262,9 +262,10 @@ typedef struct petl_recon_request {
 PETL_EXTERN void* petl_service_create(double hbm_budget_bytes, int max_slots);
 /* petl_service_create with flags: PETL_SERVICE_BATCHED runs the requests that allow it (fp32 MLEM, one
  * subset, at most two planograms on the fast kernels with up to 255 u0 bins and 128-voxel in-plane and
- * slice axes, PETL-mode back projection) in batched persistent iterations: one captured graph of six
- * persistent kernels advances every such request on the device by one iteration, whatever its size, with
- * work items taken from device queues; the others run their own captured iteration as without the flag.
+ * slice axes, PETL-mode back projection) in batched iterations: one launch per phase of an iteration
+ * (forward with the ratio, per planogram the two back projector stages, the update) advances every such
+ * request on the device by one iteration, whatever its size, its grid covering all their work items; the
+ * others run their own captured iteration as without the flag.
  * No reference counterpart (the reference has no service). */
 #define PETL_SERVICE_BATCHED 1
 PETL_EXTERN void* petl_service_create_ex(double hbm_budget_bytes, int max_slots, int flags);

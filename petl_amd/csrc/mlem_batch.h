@@ -1,13 +1,13 @@
-// Batched persistent MLEM iterations: one launch per phase of an iteration advances every request in the
-// batch, whatever its panel and volume size (the serving path's batched form).
+// Batched MLEM iterations: one launch per phase of an iteration advances every request in the batch,
+// whatever its panel and volume size (the serving path's batched form).
 //
-// An iteration is six persistent kernels captured once as a graph: forward with the ratio epilogue of every
-// (request, planogram); for planogram 0 then 1 the back projector's u0 gather and u1 gather; the image
-// update.  Each kernel runs a fixed grid whose workgroups (or waves) take work items from a device queue
-// (an atomic counter) over the prefix sums of the requests' work in a device table, so the captured graph
-// serves any mix: between iterations only the table is rewritten (one small host-to-device copy, on the
-// same stream).  Every work item runs the same code as the per-request kernels (shared bodies), so a
-// request's image does not depend on the batch it runs in.
+// An iteration is: the forward with the ratio epilogue of every (request, planogram); for planogram 0 then
+// 1, the back projector's u0 gather and u1 gather; the image update.  Each phase is one launch (one more per
+// group of requests beyond what one launch's kernel arguments hold) whose flat grid covers the work items of
+// all requests: a workgroup finds its request from the prefix sums of the requests' item counts in the
+// kernel arguments, where the requests' geometry and buffers travel too.  Every work item runs the same
+// code as the per-request kernels (shared bodies), so a request's image does not depend on the batch it
+// runs in.
 #pragma once
 #include <vector>
 #include "proj_fast.h"
@@ -41,10 +41,9 @@ public:
     MlemBatch& operator=(const MlemBatch&) = delete;
     // whether a request whose planograms have these fast plans (null: not a fast geometry) can run batched:
     // at most BATCH_MAX_PLANOGRAMS planograms, each on the fast kernels with at most 255 u0 bins, in-plane
-    // axes of at most 128 voxels and LDS needs within the captured launches'
+    // and slice axes of at most 128 voxels
     static bool eligible(const std::vector<const FastPlan*>& plans);
-    // enqueues one iteration of every job on s (at most BATCH_MAX_REQUESTS): the table copy, then the six
-    // captured phases
+    // enqueues one iteration of every job on s (at most BATCH_MAX_REQUESTS)
     void launch(const std::vector<BatchIterJob>& jobs, hipStream_t s);
 
 private:

@@ -47,7 +47,7 @@ public:
     // holds them as kernel arguments, so it can be replayed for another request only when they are equal
     float data_scale() const { return s_y_; }
     float image_scale() const { return s_f_; }
-    // whether the job can run in batched persistent iterations (MlemBatch): fp32 MLEM (one subset, no mask,
+    // whether the job can run in batched iterations (MlemBatch): fp32 MLEM (one subset, no mask,
     // response or log-likelihood) in PETL mode on fast plans that MlemBatch takes; call after allocate()
     bool batch_eligible();
     // iteration `it` of the job as a batched-iteration entry (after setup()).  With a prior the image

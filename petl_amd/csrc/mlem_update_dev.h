@@ -1,5 +1,5 @@
 // Per-voxel MLEM updates (src/petl_server.py:1586-1602), shared by the update kernels and the batched
-// persistent iteration, so that every launch form computes the same bits.
+// iterations, so that every launch form computes the same bits.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>

@@ -1456,15 +1456,12 @@ void fast_back_st(const FastPlan& fp, int st, const void* g, float* f, int mode,
 
 
 // =============================================================================================
-// batched persistent MLEM iterations (mlem_batch.h)
+// batched MLEM iterations (mlem_batch.h)
 // =============================================================================================
 namespace {
 
-enum BatchPhase { PH_F = 0, PH_H0, PH_Z0, PH_H1, PH_Z1, PH_U, PH_COUNT };
-constexpr int BATCH_ZC = 8;                 // z rows per back_z work item (the sums do not depend on it)
-constexpr int BATCH_FWD_LDS = 2 * 8 * FWD_HP * 4 + 128 * 16;  // row buffers + z table of Ns <= 128
-constexpr int BATCH_BACK_LDS = 40 * 1024;   // back_h2 row buffers or transpose tile, at most
-constexpr int BATCH_GRID_FWD = 1024, BATCH_GRID_H2 = 2048, BATCH_GRID_Z = 2048, BATCH_GRID_U = 1024;
+constexpr int BATCH_ZC = 8;            // z rows per back_z work item (the sums do not depend on it)
+constexpr int BATCH_H2_THREADS = 128;  // back_h2 lanes = slices (the batched form takes Ns <= 128)
 
 struct BFwd {
     FwdArgs A;
@@ -1500,140 +1497,94 @@ struct BUpd {
     HuberParams h;
 };
 
-// the per-iteration table: job lists and the prefix sums of their work items
-struct BatchTable {
-    int nf, nb[BATCH_MAX_PLANOGRAMS], nu;
-    int fs[BATCH_MAX_PLANOGRAMS * BATCH_MAX_REQUESTS + 1];
-    int hs[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS + 1];
-    int zs[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS + 1];
-    int us[BATCH_MAX_REQUESTS + 1];
-    BFwd fwd[BATCH_MAX_PLANOGRAMS * BATCH_MAX_REQUESTS];
-    BBack back[BATCH_MAX_PLANOGRAMS][BATCH_MAX_REQUESTS];
-    BUpd upd[BATCH_MAX_REQUESTS];
+// A phase's jobs travel as the launch's kernel argument (read with scalar loads, like any kernel
+// argument; a table in global memory would cost the forward ~50 registers, since the compiler cannot
+// prove that the work's stores leave it unchanged).  start[] holds the prefix sums of the jobs' work
+// items; blockIdx.x is the flat item index.
+constexpr int KARG_BYTES = 3800;  // below the 4 KB kernel-argument limit
+template <typename Job>
+struct PhaseArgs {
+    static constexpr int MAXJ = (KARG_BYTES - 8) / int(sizeof(Job) + sizeof(int));
+    int n;
+    int start[MAXJ + 1];
+    Job job[MAXJ];
 };
+static_assert(sizeof(PhaseArgs<BFwd>) <= 4096 && sizeof(PhaseArgs<BBack>) <= 4096 && sizeof(PhaseArgs<BUpd>) <= 4096,
+              "kernel arguments of the batched phases");
+static_assert(PhaseArgs<BFwd>::MAXJ >= 4 && PhaseArgs<BBack>::MAXJ >= 4, "jobs per batched launch");
 
 // the job whose prefix range [start[j], start[j + 1]) holds item (start[0] = 0, item < start[n])
-__device__ __forceinline__ int batch_job(const int* __restrict__ start, int n, int item)
+template <typename PA>
+__device__ __forceinline__ int batch_job(const PA& A, int item)
 {
     int j = 0;
-    while (j + 1 < n && item >= start[j + 1])
+    while (j + 1 < A.n && item >= A.start[j + 1])
         j++;
     return j;
 }
 
-// next work item of a workgroup-level queue (every thread gets the same uniform value)
-__device__ __forceinline__ int batch_next_block(unsigned* ctr, int* s_item)
+__global__ void __launch_bounds__(FWD_THREADS) batch_fwd_kernel(const PhaseArgs<BFwd> P)
 {
-    if (threadIdx.x == 0)
-        *s_item = int(atomicAdd(ctr, 1u));
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(*s_item);
-    __syncthreads();
-    return it;
+    const int item = blockIdx.x;
+    const int j = batch_job(P, item);
+    const BFwd& J = P.job[j];
+    int l = item - P.start[j];
+    const int grp = l % J.ngroups;
+    l /= J.ngroups;
+    const int iu1 = l % J.nu1, iv1 = l / J.nu1;
+    fwd_body<FWD_J, 4, 8, 1, STORE_F32, STORE_F32, EK_RUNTIME, 128>(J.g, J.f, J.A, J.ftab, J.fwp, J.ztab, J.fscale,
+                                                                    EPI_RATIO, J.y, nullptr, grp, iu1, iv1, 0);
 }
 
-// next work item of a wave-level queue
-__device__ __forceinline__ int batch_next_wave(unsigned* ctr)
+__global__ void __launch_bounds__(BATCH_H2_THREADS) batch_h2_kernel(const PhaseArgs<BBack> P)
 {
-    int it = 0;
-    if ((threadIdx.x & 63) == 0)
-        it = int(atomicAdd(ctr, 1u));
-    return __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    const int item = blockIdx.x;
+    const int j = batch_job(P, item);
+    const BBack& J = P.job[j];
+    int l = item - P.start[j];
+    const int gxi = l % J.gx;
+    l /= J.gx;
+    const int iu1 = l % J.nu1, iv1 = l / J.nu1;
+    const int sgr = gxi / J.A.nagroups, ag = gxi - sgr * J.A.nagroups;
+    back_h2_body<BACK_AG, STORE_F32, true, false>(J.g, J.h2, J.A, J.xtab, J.xrow, J.scale, J.rows, ag, sgr, iu1, iv1);
 }
 
-// Each phase clears the queue counter of the phase before it (finished: the launches are ordered on one
-// stream), so that every counter is zero again when its phase next runs.
-__global__ void __launch_bounds__(FWD_THREADS) batch_fwd_kernel(const BatchTable* __restrict__ T, unsigned* ctr)
+// z items: start[] of these arguments holds the prefix sums of the back_z work (one wave each)
+__global__ void __launch_bounds__(64) batch_z_kernel(const PhaseArgs<BBack> P)
 {
-    __shared__ int s_item;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        ctr[PH_U] = 0;
-    const int nf = T->nf, total = T->fs[nf];
-    for (;;) {
-        const int item = batch_next_block(ctr + PH_F, &s_item);
-        if (item >= total)
-            break;
-        const int j = batch_job(T->fs, nf, item);
-        const BFwd& J = T->fwd[j];
-        int l = item - T->fs[j];
-        const int grp = l % J.ngroups;
-        l /= J.ngroups;
-        const int iu1 = l % J.nu1, iv1 = l / J.nu1;
-        fwd_body<FWD_J, 4, 8, 1, STORE_F32, STORE_F32, EK_RUNTIME, 128>(J.g, J.f, J.A, J.ftab, J.fwp, J.ztab, J.fscale,
-                                                                        EPI_RATIO, J.y, nullptr, grp, iu1, iv1, 0);
-        __syncthreads();  // the next item rewrites the LDS buffers
-    }
+    const int item = blockIdx.x;
+    const int j = batch_job(P, item);
+    const BBack& J = P.job[j];
+    int l = item - P.start[j];
+    const int ac = l % J.nac;
+    l /= J.nac;
+    const int is = l % J.A.Ns, zc = l / J.A.Ns;
+    back_z_body<BATCH_ZC>(J.h2, J.f, J.A, J.ztab, J.rows, threadIdx.x, ac, is, zc * BATCH_ZC, J.overwrite != 0);
 }
 
-__global__ void __launch_bounds__(256) batch_h2_kernel(const BatchTable* __restrict__ T, unsigned* ctr, int p)
+__global__ void __launch_bounds__(64) batch_update_kernel(const PhaseArgs<BUpd> P)
 {
-    __shared__ int s_item;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        ctr[p == 0 ? PH_F : PH_Z0] = 0;
-    const int n = T->nb[p], total = T->hs[p][n];
-    for (;;) {
-        const int item = batch_next_block(ctr + (p == 0 ? PH_H0 : PH_H1), &s_item);
-        if (item >= total)
-            break;
-        const int j = batch_job(T->hs[p], n, item);
-        const BBack& J = T->back[p][j];
-        int l = item - T->hs[p][j];
-        const int gxi = l % J.gx;
-        l /= J.gx;
-        const int iu1 = l % J.nu1, iv1 = l / J.nu1;
-        const int sgr = gxi / J.A.nagroups, ag = gxi - sgr * J.A.nagroups;
-        back_h2_body<BACK_AG, STORE_F32, true, false>(J.g, J.h2, J.A, J.xtab, J.xrow, J.scale, J.rows, ag, sgr, iu1,
-                                                      iv1);
-        __syncthreads();
-    }
+    const int item = blockIdx.x;
+    const int j = batch_job(P, item);
+    const BUpd& U = P.job[j];
+    int l = item - P.start[j];
+    const int kc = l % U.nk;
+    l /= U.nk;
+    const int jj = l % U.N2, ii = l / U.N2;
+    const int k = kc * 64 + int(threadIdx.x);
+    if (k >= U.N3)
+        return;
+    if (U.prior == 0)
+        mlem_update_at(U.fo, U.d, U.S, (int64_t(ii) * U.N2 + jj) * U.N3 + k);
+    else if (U.prior == 1)
+        mlem_update_prior_at<0>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
+    else
+        mlem_update_prior_at<1>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
 }
 
-__global__ void __launch_bounds__(256) batch_z_kernel(const BatchTable* __restrict__ T, unsigned* ctr, int p)
+size_t batch_fwd_lds(const FastPlan& fp)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        ctr[p == 0 ? PH_H0 : PH_H1] = 0;
-    const int lane = threadIdx.x & 63;
-    const int n = T->nb[p], total = T->zs[p][n];
-    for (;;) {
-        const int item = batch_next_wave(ctr + (p == 0 ? PH_Z0 : PH_Z1));
-        if (item >= total)
-            break;
-        const int j = batch_job(T->zs[p], n, item);
-        const BBack& J = T->back[p][j];
-        int l = item - T->zs[p][j];
-        const int ac = l % J.nac;
-        l /= J.nac;
-        const int is = l % J.A.Ns, zc = l / J.A.Ns;
-        back_z_body<BATCH_ZC>(J.h2, J.f, J.A, J.ztab, J.rows, lane, ac, is, zc * BATCH_ZC, J.overwrite != 0);
-    }
-}
-
-__global__ void __launch_bounds__(256) batch_update_kernel(const BatchTable* __restrict__ T, unsigned* ctr)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        ctr[PH_Z1] = 0;
-    const int lane = threadIdx.x & 63;
-    const int n = T->nu, total = T->us[n];
-    for (;;) {
-        const int item = batch_next_wave(ctr + PH_U);
-        if (item >= total)
-            break;
-        const int j = batch_job(T->us, n, item);
-        const BUpd& U = T->upd[j];
-        int l = item - T->us[j];
-        const int kc = l % U.nk;
-        l /= U.nk;
-        const int jj = l % U.N2, ii = l / U.N2;
-        const int k = kc * 64 + lane;
-        if (k >= U.N3)
-            continue;
-        if (U.prior == 0)
-            mlem_update_at(U.fo, U.d, U.S, (int64_t(ii) * U.N2 + jj) * U.N3 + k);
-        else if (U.prior == 1)
-            mlem_update_prior_at<0>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
-        else
-            mlem_update_prior_at<1>(U.fo, U.fi, U.d, U.S, ii, jj, k, U.N1, U.N2, U.N3, U.delta, U.beta, U.h);
-    }
+    return size_t(2) * fp.R * fp.SB * fp.hrow * sizeof(float) + size_t(fp.Ns) * sizeof(int4);
 }
 
 size_t batch_back_lds(const FastPlan& fp)
@@ -1641,107 +1592,61 @@ size_t batch_back_lds(const FastPlan& fp)
     return std::max(size_t(2) * BACK_RB * fp.srow, size_t(std::min(fp.Ns, BACK_SG)) * (BACK_TW + 1)) * sizeof(float);
 }
 
+// launches `kernel` over the jobs in groups of PhaseArgs::MAXJ; items(job) = its work items, lds(job) its LDS
+template <typename Job, typename Kern, typename Items, typename Lds>
+void launch_phase(Kern kernel, const std::vector<Job>& jobs, Items items, Lds lds, int threads, hipStream_t s)
+{
+    using PA = PhaseArgs<Job>;
+    for (size_t b = 0; b < jobs.size(); b += PA::MAXJ) {
+        PA A{};
+        A.n = int(std::min<size_t>(PA::MAXJ, jobs.size() - b));
+        A.start[0] = 0;
+        size_t shared = 0;
+        for (int k = 0; k < A.n; k++) {
+            A.job[k] = jobs[b + k];
+            A.start[k + 1] = A.start[k] + items(A.job[k]);
+            shared = std::max(shared, lds(k));
+        }
+        if (A.start[A.n] == 0)
+            continue;
+        hipLaunchKernelGGL(kernel, dim3(A.start[A.n]), dim3(threads), shared, s, A);
+        PETL_LAUNCH_CHECK();
+    }
+}
+
 }  // namespace
 
-struct MlemBatch::Impl {
-    static constexpr int RING = 4;     // host tables in flight
-    BatchTable* dev = nullptr;
-    unsigned* ctr = nullptr;
-    BatchTable* host[RING] = {};
-    hipEvent_t copied[RING] = {};
-    int next = 0;
-    hipStream_t cap = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+struct MlemBatch::Impl {};
 
-    void capture()
-    {
-        PETL_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(batch_fwd_kernel, dim3(BATCH_GRID_FWD), dim3(FWD_THREADS), BATCH_FWD_LDS, cap, dev, ctr);
-        for (int p = 0; p < BATCH_MAX_PLANOGRAMS; p++) {
-            hipLaunchKernelGGL(batch_h2_kernel, dim3(BATCH_GRID_H2), dim3(256), BATCH_BACK_LDS, cap, dev, ctr, p);
-            hipLaunchKernelGGL(batch_z_kernel, dim3(BATCH_GRID_Z), dim3(256), 0, cap, dev, ctr, p);
-        }
-        hipLaunchKernelGGL(batch_update_kernel, dim3(BATCH_GRID_U), dim3(256), 0, cap, dev, ctr);
-        PETL_HIP_CHECK(hipStreamEndCapture(cap, &graph));
-        PETL_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-};
+MlemBatch::MlemBatch() : impl_(nullptr) {}
 
-MlemBatch::MlemBatch() : impl_(new Impl)
-{
-    static_assert(PH_COUNT == 6, "six phases");
-    PETL_HIP_CHECK(hipMalloc(&impl_->dev, sizeof(BatchTable)));
-    PETL_HIP_CHECK(hipMalloc(&impl_->ctr, 256));
-    PETL_HIP_CHECK(hipMemset(impl_->ctr, 0, 256));  // every queue starts empty; each phase clears its predecessor
-    for (int i = 0; i < Impl::RING; i++) {
-        PETL_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&impl_->host[i]), sizeof(BatchTable)));
-        PETL_HIP_CHECK(hipEventCreateWithFlags(&impl_->copied[i], hipEventDisableTiming));
-    }
-    PETL_HIP_CHECK(hipStreamCreateWithFlags(&impl_->cap, hipStreamNonBlocking));
-    PETL_HIP_CHECK(hipDeviceSynchronize());
-    impl_->capture();
-}
-
-MlemBatch::~MlemBatch()
-{
-    if (impl_->exec)
-        (void)hipGraphExecDestroy(impl_->exec);
-    if (impl_->graph)
-        (void)hipGraphDestroy(impl_->graph);
-    for (int i = 0; i < Impl::RING; i++) {
-        if (impl_->copied[i])
-            (void)hipEventSynchronize(impl_->copied[i]);
-        if (impl_->copied[i])
-            (void)hipEventDestroy(impl_->copied[i]);
-        if (impl_->host[i])
-            (void)hipHostFree(impl_->host[i]);
-    }
-    if (impl_->dev)
-        (void)hipFree(impl_->dev);
-    if (impl_->ctr)
-        (void)hipFree(impl_->ctr);
-    delete impl_;
-}
+MlemBatch::~MlemBatch() {}
 
 bool MlemBatch::eligible(const std::vector<const FastPlan*>& plans)
 {
     if (plans.empty() || plans.size() > size_t(BATCH_MAX_PLANOGRAMS))
         return false;
-    for (const FastPlan* fp : plans) {
-        if (!fp || fp->C > 4 || fp->Na > 128 || fp->Ns > 128 || fp->J != FWD_J || fp->R != 1 || fp->SB != 8)
+    for (const FastPlan* fp : plans)
+        if (!fp || fp->C > 4 || fp->Na > 128 || fp->Ns > BATCH_H2_THREADS || fp->J != FWD_J || fp->R != 1 ||
+            fp->SB != 8)
             return false;
-        if (batch_back_lds(*fp) > size_t(BATCH_BACK_LDS))
-            return false;
-    }
     return true;
 }
 
 void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
 {
-    if (jobs.size() > size_t(BATCH_MAX_REQUESTS))
-        throw hip_error("batched MLEM: too many requests in one iteration");
-    Impl& I = *impl_;
-    const int slot = I.next;
-    I.next = (I.next + 1) % Impl::RING;
-    PETL_HIP_CHECK(hipEventSynchronize(I.copied[slot]));  // the copy that last read this host table is done
-    BatchTable& T = *I.host[slot];
-    T.nf = 0;
-    T.nu = 0;
-    T.fs[0] = 0;
-    T.us[0] = 0;
-    for (int p = 0; p < BATCH_MAX_PLANOGRAMS; p++) {
-        T.nb[p] = 0;
-        T.hs[p][0] = 0;
-        T.zs[p][0] = 0;
-    }
+    std::vector<BFwd> fw;
+    std::vector<const FastPlan*> fplan;
+    std::vector<BBack> bk[BATCH_MAX_PLANOGRAMS];
+    std::vector<const FastPlan*> bplan[BATCH_MAX_PLANOGRAMS];
+    std::vector<BUpd> up;
     for (const BatchIterJob& jb : jobs) {
         std::vector<const FastPlan*> pl(jb.plan, jb.plan + jb.nplans);
         if (!eligible(pl))
             throw hip_error("batched MLEM: a request outside the batched form");
         for (int p = 0; p < jb.nplans; p++) {
             const FastPlan& fp = *jb.plan[p];
-            BFwd& F = T.fwd[T.nf];
+            BFwd F{};
             F.A = fwd_args(fp, 0, 1);
             F.g = jb.ratio[p];
             F.f = jb.f_in;
@@ -1752,10 +1657,9 @@ void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
             F.y = jb.data[p];
             F.ngroups = fp.ngroups;
             F.nu1 = fp.P.Nu1;
-            T.fs[T.nf + 1] = T.fs[T.nf] + fp.ngroups * fp.P.Nu1 * fp.P.Nv1;
-            T.nf++;
-            const int nb = T.nb[p];
-            BBack& B = T.back[p][nb];
+            fw.push_back(F);
+            fplan.push_back(&fp);
+            BBack B{};
             B.A = back_args(fp, BACK_PETL, 0, 1);
             B.g = jb.ratio[p];
             B.h2 = jb.h2;
@@ -1770,11 +1674,10 @@ void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
             B.nac = (fp.Na + 63) / 64;
             B.nzc = (fp.Nz + BATCH_ZC - 1) / BATCH_ZC;
             B.overwrite = (p == 0) ? 1 : 0;  // d = the first planogram's sum, then += the next ones
-            T.hs[p][nb + 1] = T.hs[p][nb] + B.gx * fp.P.Nu1 * fp.P.Nv1;
-            T.zs[p][nb + 1] = T.zs[p][nb] + B.nac * fp.Ns * B.nzc;
-            T.nb[p]++;
+            bk[p].push_back(B);
+            bplan[p].push_back(&fp);
         }
-        BUpd& U = T.upd[T.nu];
+        BUpd U{};
         U.fo = jb.f_out;
         U.fi = jb.f_in;
         U.d = jb.d;
@@ -1786,7 +1689,6 @@ void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
         U.prior = (jb.beta > 0.0f) ? jb.prior : 0;
         U.delta = jb.delta;
         U.beta = jb.beta;
-        U.h = HuberParams{};
         if (U.prior == 2) {
             float dl = jb.delta, bt = jb.beta;
             U.h = huber_params(dl, bt, jb.p);
@@ -1795,12 +1697,30 @@ void MlemBatch::launch(const std::vector<BatchIterJob>& jobs, hipStream_t s)
         }
         if (U.prior == 0 && U.fo != U.fi)
             throw hip_error("batched MLEM: the update without a prior is in place");
-        T.us[T.nu + 1] = T.us[T.nu] + U.nk * U.N2 * U.N1;
-        T.nu++;
+        up.push_back(U);
     }
-    PETL_HIP_CHECK(hipMemcpyAsync(I.dev, &T, sizeof(BatchTable), hipMemcpyHostToDevice, s));
-    PETL_HIP_CHECK(hipEventRecord(I.copied[slot], s));
-    PETL_HIP_CHECK(hipGraphLaunch(I.exec, s));
+    // forward with the ratio epilogue of every (request, planogram)
+    size_t base = 0;
+    for (size_t b = 0; b < fw.size(); b += PhaseArgs<BFwd>::MAXJ) {
+        const size_t e = std::min(fw.size(), b + PhaseArgs<BFwd>::MAXJ);
+        std::vector<BFwd> part(fw.begin() + long(b), fw.begin() + long(e));
+        launch_phase(batch_fwd_kernel, part, [](const BFwd& F) { return F.ngroups * F.nu1 * F.A.P.Nv1; },
+                     [&](int k) { return batch_fwd_lds(*fplan[b + size_t(k)]); }, FWD_THREADS, s);
+    }
+    (void)base;
+    // back projection, planogram 0 then 1: the u0 gather, then the u1 gather into d
+    for (int p = 0; p < BATCH_MAX_PLANOGRAMS; p++) {
+        for (size_t b = 0; b < bk[p].size(); b += PhaseArgs<BBack>::MAXJ) {
+            const size_t e = std::min(bk[p].size(), b + PhaseArgs<BBack>::MAXJ);
+            std::vector<BBack> part(bk[p].begin() + long(b), bk[p].begin() + long(e));
+            launch_phase(batch_h2_kernel, part, [](const BBack& B) { return B.gx * B.nu1 * B.A.P.Nv1; },
+                         [&](int k) { return batch_back_lds(*bplan[p][b + size_t(k)]); }, BATCH_H2_THREADS, s);
+            launch_phase(batch_z_kernel, part, [](const BBack& B) { return B.nac * B.A.Ns * B.nzc; },
+                         [](int) { return size_t(0); }, 64, s);
+        }
+    }
+    launch_phase(batch_update_kernel, up, [](const BUpd& U) { return U.nk * U.N2 * U.N1; }, [](int) { return size_t(0); },
+                 64, s);
 }
 
 }  // namespace petl

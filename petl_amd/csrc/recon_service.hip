@@ -57,7 +57,7 @@ struct Slot {
     std::string sig;                       // geometry and MLEM parameters the resources were built for
     size_t bytes = 0;                      // device bytes the resources hold
     float cap_sy = 0.0f, cap_sf = 0.0f;    // binary16 data / image scales the captured iteration holds
-    bool batched = false;                  // runs in the batched persistent iterations, not its own graph
+    bool batched = false;                  // runs in the batched iterations, not its own graph
     hipEvent_t fetched = nullptr;          // batched: the image copy to `pinned` is done
 };
 
@@ -433,7 +433,7 @@ private:
             s.job->setup(req->y.data(), s.pinned, nullptr, nullptr, s.stream);
             PETL_HIP_CHECK(hipStreamSynchronize(s.stream));
             const Clock::time_point t2 = Clock::now();
-            // requests the batched persistent iterations take join them; the others replay a graph of their own
+            // requests the batched iterations take join them; the others replay a graph of their own
             s.batched = batch_ && s.job->batch_eligible();
             if (s.batched)
                 PETL_HIP_CHECK(hipEventCreateWithFlags(&s.fetched, hipEventDisableTiming));
@@ -691,7 +691,7 @@ private:
     long pool_hits_ = 0;
     long recaptures_ = 0;  // pooled iterations captured again for other binary16 scales
     long batched_requests_ = 0, batch_rounds_ = 0, batch_peak_ = 0;
-    // batched persistent iterations (null when the service was created without them)
+    // batched iterations (null when the service was created without them)
     static constexpr int ROUND_EVENTS = 4;
     std::unique_ptr<MlemBatch> batch_;
     hipStream_t bstream_ = nullptr;

@@ -95,7 +95,7 @@ def make_request(psis, D, L, H, T, n, voxel, y, num_iter, v_m0=None, v_m1=None, 
 class ReconService:
     def __init__(self, hbm_budget_bytes=0.0, max_slots=8, batched=False):
         """batched: requests that allow it (fp32 MLEM on the fast kernels, see petl_service_create_ex) run in
-        batched persistent iterations, one launch per phase for all of them."""
+        batched iterations, one launch per phase for all of them."""
         self.lib = _declare(load_library())
         self.h = self.lib.petl_service_create_ex(float(hbm_budget_bytes), int(max_slots), 1 if batched else 0)
         if not self.h:

@@ -237,7 +237,7 @@ def test_poisson_arrivals_over_all_panel_sizes():
 
 
 # ---------------------------------------------------------------------------------------------------------
-# batched persistent iterations (ReconService(batched=True))
+# batched iterations (ReconService(batched=True))
 # ---------------------------------------------------------------------------------------------------------
 BATCH_MIX = [
     ("p48", 81, 3, {}),
