@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include "mlem_job.h"
+#include "proj_general.h"
 #include "proj_kernels.h"
 #include "vec_kernels.h"
 
@@ -81,11 +82,13 @@ size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
     for (const auto& pl : m.plans)  // the fast back projector's workspace, the largest of the planograms
         if (m.path == 0)
             back_ws = std::max(back_ws, fast_workspace_estimate(pl.geom(), V));
-    // the general forward's transposed volume copy
+    // the general forward's copies of the volume: the transposed one (per-bin kernels) and the two padded
+    // ones (separable kernels); which of them a model's planograms use is decided when their plans are built,
+    // so both are counted
     size_t vol_t = 0;
     for (const auto& pl : m.plans)
         if (m.path != 0 || fast_workspace_estimate(pl.geom(), V) == 0)
-            vol_t = size_t(m.vol_elems()) * sizeof(float);
+            vol_t = (size_t(m.vol_elems()) + 2 * gen_padded_elems(V)) * sizeof(float);
     const size_t vol = size_t(m.vol_elems()) * sizeof(float);
     const size_t es = storage_bytes(prm.storage), ei = storage_bytes(prm.image_storage);
     const bool st16 = prm.storage != STORE_F32 || prm.image_storage != STORE_F32;
@@ -100,7 +103,8 @@ size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
 size_t MlemJob::allocated_bytes() const
 {
     const DeviceState& D = m_.dev;
-    size_t n = D.f.bytes + D.f2.bytes + D.d.bytes + D.S.bytes + D.h2.bytes + D.stage.bytes + D.red.bytes + D.fT.bytes;
+    size_t n = D.f.bytes + D.f2.bytes + D.d.bytes + D.S.bytes + D.h2.bytes + D.stage.bytes + D.red.bytes + D.fT.bytes +
+               D.fpad.bytes;
     for (const auto* v : {&D.y, &D.aux, &D.resp, &D.g})
         for (const DevBuf& b : *v)
             n += b.bytes;

@@ -54,6 +54,7 @@ struct DeviceState {
     DevBuf h2;                                    // fast back projector workspace
     size_t h2_need = 0;                           // bytes it needs for one planogram (the largest)
     DevBuf fT;                                    // the volume with x and y swapped (general forward)
+    DevBuf fpad;                                  // the separable general forward's two padded copies
     DevBuf red;                                   // reduction scratch
     BinScratch bin;                               // list-mode binning
     DevBuf events, counter;                       // staged coincidences, accepted-event counters
@@ -92,8 +93,13 @@ struct Model {
     // allocates the fast back projector's workspace (one planogram at a time) if it is not there yet
     void ensure_workspace();
     bool uses_fast(int i);
-    // the general forward of planogram i (transposing f into dev.fT first when `transpose` is set)
-    void gen_forward_one(size_t i, float* g, const float* f, bool transpose, int epilogue, const float* y,
+    // Which copies of the volume the general forwards of one call have made so far (dev.fT transposed,
+    // dev.fpad padded): one of each serves every general planogram of the call.
+    struct GenCopies {
+        bool transposed = false, padded = false;
+    };
+    // the general forward of planogram i, making the copy of f its kernel reads first unless `made` has it
+    void gen_forward_one(size_t i, float* g, const float* f, GenCopies& made, int epilogue, const float* y,
                          const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets);
     // g_i = A_i f for all planograms (device pointers), with an optional epilogue (ratio, or residual with
     // the weights wts; the y, resp, wts lists may be null).  v1 subset (first, stride) restricts the views;

@@ -207,32 +207,46 @@ bool Model::uses_fast(int i)
     return i >= 0 && i < int(dev.fast.size()) && dev.fast[i] != nullptr;
 }
 
-void Model::gen_forward_one(size_t i, float* g, const float* f, bool transpose, int epilogue, const float* y,
+void Model::gen_forward_one(size_t i, float* g, const float* f, GenCopies& made, int epilogue, const float* y,
                             const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets)
 {
     const GenPlan& gp = *dev.gen[i];
-    if (gp.any_xs && transpose) {
-        dev.fT.ensure(size_t(vol_elems()) * sizeof(float));
-        launch_transpose_xy(f, dev.fT.f(), vgeom(), s);
+    const VolGeom V = vgeom();
+    GenVolume vol;
+    vol.f = f;
+    if (gp.sep) {
+        const size_t n = gen_padded_elems(V);
+        if (!made.padded) {
+            dev.fpad.ensure(2 * n * sizeof(float));
+            launch_pad_volumes(f, dev.fpad.f(), dev.fpad.f() + n, V, s);
+            made.padded = true;
+        }
+        vol.fpy = dev.fpad.f();
+        vol.fpx = dev.fpad.f() + n;
+    } else if (gp.any_xs) {
+        if (!made.transposed) {
+            dev.fT.ensure(size_t(vol_elems()) * sizeof(float));
+            launch_transpose_xy(f, dev.fT.f(), V, s);
+            made.transposed = true;
+        }
+        vol.fT = dev.fT.f();
     }
-    gen_forward(gp, g, f, gp.any_xs ? dev.fT.f() : nullptr, epilogue, y, resp, wts, s, subset, nsubsets);
+    gen_forward(gp, g, vol, epilogue, y, resp, wts, s, subset, nsubsets);
 }
 
 void Model::forward(float* const* g, const float* f, int epilogue, float* const* y, float* const* resp, hipStream_t s,
                     int subset, int nsubsets, float* const* wts)
 {
     prepare();
-    bool transposed = false;  // one transposed copy of f serves every general planogram of the call
+    GenCopies made;
     for (size_t i = 0; i < plans.size(); i++) {
         const float* yi = y ? y[i] : nullptr;
         const float* ri = resp ? resp[i] : nullptr;
         const float* wi = wts ? wts[i] : nullptr;
-        if (dev.fast[i]) {
+        if (dev.fast[i])
             fast_forward(*dev.fast[i], g[i], f, epilogue, yi, ri, s, subset, nsubsets, wi);
-        } else {
-            gen_forward_one(i, g[i], f, !transposed, epilogue, yi, ri, wi, s, subset, nsubsets);
-            transposed = transposed || dev.gen[i]->any_xs;
-        }
+        else
+            gen_forward_one(i, g[i], f, made, epilogue, yi, ri, wi, s, subset, nsubsets);
     }
 }
 
@@ -283,8 +297,10 @@ void Model::forward_batch(float* const* g, const float* f, int batch, hipStream_
             continue;
         }
         const int64_t G = plans[i].elems();
-        for (int b = 0; b < batch; b++)
-            gen_forward_one(i, g[i] + b * G, f + b * nv, true, EPI_NONE, nullptr, nullptr, nullptr, s, 0, 1);
+        for (int b = 0; b < batch; b++) {
+            GenCopies made;  // each batch element's own copies
+            gen_forward_one(i, g[i] + b * G, f + b * nv, made, EPI_NONE, nullptr, nullptr, nullptr, s, 0, 1);
+        }
     }
 }
 
@@ -315,15 +331,14 @@ void Model::forward_st(int sf, int sg, void* const* g, const void* f, int epilog
                        hipStream_t s, int subset, int nsubsets, float f_scale, float y_scale, float g_scale)
 {
     prepare();
-    bool transposed = false;
+    GenCopies made;
     for (size_t i = 0; i < plans.size(); i++) {
         if (!dev.fast[i] && (sf != STORE_F32 || sg != STORE_F32))
             throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
         if (!dev.fast[i]) {
-            gen_forward_one(i, static_cast<float*>(g[i]), static_cast<const float*>(f), !transposed, epilogue,
+            gen_forward_one(i, static_cast<float*>(g[i]), static_cast<const float*>(f), made, epilogue,
                             y ? static_cast<const float*>(y[i]) : nullptr, resp ? resp[i] : nullptr, nullptr, s,
                             subset, nsubsets);
-            transposed = transposed || dev.gen[i]->any_xs;
             continue;
         }
         fast_forward_st(*dev.fast[i], sf, sg, g[i], f, epilogue, y ? y[i] : nullptr, resp ? resp[i] : nullptr, s, subset,

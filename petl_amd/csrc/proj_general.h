@@ -20,7 +20,16 @@
 //
 // The forward and the matched back projector evaluate the crossings and overlaps with the same
 // functions on the same table entries, and a tap counts exactly when both overlaps are positive, so the
-// matched back projector is the exact transpose of the forward.  The PETL-mode back projector follows the
+// matched back projector is the exact transpose of the forward.
+//
+// Separable form.  When a u1 row steps the z crossing by exactly one voxel (Bz = 1: crystal pitch = voxel
+// height) and a bin's footprint is at most one voxel wide on both axes (hb, hz <= 1/2), a bin meets at
+// most two cells per axis in each slice: i0 = floor(xi + 1/2 - hb) and i0 + 1 along a, k0 + iu1 and
+// k0 + iu1 + 1 along z with k0 = floor(zb + 1/2 - hz) taken from row 0's crossing zb = Z0 + Bs s + Bu iu0,
+// and the z overlaps are row 0's for every row.  Those kernels (gen_sep = true) walk each bin column up
+// the u1 rows with a sliding window of z values read from zero-padded volume copies, and the matched back
+// projector applies the same rule per voxel, so the pair stays an exact transpose; other pitches use the
+// per-bin kernels.  The PETL-mode back projector follows the
 // reference's voxel-driven semantics instead (src/projectors.cu:240-377): per voxel and view row v1 the
 // voxel-centre u1 test and u1 interval, per v0 the projection of the voxel's mid-line along the dominant
 // axis onto u0, exact bilinear weights.
@@ -29,6 +38,9 @@
 #include "petl_common.h"
 
 namespace petl {
+
+constexpr int GEN_RB = 16;          // u1 rows per pass of the separable forward (register accumulators)
+constexpr int GEN_PZ = GEN_RB + 2;  // zero planes below and above the volume in its padded copies
 
 struct GenView {
     float Au;     // d xi / d iu0
@@ -45,6 +57,7 @@ struct GenPlan {
     int Nsmax = 0;         // max(Nx, Ny): row length of xi0
     float Bz = 0, hz = 0;  // d zeta / d iu1; half height of a bin's footprint in voxels
     bool any_xs = false;   // some view takes x planes (reads the transposed volume)
+    bool sep = false;      // the separable kernels apply (Bz = 1, hb and hz at most 1/2)
     GenView* views = nullptr;   // [Nv1][Nv0]
     float* xi0 = nullptr;       // [Nv0][Nsmax]
     float4* pcoef = nullptr;    // [Nv0] PETL back: {Cx, Cy, C0, h0}: u0 centre = x Cx + y Cy + C0 (bins)
@@ -54,13 +67,29 @@ struct GenPlan {
 
 std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V);
 
-// fT = f with x and y swapped ([z][x][y]); the forward reads x-plane views from it so that the footprint
-// axis is the contiguous one.
+// fT = f with x and y swapped ([z][x][y]); the per-bin forward reads x-plane views from it so that the
+// footprint axis is the contiguous one.
 void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_t s);
 
+// The separable forward's zero-padded copies of f: fpy[z + GEN_PZ][y][x + 1] (y planes) and
+// fpx[z + GEN_PZ][x][y + 1] (x planes), each gen_padded_elems(V) floats with GEN_PZ planes of zeros below
+// and above the volume and one zero column on each side of the footprint axis (every element written, no
+// separate clear).
+size_t gen_padded_elems(const VolGeom& V);
+void launch_pad_volumes(const float* f, float* fpy, float* fpx, const VolGeom& V, hipStream_t s);
+
+// The volume forms a general forward reads: f itself, fT (launch_transpose_xy) for the per-bin kernels
+// when gp.any_xs, the padded copies (launch_pad_volumes) for the separable ones.
+struct GenVolume {
+    const float* f = nullptr;
+    const float* fT = nullptr;
+    const float* fpy = nullptr;
+    const float* fpx = nullptr;
+};
+
 // g = A f (+ epilogue: EPI_RATIO g = resp y / (resp A f), EPI_RESID g = (resp A f - y) w), v1 rows
-// v1_first, v1_first + v1_stride, ...; fT: launch_transpose_xy(f) when gp.any_xs, else unused.
-void gen_forward(const GenPlan& gp, float* g, const float* f, const float* fT, int epilogue, const float* y,
+// v1_first, v1_first + v1_stride, ...
+void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue, const float* y,
                  const float* resp, const float* wts, hipStream_t s, int v1_first = 0, int v1_stride = 1);
 // f += A^T g in mode BACK_PETL or BACK_MATCHED (v1 rows as above).
 void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first = 0,

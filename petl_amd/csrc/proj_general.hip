@@ -3,6 +3,11 @@
 // projector.  See proj_general.h for the model.
 //
 // Kernels:
+//   gen_fwd_sep_kernel      separable form (see proj_general.h): one wave per view (v1, v0) and tile of 64
+//                           u0 bins (lanes); each lane walks its bin column up the u1 rows in passes of
+//                           GEN_RB rows, per slice reading GEN_RB + 1 z values of each of its two a cells
+//                           from the padded copy and accumulating four taps per row;
+//   gen_back_matched_sep_kernel  its exact transpose, gathered per voxel;
 //   gen_fwd_kernel          a workgroup is one view (v1, v0) x a tile of 64 u0 bins (lanes) x 4 u1 rows
 //                           (waves); each thread walks the slices for its bin, reading the volume through
 //                           the layout whose footprint axis is contiguous (f for y planes, the transposed
@@ -133,6 +138,227 @@ __global__ void __launch_bounds__(256) gen_fwd_kernel(float* __restrict__ g, con
         }
     }
     g[gi] = out;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// separable forward
+// ---------------------------------------------------------------------------------------------------------
+// word 3 of a raw buffer resource on gfx9-family GPUs (32-bit untyped dword accesses)
+constexpr int GEN_BUFFER_FLAGS = 0x00020000;
+
+// The taps of the separable form: cells c0 and c0 + 1 of the axis with the overlaps of [c - h, c + h]
+// (h <= 1/2 meets no other cell; a negative overlap is a cell the interval misses).
+__device__ __forceinline__ int sep_cell(float c, float h) { return int(floorf(c + (0.5f - h))); }
+__device__ __forceinline__ float sep_weight(float c, float h, int cell) { return fmaxf(0.0f, overlap(c, h, float(cell))); }
+
+template <int RB>
+__global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, const float* __restrict__ fpy,
+                                                         const float* __restrict__ fpx, GenArgs A, int epilogue,
+                                                         const float* __restrict__ y,
+                                                         const float* __restrict__ resp,
+                                                         const float* __restrict__ wts)
+{
+    constexpr int PZ = GEN_PZ;
+    static_assert(PZ >= RB + 2, "the padded copies must hold every z window of a pass");
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int lane = threadIdx.x;
+    const int iu0 = blockIdx.x * 64 + lane;
+    const int iv0 = blockIdx.y, iv1 = A.v1_first + (A.v1_chunk + int(blockIdx.z)) * A.v1_stride;
+    const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
+    const int Na = w.xs ? V.Ny : V.Nx, Ns = w.xs ? V.Nx : V.Ny, NaP = Na + 2;
+    const int plane = Ns * NaP;  // one z plane of the padded copy
+    // buffer resource over the copy this view reads (its size in bytes fits 32 bits: the plan checks the
+    // padded element count); loads past it would return zero, and every load here stays inside it
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(w.xs ? fpx : fpy), 0, 4 * (V.Nz + 2 * PZ) * plane, GEN_BUFFER_FLAGS);
+    const float v0 = coord(iv0, P.Tv0, P.v0_0), v1 = coord(iv1, P.Tv1, P.v1_0);
+    const bool lane_live = iu0 < P.Nu0 && !mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0);
+    const int iu0c = min(iu0, P.Nu0 - 1);
+    const float* xr = A.xi0 + int64_t(iv0) * A.Nsmax;
+    const float hb = w.hb, hz = A.hz;
+    const int64_t grow = (int64_t(iv1) * P.Nv0 + iv0) * P.Nu1 * int64_t(P.Nu0);
+    for (int r0 = 0; r0 < P.Nu1; r0 += RB) {
+        // bit r: row r0 + r exists and is not masked (one scalar word for the pass)
+        unsigned rbits = 0;
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+            if (r0 + r < P.Nu1 && !mask_u1(P, coord(r0 + r, P.Tu1, P.u1_0), v1))
+                rbits |= 1u << r;
+        rbits = __builtin_amdgcn_readfirstlane(rbits);
+        const bool any_row = rbits != 0;
+        float acc[RB];
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+            acc[r] = 0.0f;
+        if (any_row && __any(lane_live)) {
+            float xs_next = xr[0];
+            for (int s = 0; s < Ns; s++) {
+                // the next slice's table entry is requested a slice ahead
+                const float xs_cur = xs_next;
+                xs_next = xr[min(s + 1, Ns - 1)];
+                const float xi = cross_a(w, xs_cur, iu0c);
+                const int i0 = sep_cell(xi, hb);
+                const float zb = cross_z_base(w, iu0c, s);
+                const int k0 = sep_cell(zb, hz);
+                const int kw = k0 + r0;  // the pass reads z = kw ... kw + RB
+                // a window that reaches a cell of the volume lies inside the padded copy; one that does not
+                // contributes nothing and reads a clamped in-range column instead
+                const bool live = lane_live && i0 >= -1 && i0 <= Na - 1 && kw >= -(RB + 1) && kw <= V.Nz;
+                if (!__any(live))
+                    continue;
+                float wa0 = 0.0f, wa1 = 0.0f, wz0 = 0.0f, wz1 = 0.0f;
+                if (live) {
+                    wa0 = sep_weight(xi, hb, i0);
+                    wa1 = sep_weight(xi, hb, i0 + 1);
+                    wz0 = sep_weight(zb, hz, k0);
+                    wz1 = sep_weight(zb, hz, k0 + 1);
+                }
+                const float W00 = wa0 * wz0, W01 = wa0 * wz1, W10 = wa1 * wz0, W11 = wa1 * wz1;
+                // byte offset of the lane's cell (kw, s, i0) in the padded copy; plane t of the window adds
+                // the uniform t * plane floats as the load's scalar offset
+                const int voff = 4 * (((live ? kw : 0) + PZ) * plane + s * NaP + (live ? i0 : 0) + 1);
+                float va[RB + 1], vb[RB + 1];
+#pragma unroll
+                for (int t = 0; t <= RB; t++) {
+                    va[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 4 * t * plane, 0));
+                    vb[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4, 4 * t * plane, 0));
+                }
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    acc[r] = fmaf(va[r], W00, acc[r]);
+                    acc[r] = fmaf(vb[r], W10, acc[r]);
+                    acc[r] = fmaf(va[r + 1], W01, acc[r]);
+                    acc[r] = fmaf(vb[r + 1], W11, acc[r]);
+                }
+            }
+        }
+        if (iu0 >= P.Nu0)
+            continue;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            if (r0 + r >= P.Nu1)
+                break;
+            const int64_t gi = grow + int64_t(r0 + r) * P.Nu0 + iu0;
+            float out = (lane_live && (rbits >> r & 1u)) ? acc[r] * w.scale : 0.0f;
+            {
+#pragma clang fp contract(off)
+                if (resp)
+                    out = out * resp[gi];
+                if (epilogue == EPI_RATIO) {
+                    // dataCube RDIV (src/data_cube.cpp:150-155): y / Pf, 0 where Pf == 0
+                    out = (out == 0.0f) ? 0.0f : y[gi] / out;
+                    if (resp)
+                        out = out * resp[gi];
+                } else if (epilogue == EPI_RESID) {
+                    // Pf - g, weighted in place (src/petl_server.py:1790-1792)
+                    out = out - y[gi];
+                    if (wts)
+                        out = out * wts[gi];
+                }
+            }
+            g[gi] = out;
+        }
+    }
+}
+
+// the exact transpose of gen_fwd_sep_kernel, gathered per voxel: the bins whose two a cells include this
+// voxel's column (candidates from the inverted crossing, each re-evaluated with the forward's functions),
+// then the (at most two) u1 rows whose z window places this voxel at tap 0 or 1.  f += A^T g.
+__global__ void __launch_bounds__(256) gen_back_matched_sep_kernel(const float* __restrict__ g,
+                                                                   float* __restrict__ f, GenArgs A)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63), iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int iz = blockIdx.z;
+    if (ix >= V.Nx || iy >= V.Ny)
+        return;
+    const float hz = A.hz;
+    const int64_t row = int64_t(P.Nu1) * P.Nu0;
+    float sum = 0.0f;
+    for (int iv1 = A.v1_first; iv1 < P.Nv1; iv1 += A.v1_stride) {
+        const float v1 = coord(iv1, P.Tv1, P.v1_0);
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
+            const int i = w.xs ? iy : ix, s = w.xs ? ix : iy;
+            const float fi = float(i), hb = w.hb;
+            const float xs0 = A.xi0[int64_t(iv0) * A.Nsmax + s];
+            float lo = (fi - hb - 0.5f - xs0) / w.Au, hi = (fi + hb + 0.5f - xs0) / w.Au;
+            if (lo > hi) {
+                const float t = lo;
+                lo = hi;
+                hi = t;
+            }
+            const int b0 = max(0, int(floorf(lo)) - 1), b1 = min(P.Nu0 - 1, int(ceilf(hi)) + 1);
+            if (b0 > b1)
+                continue;
+            const float v0 = coord(iv0, P.Tv0, P.v0_0);
+            const float* gv = g + (int64_t(iv1) * P.Nv0 + iv0) * row;
+            float acc = 0.0f;
+            for (int iu0 = b0; iu0 <= b1; iu0++) {
+                const float xi = cross_a(w, xs0, iu0);
+                const int ta = i - sep_cell(xi, hb);
+                if (ta < 0 || ta > 1)
+                    continue;
+                const float wa = sep_weight(xi, hb, i);
+                if (!(wa > 0.0f) || mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0))
+                    continue;
+                const float zb = cross_z_base(w, iu0, s);
+                const int k0 = sep_cell(zb, hz);
+#pragma unroll
+                for (int tz = 0; tz < 2; tz++) {
+                    const int iu1 = iz - k0 - tz;
+                    if (iu1 < 0 || iu1 >= P.Nu1)
+                        continue;
+                    const float wz = sep_weight(zb, hz, k0 + tz);
+                    if (!(wz > 0.0f) || mask_u1(P, coord(iu1, P.Tu1, P.u1_0), v1))
+                        continue;
+                    acc += gv[int64_t(iu1) * P.Nu0 + iu0] * (wa * wz);
+                }
+            }
+            sum += acc * w.scale;
+        }
+    }
+    f[(int64_t(iz) * V.Ny + iy) * V.Nx + ix] += sum;
+}
+
+// fpy[z + PZ][y][x + 1] and fpx[z + PZ][x][y + 1] from f[z][y][x], zeros elsewhere: one thread per element
+// of the larger padded plane set, 32 x 32 tiles through LDS for the transposed copy
+__global__ void __launch_bounds__(256) pad_volumes_kernel(const float* __restrict__ f, float* __restrict__ fpy,
+                                                          float* __restrict__ fpx, int Nx, int Ny, int Nz)
+{
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int zp = blockIdx.z, z = zp - GEN_PZ;  // padded plane
+    const bool zin = z >= 0 && z < Nz;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int64_t zo = int64_t(z) * Nx * Ny;
+    // fpy: row y, column x + 1 (columns 0 and Nx + 1 are zero)
+    for (int r = ty; r < 32; r += 8) {
+        const int x = bx + tx, yy = by + r;
+        float v = 0.0f;
+        if (zin && x < Nx && yy < Ny)
+            v = f[zo + int64_t(yy) * Nx + x];
+        tile[r][tx] = v;
+        if (yy < Ny && x < Nx)
+            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2) + x + 1] = v;
+        if (yy < Ny && x == 0)
+            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2)] = 0.0f;
+        if (yy < Ny && x == Nx - 1)
+            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2) + Nx + 1] = 0.0f;
+    }
+    __syncthreads();
+    // fpx: row x, column y + 1 (columns 0 and Ny + 1 are zero)
+    for (int r = ty; r < 32; r += 8) {
+        const int yy = by + tx, x = bx + r;
+        if (x < Nx && yy < Ny)
+            fpx[(int64_t(zp) * Nx + x) * (Ny + 2) + yy + 1] = tile[tx][r];
+        if (x < Nx && yy == 0)
+            fpx[(int64_t(zp) * Nx + x) * (Ny + 2)] = 0.0f;
+        if (x < Nx && yy == Ny - 1)
+            fpx[(int64_t(zp) * Nx + x) * (Ny + 2) + Ny + 1] = 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -342,6 +568,7 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
         double pa, ps, rs, Ta, S0, Ts, dom_inv;
     };
     std::vector<PerV0> pv(P.Nv0);
+    float max_hb = 0.0f;
     for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
         const float v0f = coord_host(iv0, P.Tv0, P.v0_0);
         // slice axis: the reference's float decision (src/projectors.cu:77)
@@ -385,7 +612,10 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
             w.hb = hb;
         }
         gp->any_xs = gp->any_xs || xs;
+        max_hb = std::max(max_hb, hb);
     }
+    gp->sep = gp->Bz == 1.0f && gp->hz <= 0.5f && max_hb <= 0.5f &&
+              gen_padded_elems(V) < (size_t(1) << 29);  // byte offsets of the padded copies fit 31 bits
     for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
         const double v1 = coord_host(iv1, P.Tv1, P.v1_0);
         for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
@@ -426,19 +656,44 @@ void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_
     PETL_LAUNCH_CHECK();
 }
 
-void gen_forward(const GenPlan& gp, float* g, const float* f, const float* fT, int epilogue, const float* y,
+size_t gen_padded_elems(const VolGeom& V)
+{
+    return size_t(V.Nz + 2 * GEN_PZ) * V.Ny * (V.Nx + 2) > size_t(V.Nz + 2 * GEN_PZ) * V.Nx * (V.Ny + 2)
+               ? size_t(V.Nz + 2 * GEN_PZ) * V.Ny * (V.Nx + 2)
+               : size_t(V.Nz + 2 * GEN_PZ) * V.Nx * (V.Ny + 2);
+}
+
+void launch_pad_volumes(const float* f, float* fpy, float* fpx, const VolGeom& V, hipStream_t s)
+{
+    hipLaunchKernelGGL(pad_volumes_kernel, dim3((V.Nx + 31) / 32, (V.Ny + 31) / 32, V.Nz + 2 * GEN_PZ), dim3(256), 0,
+                       s, f, fpy, fpx, V.Nx, V.Ny, V.Nz);
+    PETL_LAUNCH_CHECK();
+}
+
+void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue, const float* y,
                  const float* resp, const float* wts, hipStream_t s, int v1_first, int v1_stride)
 {
-    if (gp.any_xs && !fT)
-        throw hip_error("general forward: the transposed volume is missing");
     GenArgs A = args_of(gp, v1_first, v1_stride);
     const PlanGeom& P = gp.P;
-    const int tiles = ((P.Nu0 + 63) / 64) * ((P.Nu1 + 3) / 4);
     const int nv1 = (P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+    if (gp.sep) {
+        if (!vol.fpy || !vol.fpx)
+            throw hip_error("general forward: the padded volume copies are missing");
+        for (int c = 0; c < nv1; c += 65535) {
+            A.v1_chunk = c;
+            hipLaunchKernelGGL(gen_fwd_sep_kernel<GEN_RB>, dim3((P.Nu0 + 63) / 64, P.Nv0, std::min(65535, nv1 - c)),
+                               dim3(64), 0, s, g, vol.fpy, vol.fpx, A, epilogue, y, resp, wts);
+            PETL_LAUNCH_CHECK();
+        }
+        return;
+    }
+    if (gp.any_xs && !vol.fT)
+        throw hip_error("general forward: the transposed volume is missing");
+    const int tiles = ((P.Nu0 + 63) / 64) * ((P.Nu1 + 3) / 4);
     for (int c = 0; c < nv1; c += 65535) {
         A.v1_chunk = c;
-        hipLaunchKernelGGL(gen_fwd_kernel, dim3(tiles, P.Nv0, std::min(65535, nv1 - c)), dim3(256), 0, s, g, f, fT, A,
-                           epilogue, y, resp, wts);
+        hipLaunchKernelGGL(gen_fwd_kernel, dim3(tiles, P.Nv0, std::min(65535, nv1 - c)), dim3(256), 0, s, g, vol.f,
+                           vol.fT, A, epilogue, y, resp, wts);
         PETL_LAUNCH_CHECK();
     }
 }
@@ -447,7 +702,10 @@ void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t
 {
     const GenArgs A = args_of(gp, v1_first, v1_stride);
     const VolGeom& V = gp.V;
-    if (mode == BACK_MATCHED) {
+    if (mode == BACK_MATCHED && gp.sep) {
+        hipLaunchKernelGGL(gen_back_matched_sep_kernel, dim3((V.Nx + 63) / 64, (V.Ny + 3) / 4, V.Nz), dim3(256), 0, s,
+                           g, f, A);
+    } else if (mode == BACK_MATCHED) {
         hipLaunchKernelGGL(gen_back_matched_kernel, dim3((V.Nx + 63) / 64, (V.Ny + 3) / 4, V.Nz), dim3(256), 0, s, g,
                            f, A);
     } else {
