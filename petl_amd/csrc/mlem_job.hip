@@ -85,10 +85,14 @@ size_t MlemJob::device_bytes(Model& m, const MlemParams& prm)
     // the general forward's copies of the volume: the transposed one (per-bin kernels) and the two padded
     // ones (separable kernels); which of them a model's planograms use is decided when their plans are built,
     // so both are counted
-    size_t vol_t = 0;
+    // and the separable PETL back's partial volumes (the largest of the general planograms')
+    size_t vol_t = 0, gen_ws = 0;
     for (const auto& pl : m.plans)
-        if (m.path != 0 || fast_workspace_estimate(pl.geom(), V) == 0)
+        if (m.path != 0 || fast_workspace_estimate(pl.geom(), V) == 0) {
             vol_t = (size_t(m.vol_elems()) + 2 * gen_padded_elems(V)) * sizeof(float);
+            gen_ws = std::max(gen_ws, gen_back_workspace_estimate(pl.geom(), V));
+        }
+    vol_t += gen_ws;
     const size_t vol = size_t(m.vol_elems()) * sizeof(float);
     const size_t es = storage_bytes(prm.storage), ei = storage_bytes(prm.image_storage);
     const bool st16 = prm.storage != STORE_F32 || prm.image_storage != STORE_F32;
@@ -104,7 +108,7 @@ size_t MlemJob::allocated_bytes() const
 {
     const DeviceState& D = m_.dev;
     size_t n = D.f.bytes + D.f2.bytes + D.d.bytes + D.S.bytes + D.h2.bytes + D.stage.bytes + D.red.bytes + D.fT.bytes +
-               D.fpad.bytes;
+               D.fpad.bytes + D.genws.bytes;
     for (const auto* v : {&D.y, &D.aux, &D.resp, &D.g})
         for (const DevBuf& b : *v)
             n += b.bytes;

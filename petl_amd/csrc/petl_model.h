@@ -55,6 +55,7 @@ struct DeviceState {
     size_t h2_need = 0;                           // bytes it needs for one planogram (the largest)
     DevBuf fT;                                    // the volume with x and y swapped (general forward)
     DevBuf fpad;                                  // the separable general forward's two padded copies
+    DevBuf genws;                                 // the separable general PETL back's partial volumes
     DevBuf red;                                   // reduction scratch
     BinScratch bin;                               // list-mode binning
     DevBuf events, counter;                       // staged coincidences, accepted-event counters
@@ -101,6 +102,9 @@ struct Model {
     // the general forward of planogram i, making the copy of f its kernel reads first unless `made` has it
     void gen_forward_one(size_t i, float* g, const float* f, GenCopies& made, int epilogue, const float* y,
                          const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets);
+    // the separable general PETL back's partial volumes for planogram i (allocated on first use), or null
+    // when its kernel needs none (matched mode, per-voxel kernel)
+    float* gen_back_workspace(size_t i, int mode);
     // g_i = A_i f for all planograms (device pointers), with an optional epilogue (ratio, or residual with
     // the weights wts; the y, resp, wts lists may be null).  v1 subset (first, stride) restricts the views;
     // (0,1) = all.

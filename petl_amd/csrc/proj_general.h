@@ -22,6 +22,11 @@
 // functions on the same table entries, and a tap counts exactly when both overlaps are positive, so the
 // matched back projector is the exact transpose of the forward.
 //
+// The separable PETL-mode back projector (petl_np > 0, voxel height = crystal pitch) uses that a voxel's u1
+// interval moves by exactly one row per voxel height: per column it takes the pass's first height's two u1
+// rows and weights for every height of the pass (the voxel-centre test stays per height) and reads a
+// sliding window of planogram rows per u0 cell.
+//
 // Separable form.  When a u1 row steps the z crossing by exactly one voxel (Bz = 1: crystal pitch = voxel
 // height) and a bin's footprint is at most one voxel wide on both axes (hb, hz <= 1/2), a bin meets at
 // most two cells per axis in each slice: i0 = floor(xi + 1/2 - hb) and i0 + 1 along a, k0 + iu1 and
@@ -58,6 +63,7 @@ struct GenPlan {
     float Bz = 0, hz = 0;  // d zeta / d iu1; half height of a bin's footprint in voxels
     bool any_xs = false;   // some view takes x planes (reads the transposed volume)
     bool sep = false;      // the separable kernels apply (Bz = 1, hb and hz at most 1/2)
+    int petl_np = 0;       // u0 cells per column of the separable PETL back (2 or 3), 0: per-voxel kernel
     GenView* views = nullptr;   // [Nv1][Nv0]
     float* xi0 = nullptr;       // [Nv0][Nsmax]
     float4* pcoef = nullptr;    // [Nv0] PETL back: {Cx, Cy, C0, h0}: u0 centre = x Cx + y Cy + C0 (bins)
@@ -91,8 +97,13 @@ struct GenVolume {
 // v1_first, v1_first + v1_stride, ...
 void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue, const float* y,
                  const float* resp, const float* wts, hipStream_t s, int v1_first = 0, int v1_stride = 1);
-// f += A^T g in mode BACK_PETL or BACK_MATCHED (v1 rows as above).
+// f += A^T g in mode BACK_PETL or BACK_MATCHED (v1 rows as above).  workspace: at least
+// gen_back_workspace_bytes(gp) bytes for the separable PETL-mode kernel, which writes one partial volume per
+// chunk of v1 rows there and sums them in chunk order; null takes the per-voxel kernel instead.
 void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first = 0,
-              int v1_stride = 1);
+              int v1_stride = 1, float* workspace = nullptr);
+size_t gen_back_workspace_bytes(const GenPlan& gp);
+// the same figure from the geometry alone (before a plan is built)
+size_t gen_back_workspace_estimate(const PlanGeom& P, const VolGeom& V);
 
 }  // namespace petl

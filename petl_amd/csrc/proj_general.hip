@@ -323,6 +323,156 @@ __global__ void __launch_bounds__(256) gen_back_matched_sep_kernel(const float* 
     f[(int64_t(iz) * V.Ny + iy) * V.Nx + ix] += sum;
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// separable PETL-mode back projection.  One wave = 64 columns along the lane axis (x when |sin psi| <=
+// |cos psi|, so that the u1 offset v1 * t_col barely changes across the lanes, else y) x ZB heights x one
+// chunk of v1 rows; part[chunk] receives the chunk's sums (every element of it written), and
+// sum_partials_kernel adds T_x * sum over chunks to f.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int PETL_ZB = 16;  // heights per pass
+
+struct PetlSepArgs {
+    int lane_x;   // 1: lanes along x
+    int nch;      // chunks of v1 rows
+    int per;      // v1 rows (of the subset) per chunk
+    int nv1;      // v1 rows of the subset
+};
+
+template <int NP, int ZB>
+__global__ void __launch_bounds__(64) gen_back_petl_sep_kernel(const float* __restrict__ g, float* __restrict__ part,
+                                                               GenArgs A, PetlSepArgs S)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int lane = threadIdx.x;
+    const int La = S.lane_x ? V.Nx : V.Ny;
+    const int nlc = (La + 63) >> 6;
+    const int o = blockIdx.x / nlc, la = (blockIdx.x - o * nlc) * 64 + lane;
+    const bool valid = la < La;
+    const int lac = min(la, La - 1);
+    const int ix = S.lane_x ? lac : o, iy = S.lane_x ? o : lac;
+    const int zp = blockIdx.y * ZB;
+    const int ch = blockIdx.z;
+    float tcol;
+    {
+#pragma clang fp contract(off)
+        // the voxel centre's signed distance along the panel normal direction (src/projectors.cu:284)
+        tcol = coord(iy, V.Ty, V.y0) * P.cos_psi - coord(ix, V.Tx, V.x0) * P.sin_psi;
+    }
+    const float x = coord(ix, V.Tx, V.x0), yv = coord(iy, V.Ty, V.y0);
+    const float hz1 = 0.5f * V.Tz / P.Tu1;
+    const int64_t vplane = int64_t(P.Nu1) * P.Nu0;
+    const int rowb = 4 * P.Nu0;  // bytes per u1 row
+    float acc[ZB];
+#pragma unroll
+    for (int r = 0; r < ZB; r++)
+        acc[r] = 0.0f;
+    const int k1 = min(S.nv1, (ch + 1) * S.per);
+    for (int k = ch * S.per; k < k1; k++) {
+        const int iv1 = A.v1_first + k * A.v1_stride;
+        const float v1 = coord(iv1, P.Tv1, P.v1_0);
+        // voxel-centre u1 test per height (src/projectors.cu:284-289)
+        unsigned lbits = 0;
+#pragma unroll
+        for (int r = 0; r < ZB; r++) {
+            bool live;
+            {
+                // unfused, as the reference evaluates it: a voxel on the boundary keeps the reference's side
+#pragma clang fp contract(off)
+                const float u1 = coord(zp + r, V.Tz, V.z0) + v1 * tcol;
+                live = valid && zp + r < V.Nz && !(fabsf(u1) - 0.5f * P.Tu1 > P.H - P.R * fabsf(v1));
+            }
+            lbits |= unsigned(live) << r;
+        }
+        if (!__any(lbits != 0))
+            continue;
+        // the pass's u1 rows qa, qa + 1 and weights, from its first height; height zp + r reads rows qa + r
+        // and qa + r + 1
+        float u1p;
+        {
+#pragma clang fp contract(off)
+            u1p = coord(zp, V.Tz, V.z0) + v1 * tcol;
+        }
+        const float mid = (u1p - P.u1_0) / P.Tu1;
+        const float A1 = mid - hz1, B1 = mid + hz1;
+        const int qa = int(floorf(A1 + 0.5f));
+        const float w1a = fmaxf(0.0f, fminf(float(qa) + 0.5f, B1) - fmaxf(float(qa) - 0.5f, A1));
+        const float w1b = fmaxf(0.0f, fminf(float(qa) + 1.5f, B1) - fmaxf(float(qa) + 0.5f, A1));
+        // every lane's row window inside the view plane: the offsets of the window's rows go in the loads'
+        // scalar offset; otherwise each load reads its row clamped into the plane and rows outside it count
+        // as zero
+        const bool inside = __all(qa >= 0 && qa + ZB < P.Nu1);
+        float a1[ZB];
+#pragma unroll
+        for (int r = 0; r < ZB; r++)
+            a1[r] = 0.0f;
+        const GenView* wr = A.views + int64_t(iv1) * P.Nv0;
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const float4 pc = A.pcoef[iv0];
+            const float l = wr[iv0].lpetl;
+            const float c0 = fmaf(x, pc.x, fmaf(yv, pc.y, pc.z));
+            const float A0 = c0 - pc.w, B0 = c0 + pc.w;
+            const int pa = int(floorf(A0 + 0.5f));
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float*>(g + (int64_t(iv1) * P.Nv0 + iv0) * vplane), 0, int(4 * vplane), GEN_BUFFER_FLAGS);
+#pragma unroll
+            for (int pt = 0; pt < NP; pt++) {
+                const int p = pa + pt;
+                const float w0 = (p >= 0 && p < P.Nu0)
+                                     ? fmaxf(0.0f, fminf(float(p) + 0.5f, B0) - fmaxf(float(p) - 0.5f, A0))
+                                     : 0.0f;
+                const float wl = w0 * l;
+                const float Wa = wl * w1a, Wb = wl * w1b;
+                const int pc4 = 4 * min(max(p, 0), P.Nu0 - 1);
+                float G[ZB + 1];
+                if (inside) {
+                    const int voff = qa * rowb + pc4;
+#pragma unroll
+                    for (int t = 0; t <= ZB; t++)
+                        G[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, t * rowb, 0));
+                } else {
+#pragma unroll
+                    for (int t = 0; t <= ZB; t++) {
+                        const int q = qa + t;
+                        const float v = __builtin_bit_cast(
+                            float, __builtin_amdgcn_raw_buffer_load_b32(rs, min(max(q, 0), P.Nu1 - 1) * rowb + pc4, 0, 0));
+                        G[t] = (q >= 0 && q < P.Nu1) ? v : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < ZB; r++) {
+                    a1[r] = fmaf(G[r], Wa, a1[r]);
+                    a1[r] = fmaf(G[r + 1], Wb, a1[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ZB; r++)
+            if (lbits >> r & 1u)
+                acc[r] += a1[r];
+    }
+    if (!valid)
+        return;
+    float* pc = part + int64_t(ch) * V.Nx * V.Ny * V.Nz;
+#pragma unroll
+    for (int r = 0; r < ZB; r++)
+        if (zp + r < V.Nz)
+            pc[(int64_t(zp + r) * V.Ny + iy) * V.Nx + ix] = acc[r];
+}
+
+// f += T_x * sum over chunks of part[chunk], in chunk order
+__global__ void __launch_bounds__(256) sum_partials_kernel(float* __restrict__ f, const float* __restrict__ part,
+                                                           int nch, int64_t n, float scale)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    float t = 0.0f;
+    for (int c = 0; c < nch; c++)
+        t += part[int64_t(c) * n + i];
+    f[i] += scale * t;
+}
+
 // fpy[z + PZ][y][x + 1] and fpx[z + PZ][x][y + 1] from f[z][y][x], zeros elsewhere: one thread per element
 // of the larger padded plane set, 32 x 32 tiles through LDS for the transposed copy
 __global__ void __launch_bounds__(256) pad_volumes_kernel(const float* __restrict__ f, float* __restrict__ fpy,
@@ -568,7 +718,7 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
         double pa, ps, rs, Ta, S0, Ts, dom_inv;
     };
     std::vector<PerV0> pv(P.Nv0);
-    float max_hb = 0.0f;
+    float max_hb = 0.0f, max_h0 = 0.0f;
     for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
         const float v0f = coord_host(iv0, P.Tv0, P.v0_0);
         // slice axis: the reference's float decision (src/projectors.cu:77)
@@ -605,6 +755,7 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
         const double h0 = ydom ? 0.5 * double(V.Ty) * std::fabs(wy) / Tu0 : 0.5 * double(V.Tx) * std::fabs(wx) / Tu0;
         pcoef[iv0] = make_float4(float(wx / Tu0), float(wy / Tu0), float(-double(P.u0_0) / Tu0), float(h0));
         q.dom_inv = 1.0 / std::max(std::fabs(wy), std::fabs(wx));
+        max_h0 = std::max(max_h0, pcoef[iv0].w);
         for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
             GenView& w = views[size_t(iv1) * P.Nv0 + iv0];
             w.xs = xs;
@@ -616,6 +767,10 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
     }
     gp->sep = gp->Bz == 1.0f && gp->hz <= 0.5f && max_hb <= 0.5f &&
               gen_padded_elems(V) < (size_t(1) << 29);  // byte offsets of the padded copies fit 31 bits
+    // the separable PETL back: one row per voxel height, a u0 interval meeting at most NP cells, view planes
+    // whose byte offsets fit 31 bits
+    if (gp->Bz == 1.0f && size_t(P.Nu1) * P.Nu0 < (size_t(1) << 29))
+        gp->petl_np = max_h0 <= 0.5f ? 2 : (max_h0 <= 1.0f ? 3 : 0);
     for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
         const double v1 = coord_host(iv1, P.Tv1, P.v1_0);
         for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
@@ -698,10 +853,67 @@ void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue
     }
 }
 
-void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first, int v1_stride)
+namespace {
+
+// lanes along x when |sin psi| <= |cos psi|; chunks of v1 rows so that the launch has about 8192 waves, at
+// most 1 GiB of partial volumes
+PetlSepArgs petl_sep_args(const PlanGeom& P, const VolGeom& V, int nv1)
+{
+    PetlSepArgs S;
+    S.lane_x = std::fabs(P.sin_psi) <= std::fabs(P.cos_psi) ? 1 : 0;
+    const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
+    const int64_t waves = int64_t(Lo) * ((La + 63) / 64) * ((V.Nz + PETL_ZB - 1) / PETL_ZB);
+    const int64_t vbytes = int64_t(V.Nx) * V.Ny * V.Nz * 4;
+    int nch = int(std::max<int64_t>(1, (8192 + waves - 1) / waves));
+    nch = int(std::min<int64_t>({int64_t(nch), int64_t(std::max(1, nv1)), int64_t(65535),
+                                 std::max<int64_t>(1, (int64_t(1) << 30) / vbytes)}));
+    S.nv1 = nv1;
+    S.per = (nv1 + nch - 1) / nch;
+    S.nch = (nv1 + S.per - 1) / S.per;
+    return S;
+}
+
+}  // namespace
+
+size_t gen_back_workspace_bytes(const GenPlan& gp)
+{
+    if (!gp.petl_np)
+        return 0;
+    return size_t(petl_sep_args(gp.P, gp.V, gp.P.Nv1).nch) * gp.V.Nx * gp.V.Ny * gp.V.Nz * sizeof(float);
+}
+
+size_t gen_back_workspace_estimate(const PlanGeom& P, const VolGeom& V)
+{
+    if (float(double(P.Tu1) / double(V.Tz)) != 1.0f)
+        return 0;
+    return size_t(petl_sep_args(P, V, P.Nv1).nch) * V.Nx * V.Ny * V.Nz * sizeof(float);
+}
+
+void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first, int v1_stride,
+              float* workspace)
 {
     const GenArgs A = args_of(gp, v1_first, v1_stride);
     const VolGeom& V = gp.V;
+    if (mode != BACK_MATCHED && gp.petl_np && workspace) {
+        const int nv1 = (gp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+        if (nv1 <= 0)
+            return;
+        // a subset has no more v1 rows than the whole planogram, so it needs no more chunks than the
+        // workspace holds
+        const PetlSepArgs S = petl_sep_args(gp.P, V, nv1);
+        const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
+        const dim3 grid(unsigned(Lo * ((La + 63) / 64)), unsigned((V.Nz + PETL_ZB - 1) / PETL_ZB), unsigned(S.nch));
+        if (gp.petl_np == 2)
+            hipLaunchKernelGGL((gen_back_petl_sep_kernel<2, PETL_ZB>), grid, dim3(64), 0, s, g, workspace, A, S);
+        else
+            hipLaunchKernelGGL((gen_back_petl_sep_kernel<3, PETL_ZB>), grid, dim3(64), 0, s, g, workspace, A, S);
+        PETL_LAUNCH_CHECK();
+        const int64_t n = int64_t(V.Nx) * V.Ny * V.Nz;
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, f, workspace, S.nch,
+                           n, V.Tx);
+        PETL_LAUNCH_CHECK();
+        return;
+    }
     if (mode == BACK_MATCHED && gp.sep) {
         hipLaunchKernelGGL(gen_back_matched_sep_kernel, dim3((V.Nx + 63) / 64, (V.Ny + 3) / 4, V.Nz), dim3(256), 0, s,
                            g, f, A);
