@@ -3,10 +3,10 @@
 // projector.  See proj_general.h for the model.
 //
 // Kernels:
-//   gen_fwd_sep_kernel      separable form (see proj_general.h): one wave per view (v1, v0) and tile of 64
-//                           u0 bins (lanes); each lane walks its bin column up the u1 rows in passes of
-//                           GEN_RB rows, per slice reading GEN_RB + 1 z values of each of its two a cells
-//                           from the padded copy and accumulating four taps per row;
+//   gen_fwd_sep_kernel      separable form (see proj_general.h): one wave per view (v1, v0), tile of 64
+//                           u0 bins (lanes) and pass of GEN_RB u1 rows; each lane walks its bin column up
+//                           the pass's rows, per slice reading GEN_RB + 1 z values of each of its two a
+//                           cells from the padded copy and accumulating four taps per row;
 //   gen_back_matched_sep_kernel  its exact transpose, gathered per voxel;
 //   gen_fwd_kernel          a workgroup is one view (v1, v0) x a tile of 64 u0 bins (lanes) x 4 u1 rows
 //                           (waves); each thread walks the slices for its bin, reading the volume through
@@ -163,7 +163,10 @@ __global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, 
     const PlanGeom& P = A.P;
     const VolGeom& V = A.V;
     const int lane = threadIdx.x;
-    const int iu0 = blockIdx.x * 64 + lane;
+    // blockIdx.x = pass * (u0 tiles) + tile: one pass of RB u1 rows per wave
+    const int ntl = (P.Nu0 + 63) >> 6;
+    const int pass = blockIdx.x / ntl;
+    const int iu0 = (blockIdx.x - pass * ntl) * 64 + lane;
     const int iv0 = blockIdx.y, iv1 = A.v1_first + (A.v1_chunk + int(blockIdx.z)) * A.v1_stride;
     const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
     const int Na = w.xs ? V.Ny : V.Nx, Ns = w.xs ? V.Nx : V.Ny, NaP = Na + 2;
@@ -178,7 +181,8 @@ __global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, 
     const float* xr = A.xi0 + int64_t(iv0) * A.Nsmax;
     const float hb = w.hb, hz = A.hz;
     const int64_t grow = (int64_t(iv1) * P.Nv0 + iv0) * P.Nu1 * int64_t(P.Nu0);
-    for (int r0 = 0; r0 < P.Nu1; r0 += RB) {
+    {
+        const int r0 = pass * RB;
         // bit r: row r0 + r exists and is not masked (one scalar word for the pass)
         unsigned rbits = 0;
 #pragma unroll
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, 
             }
         }
         if (iu0 >= P.Nu0)
-            continue;
+            return;  // no barrier follows
 #pragma unroll
         for (int r = 0; r < RB; r++) {
             if (r0 + r >= P.Nu1)
@@ -836,7 +840,8 @@ void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue
             throw hip_error("general forward: the padded volume copies are missing");
         for (int c = 0; c < nv1; c += 65535) {
             A.v1_chunk = c;
-            hipLaunchKernelGGL(gen_fwd_sep_kernel<GEN_RB>, dim3((P.Nu0 + 63) / 64, P.Nv0, std::min(65535, nv1 - c)),
+            hipLaunchKernelGGL(gen_fwd_sep_kernel<GEN_RB>,
+                               dim3(((P.Nu0 + 63) / 64) * ((P.Nu1 + GEN_RB - 1) / GEN_RB), P.Nv0, std::min(65535, nv1 - c)),
                                dim3(64), 0, s, g, vol.fpy, vol.fpx, A, epilogue, y, resp, wts);
             PETL_LAUNCH_CHECK();
         }
