@@ -102,9 +102,9 @@ struct Model {
     // the general forward of planogram i, making the copy of f its kernel reads first unless `made` has it
     void gen_forward_one(size_t i, float* g, const float* f, GenCopies& made, int epilogue, const float* y,
                          const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets);
-    // the separable general PETL back's partial volumes for planogram i (allocated on first use), or null
-    // when its kernel needs none (matched mode, per-voxel kernel)
-    float* gen_back_workspace(size_t i, int mode);
+    // the partial volumes of planogram i's column-driven general back projectors (allocated on first use),
+    // or null when its kernels are the per-voxel ones, which need none
+    float* gen_back_workspace(size_t i);
     // g_i = A_i f for all planograms (device pointers), with an optional epilogue (ratio, or residual with
     // the weights wts; the y, resp, wts lists may be null).  v1 subset (first, stride) restricts the views;
     // (0,1) = all.

@@ -269,9 +269,9 @@ int64_t Model::forward_sumsq_parts(int i, const float* f, const float* resp, con
     return n;
 }
 
-float* Model::gen_back_workspace(size_t i, int mode)
+float* Model::gen_back_workspace(size_t i)
 {
-    const size_t b = mode == BACK_MATCHED ? 0 : gen_back_workspace_bytes(*dev.gen[i]);
+    const size_t b = gen_back_workspace_bytes(*dev.gen[i]);
     if (!b)
         return nullptr;
     dev.genws.ensure(b);
@@ -294,7 +294,7 @@ void Model::back(float* const* g, float* f, int mode, hipStream_t s, int subset,
         if (dev.fast[i])
             fast_back(*dev.fast[i], g[i], f, mode, workspace, s, subset, nsubsets);
         else
-            gen_back(*dev.gen[i], g[i], f, mode, s, subset, nsubsets, own ? gen_back_workspace(i, mode) : nullptr);
+            gen_back(*dev.gen[i], g[i], f, mode, s, subset, nsubsets, own ? gen_back_workspace(i) : nullptr);
     }
 }
 
@@ -335,7 +335,7 @@ void Model::back_batch(float* const* g, float* f, int batch, int mode, hipStream
             continue;
         }
         for (int b = 0; b < batch; b++)
-            gen_back(*dev.gen[i], g[i] + b * G, f + b * nv, mode, s, 0, 1, gen_back_workspace(i, mode));
+            gen_back(*dev.gen[i], g[i] + b * G, f + b * nv, mode, s, 0, 1, gen_back_workspace(i));
     }
 }
 
@@ -368,7 +368,7 @@ void Model::back_st(int st, void* const* g, float* f, int mode, hipStream_t s, i
             throw hip_error("16-bit storage needs the fast kernels (psi 0 or 90 degrees, voxel pitch = crystal pitch)");
         if (!dev.fast[i]) {
             gen_back(*dev.gen[i], static_cast<const float*>(g[i]), f, mode, s, subset, nsubsets,
-                     gen_back_workspace(i, mode));
+                     gen_back_workspace(i));
             continue;
         }
         fast_back_st(*dev.fast[i], st, g[i], f, mode, dev.h2.f(), s, subset, nsubsets, g_scale);

@@ -22,6 +22,10 @@
 // functions on the same table entries, and a tap counts exactly when both overlaps are positive, so the
 // matched back projector is the exact transpose of the forward.
 //
+// With the separable forward, matched mode takes a column-driven form of its transpose too (per column and
+// pass of heights, per view the bins whose two cells include the column, each with a window of rows), which
+// writes per-chunk partial volumes like the PETL form.
+//
 // The separable PETL-mode back projector (petl_np > 0, voxel height = crystal pitch) uses that a voxel's u1
 // interval moves by exactly one row per voxel height: per column it takes the pass's first height's two u1
 // rows and weights for every height of the pass (the voxel-centre test stays per height) and reads a
@@ -67,6 +71,7 @@ struct GenPlan {
     GenView* views = nullptr;   // [Nv1][Nv0]
     float* xi0 = nullptr;       // [Nv0][Nsmax]
     float4* pcoef = nullptr;    // [Nv0] PETL back: {Cx, Cy, C0, h0}: u0 centre = x Cx + y Cy + C0 (bins)
+    int2* rows = nullptr;       // [Nv1] the u1 rows the forward's mask leaves, {first, last} (first > last: none)
     void* block = nullptr;
     ~GenPlan();
 };
@@ -98,8 +103,9 @@ struct GenVolume {
 void gen_forward(const GenPlan& gp, float* g, const GenVolume& vol, int epilogue, const float* y,
                  const float* resp, const float* wts, hipStream_t s, int v1_first = 0, int v1_stride = 1);
 // f += A^T g in mode BACK_PETL or BACK_MATCHED (v1 rows as above).  workspace: at least
-// gen_back_workspace_bytes(gp) bytes for the separable PETL-mode kernel, which writes one partial volume per
-// chunk of v1 rows there and sums them in chunk order; null takes the per-voxel kernel instead.
+// gen_back_workspace_bytes(gp) bytes for the column-driven kernels (separable PETL mode, and matched mode
+// with the separable forward), which write one partial volume per chunk of v1 rows there and sum them in
+// chunk order; null takes the per-voxel kernels instead.
 void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first = 0,
               int v1_stride = 1, float* workspace = nullptr);
 size_t gen_back_workspace_bytes(const GenPlan& gp);

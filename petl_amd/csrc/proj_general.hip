@@ -464,7 +464,103 @@ __global__ void __launch_bounds__(64) gen_back_petl_sep_kernel(const float* __re
             pc[(int64_t(zp + r) * V.Ny + iy) * V.Nx + ix] = acc[r];
 }
 
-// f += T_x * sum over chunks of part[chunk], in chunk order
+// ---------------------------------------------------------------------------------------------------------
+// column-driven matched back projection (the exact transpose of gen_fwd_sep_kernel with the same weights).
+// Lanes and chunks as gen_back_petl_sep_kernel.  Per view and column: the bins iu0 whose two cells include the
+// column's in-plane index (candidates from the inverted crossing, each re-evaluated with the forward's
+// functions), and per such bin the rows a height of the pass takes: height z reads row z - k0 (weight wz0)
+// and row z - k0 - 1 (weight wz1), a window of ZB + 1 rows; rows the forward's u1 mask zeroes count as zero.
+// ---------------------------------------------------------------------------------------------------------
+template <int ZB>
+__global__ void __launch_bounds__(64) gen_back_matched_col_kernel(const float* __restrict__ g,
+                                                                  float* __restrict__ part, GenArgs A,
+                                                                  const int2* __restrict__ mrows, PetlSepArgs S)
+{
+    const PlanGeom& P = A.P;
+    const VolGeom& V = A.V;
+    const int lane = threadIdx.x;
+    const int La = S.lane_x ? V.Nx : V.Ny;
+    const int nlc = (La + 63) >> 6;
+    const int o = blockIdx.x / nlc, la = (blockIdx.x - o * nlc) * 64 + lane;
+    const bool valid = la < La;
+    const int lac = min(la, La - 1);
+    const int ix = S.lane_x ? lac : o, iy = S.lane_x ? o : lac;
+    const int zp = blockIdx.y * ZB;
+    const int ch = blockIdx.z;
+    const float hz = A.hz;
+    const int64_t vplane = int64_t(P.Nu1) * P.Nu0;
+    const int rowb = 4 * P.Nu0;
+    float acc[ZB];
+#pragma unroll
+    for (int r = 0; r < ZB; r++)
+        acc[r] = 0.0f;
+    const int k1 = min(S.nv1, (ch + 1) * S.per);
+    for (int k = ch * S.per; k < k1; k++) {
+        const int iv1 = A.v1_first + k * A.v1_stride;
+        const int2 mr = mrows[iv1];  // unmasked rows of this view row
+        if (mr.x > mr.y)
+            continue;
+        for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
+            const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
+            const int i = w.xs ? iy : ix, s = w.xs ? ix : iy;
+            const float fi = float(i), hb = w.hb;
+            const float xs0 = A.xi0[int64_t(iv0) * A.Nsmax + s];
+            float lo = (fi - hb - 0.5f - xs0) / w.Au, hi = (fi + hb + 0.5f - xs0) / w.Au;
+            if (lo > hi) {
+                const float t = lo;
+                lo = hi;
+                hi = t;
+            }
+            const int b0 = max(0, int(floorf(lo)) - 1), b1 = min(P.Nu0 - 1, int(ceilf(hi)) + 1);
+            const float v0 = coord(iv0, P.Tv0, P.v0_0);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float*>(g + (int64_t(iv1) * P.Nv0 + iv0) * vplane), 0, int(4 * vplane), GEN_BUFFER_FLAGS);
+            float av[ZB];
+#pragma unroll
+            for (int r = 0; r < ZB; r++)
+                av[r] = 0.0f;
+            for (int iu0 = b0; iu0 <= b1; iu0++) {
+                const float xi = cross_a(w, xs0, iu0);
+                const int ta = i - sep_cell(xi, hb);
+                if (!valid || ta < 0 || ta > 1)
+                    continue;
+                const float wa = sep_weight(xi, hb, i);
+                if (!(wa > 0.0f) || mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0))
+                    continue;
+                const float zb = cross_z_base(w, iu0, s);
+                const int k0 = sep_cell(zb, hz);
+                const float W0 = wa * sep_weight(zb, hz, k0), W1 = wa * sep_weight(zb, hz, k0 + 1);
+                // window rows qb ... qb + ZB: height zp + r takes G[r + 1] (weight W0) and G[r] (W1)
+                const int qb = zp - k0 - 1;
+                float G[ZB + 1];
+#pragma unroll
+                for (int t = 0; t <= ZB; t++) {
+                    const int q = qb + t;
+                    const float v = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(rs, min(max(q, 0), P.Nu1 - 1) * rowb + 4 * iu0, 0, 0));
+                    G[t] = (q >= mr.x && q <= mr.y) ? v : 0.0f;
+                }
+#pragma unroll
+                for (int r = 0; r < ZB; r++) {
+                    av[r] += G[r + 1] * W0;
+                    av[r] += G[r] * W1;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < ZB; r++)
+                acc[r] += av[r] * w.scale;
+        }
+    }
+    if (!valid)
+        return;
+    float* pc = part + int64_t(ch) * V.Nx * V.Ny * V.Nz;
+#pragma unroll
+    for (int r = 0; r < ZB; r++)
+        if (zp + r < V.Nz)
+            pc[(int64_t(zp + r) * V.Ny + iy) * V.Nx + ix] = acc[r];
+}
+
+// f += scale * sum over chunks of part[chunk], in chunk order
 __global__ void __launch_bounds__(256) sum_partials_kernel(float* __restrict__ f, const float* __restrict__ part,
                                                            int nch, int64_t n, float scale)
 {
@@ -775,6 +871,22 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
     // whose byte offsets fit 31 bits
     if (gp->Bz == 1.0f && size_t(P.Nu1) * P.Nu0 < (size_t(1) << 29))
         gp->petl_np = max_h0 <= 0.5f ? 2 : (max_h0 <= 1.0f ? 3 : 0);
+    // the rows the forward's u1 mask leaves per view row, evaluated in float as mask_u1 does
+    std::vector<int2> mrows(P.Nv1);
+    for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
+        const float v1f = coord_host(iv1, P.Tv1, P.v1_0);
+        volatile float rv = P.R * std::fabs(v1f);
+        const float thr = P.H - rv;
+        int lo = P.Nu1, hi = -1;
+        for (int iu1 = 0; iu1 < P.Nu1; iu1++) {
+            volatile float d = std::fabs(coord_host(iu1, P.Tu1, P.u1_0)) - 0.5f * P.Tu1;
+            if (!(d > thr)) {
+                lo = std::min(lo, iu1);
+                hi = std::max(hi, iu1);
+            }
+        }
+        mrows[iv1] = make_int2(lo, hi);
+    }
     for (int iv1 = 0; iv1 < P.Nv1; iv1++) {
         const double v1 = coord_host(iv1, P.Tv1, P.v1_0);
         for (int iv0 = 0; iv0 < P.Nv0; iv0++) {
@@ -794,17 +906,19 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V)
         }
     }
     const size_t nv = views.size() * sizeof(GenView), nx = xi0.size() * sizeof(float),
-                 np = pcoef.size() * sizeof(float4);
+                 np = pcoef.size() * sizeof(float4), nr = mrows.size() * sizeof(int2);
     auto al = [](size_t n) { return (n + 255) & ~size_t(255); };
     char* blk = nullptr;
-    PETL_HIP_CHECK(hipMalloc(&blk, al(nv) + al(nx) + al(np)));
+    PETL_HIP_CHECK(hipMalloc(&blk, al(nv) + al(nx) + al(np) + al(nr)));
     gp->block = blk;
     gp->views = reinterpret_cast<GenView*>(blk);
     gp->xi0 = reinterpret_cast<float*>(blk + al(nv));
     gp->pcoef = reinterpret_cast<float4*>(blk + al(nv) + al(nx));
+    gp->rows = reinterpret_cast<int2*>(blk + al(nv) + al(nx) + al(np));
     PETL_HIP_CHECK(hipMemcpy(gp->views, views.data(), nv, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(gp->xi0, xi0.data(), nx, hipMemcpyHostToDevice));
     PETL_HIP_CHECK(hipMemcpy(gp->pcoef, pcoef.data(), np, hipMemcpyHostToDevice));
+    PETL_HIP_CHECK(hipMemcpy(gp->rows, mrows.data(), nr, hipMemcpyHostToDevice));
     return gp;
 }
 
@@ -883,7 +997,7 @@ PetlSepArgs petl_sep_args(const PlanGeom& P, const VolGeom& V, int nv1)
 
 size_t gen_back_workspace_bytes(const GenPlan& gp)
 {
-    if (!gp.petl_np)
+    if (!gp.petl_np && !gp.sep)
         return 0;
     return size_t(petl_sep_args(gp.P, gp.V, gp.P.Nv1).nch) * gp.V.Nx * gp.V.Ny * gp.V.Nz * sizeof(float);
 }
@@ -900,6 +1014,21 @@ void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t
 {
     const GenArgs A = args_of(gp, v1_first, v1_stride);
     const VolGeom& V = gp.V;
+    if (mode == BACK_MATCHED && gp.sep && workspace && gp.P.Nu1 * int64_t(gp.P.Nu0) < (int64_t(1) << 29)) {
+        const int nv1 = (gp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
+        if (nv1 <= 0)
+            return;
+        const PetlSepArgs S = petl_sep_args(gp.P, V, nv1);
+        const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
+        const dim3 grid(unsigned(Lo * ((La + 63) / 64)), unsigned((V.Nz + PETL_ZB - 1) / PETL_ZB), unsigned(S.nch));
+        hipLaunchKernelGGL(gen_back_matched_col_kernel<PETL_ZB>, grid, dim3(64), 0, s, g, workspace, A, gp.rows, S);
+        PETL_LAUNCH_CHECK();
+        const int64_t n = int64_t(V.Nx) * V.Ny * V.Nz;
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, f, workspace, S.nch,
+                           n, 1.0f);
+        PETL_LAUNCH_CHECK();
+        return;
+    }
     if (mode != BACK_MATCHED && gp.petl_np && workspace) {
         const int nv1 = (gp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
         if (nv1 <= 0)
