@@ -333,9 +333,9 @@ __global__ void __launch_bounds__(256) gen_back_matched_sep_kernel(const float* 
 // chunk of v1 rows; part[chunk] receives the chunk's sums (every element of it written), and
 // sum_partials_kernel adds T_x * sum over chunks to f.
 // ---------------------------------------------------------------------------------------------------------
-constexpr int PETL_ZB = 4;  // heights per pass: few accumulators, many waves in flight
+constexpr int PETL_ZB = 4;  // heights per pass of the column kernels: few accumulators, many waves in flight
 
-struct PetlSepArgs {
+struct ColumnArgs {
     int lane_x;   // 1: lanes along x
     int nch;      // chunks of v1 rows
     int per;      // v1 rows (of the subset) per chunk
@@ -344,7 +344,7 @@ struct PetlSepArgs {
 
 template <int NP, int ZB>
 __global__ void __launch_bounds__(64) gen_back_petl_sep_kernel(const float* __restrict__ g, float* __restrict__ part,
-                                                               GenArgs A, PetlSepArgs S)
+                                                               GenArgs A, ColumnArgs S)
 {
     const PlanGeom& P = A.P;
     const VolGeom& V = A.V;
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(64) gen_back_petl_sep_kernel(const float* __re
 template <int ZB>
 __global__ void __launch_bounds__(64) gen_back_matched_col_kernel(const float* __restrict__ g,
                                                                   float* __restrict__ part, GenArgs A,
-                                                                  const int2* __restrict__ mrows, PetlSepArgs S)
+                                                                  const int2* __restrict__ mrows, ColumnArgs S)
 {
     const PlanGeom& P = A.P;
     const VolGeom& V = A.V;
@@ -977,9 +977,9 @@ namespace {
 // lanes along x when |sin psi| <= |cos psi|; chunks of v1 rows so that the launch has about 131072 waves
 // (short waves balance better: 8192 ran the PET/X preset's back projection 1.6x slower), at most 1 GiB of
 // partial volumes
-PetlSepArgs petl_sep_args(const PlanGeom& P, const VolGeom& V, int nv1)
+ColumnArgs column_args(const PlanGeom& P, const VolGeom& V, int nv1)
 {
-    PetlSepArgs S;
+    ColumnArgs S;
     S.lane_x = std::fabs(P.sin_psi) <= std::fabs(P.cos_psi) ? 1 : 0;
     const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
     const int64_t waves = int64_t(Lo) * ((La + 63) / 64) * ((V.Nz + PETL_ZB - 1) / PETL_ZB);
@@ -999,14 +999,14 @@ size_t gen_back_workspace_bytes(const GenPlan& gp)
 {
     if (!gp.petl_np && !gp.sep)
         return 0;
-    return size_t(petl_sep_args(gp.P, gp.V, gp.P.Nv1).nch) * gp.V.Nx * gp.V.Ny * gp.V.Nz * sizeof(float);
+    return size_t(column_args(gp.P, gp.V, gp.P.Nv1).nch) * gp.V.Nx * gp.V.Ny * gp.V.Nz * sizeof(float);
 }
 
 size_t gen_back_workspace_estimate(const PlanGeom& P, const VolGeom& V)
 {
     if (float(double(P.Tu1) / double(V.Tz)) != 1.0f)
         return 0;
-    return size_t(petl_sep_args(P, V, P.Nv1).nch) * V.Nx * V.Ny * V.Nz * sizeof(float);
+    return size_t(column_args(P, V, P.Nv1).nch) * V.Nx * V.Ny * V.Nz * sizeof(float);
 }
 
 void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t s, int v1_first, int v1_stride,
@@ -1018,7 +1018,7 @@ void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t
         const int nv1 = (gp.P.Nv1 - v1_first + v1_stride - 1) / v1_stride;
         if (nv1 <= 0)
             return;
-        const PetlSepArgs S = petl_sep_args(gp.P, V, nv1);
+        const ColumnArgs S = column_args(gp.P, V, nv1);
         const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
         const dim3 grid(unsigned(Lo * ((La + 63) / 64)), unsigned((V.Nz + PETL_ZB - 1) / PETL_ZB), unsigned(S.nch));
         hipLaunchKernelGGL(gen_back_matched_col_kernel<PETL_ZB>, grid, dim3(64), 0, s, g, workspace, A, gp.rows, S);
@@ -1035,7 +1035,7 @@ void gen_back(const GenPlan& gp, const float* g, float* f, int mode, hipStream_t
             return;
         // a subset has no more v1 rows than the whole planogram, so it needs no more chunks than the
         // workspace holds
-        const PetlSepArgs S = petl_sep_args(gp.P, V, nv1);
+        const ColumnArgs S = column_args(gp.P, V, nv1);
         const int La = S.lane_x ? V.Nx : V.Ny, Lo = S.lane_x ? V.Ny : V.Nx;
         const dim3 grid(unsigned(Lo * ((La + 63) / 64)), unsigned((V.Nz + PETL_ZB - 1) / PETL_ZB), unsigned(S.nch));
         if (gp.petl_np == 2)
