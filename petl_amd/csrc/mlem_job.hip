@@ -121,6 +121,7 @@ void MlemJob::allocate()
     DeviceState& D = m.dev;
     m.prepare();
     m.ensure_workspace();
+    m.ensure_general_buffers();  // a captured iteration must find them allocated
     if (prm_.storage != STORE_F32 || prm_.image_storage != STORE_F32)
         for (size_t i = 0; i < m.plans.size(); i++)
             if (!m.uses_fast(int(i)))

@@ -102,6 +102,10 @@ struct Model {
     // the general forward of planogram i, making the copy of f its kernel reads first unless `made` has it
     void gen_forward_one(size_t i, float* g, const float* f, GenCopies& made, int epilogue, const float* y,
                          const float* resp, const float* wts, hipStream_t s, int subset, int nsubsets);
+    // allocates now what the general planograms' projections would allocate on first use (the volume copies
+    // of the forward, the back projectors' partial volumes), so that a later call, e.g. under graph
+    // capture, allocates nothing
+    void ensure_general_buffers();
     // the partial volumes of planogram i's column-driven general back projectors (allocated on first use),
     // or null when its kernels are the per-voxel ones, which need none
     float* gen_back_workspace(size_t i);

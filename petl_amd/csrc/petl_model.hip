@@ -269,6 +269,22 @@ int64_t Model::forward_sumsq_parts(int i, const float* f, const float* resp, con
     return n;
 }
 
+void Model::ensure_general_buffers()
+{
+    prepare();
+    const VolGeom V = vgeom();
+    for (size_t i = 0; i < plans.size(); i++) {
+        if (dev.fast[i])
+            continue;
+        const GenPlan& gp = *dev.gen[i];
+        if (gp.sep)
+            dev.fpad.ensure(2 * gen_padded_elems(V) * sizeof(float));
+        else if (gp.any_xs)
+            dev.fT.ensure(size_t(vol_elems()) * sizeof(float));
+        gen_back_workspace(i);
+    }
+}
+
 float* Model::gen_back_workspace(size_t i)
 {
     const size_t b = gen_back_workspace_bytes(*dev.gen[i]);

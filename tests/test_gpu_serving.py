@@ -22,6 +22,8 @@ GEOS = {
     "p64": dict(psis=(0.0, 90.0), D=200.0, L=128.0, H=128.0, T=2.0, n=64, voxel=2.0),
     "p96": dict(psis=(0.0, 90.0), D=200.0, L=192.0, H=192.0, T=2.0, n=96, voxel=2.0),
     "p128": dict(psis=(0.0, 90.0), D=200.0, L=256.0, H=256.0, T=2.0, n=128, voxel=2.0),
+    # oblique panels: both planograms take the general kernels
+    "o48": dict(psis=(30.0, 120.0), D=200.0, L=96.0, H=96.0, T=2.0, n=48, voxel=2.0),
 }
 
 
@@ -315,3 +317,20 @@ def test_batched_poisson_arrivals_and_pool():
     for g, w in zip(got, want):
         np.testing.assert_array_equal(g, w)
     assert st["batched_requests"] == len(cases) and st["pool_hits"] >= 1
+
+
+def test_general_kernel_requests():
+    """Requests on oblique panels (the general kernels, whose volume copies and partial volumes are allocated
+    before the iteration is captured) beside fast-kernel requests: every image bit-identical to MLEM alone,
+    with and without the RD prior, and the admission count covers the buffers held."""
+    cases = [("o48", 81, 3, {}), ("p48", 82, 3, {}), ("o48", 83, 2, dict(prior="rd", beta=0.01, delta=2.0))]
+    want = [expected(c) for c in cases]
+    with ReconService(max_slots=4) as svc:
+        ids = [svc.submit(**kw(n, data_for(n, sd), it, **ex)) for (n, sd, it, ex) in cases]
+        got = [svc.wait(i)[0] for i in ids]
+        st = svc.stats()
+        counted, held = svc.request_footprint(**kw("o48", data_for("o48", 81), 3))
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert st["completed"] == len(cases) and st["failed"] == 0
+    assert held <= counted
