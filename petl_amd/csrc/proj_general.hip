@@ -505,13 +505,18 @@ __global__ void __launch_bounds__(64) gen_back_matched_col_kernel(const float* _
             const int i = w.xs ? iy : ix, s = w.xs ? ix : iy;
             const float fi = float(i), hb = w.hb;
             const float xs0 = A.xi0[int64_t(iv0) * A.Nsmax + s];
-            float lo = (fi - hb - 0.5f - xs0) / w.Au, hi = (fi + hb + 0.5f - xs0) / w.Au;
+            // candidate bins: the column is one of a bin's two cells when its crossing lies in
+            // [i - 3/2 + hb, i + 1/2 + hb); that interval inverted through an approximate reciprocal, widened
+            // by 0.01 bin on each side (far above the reciprocal's and the crossing's rounding), and every
+            // candidate re-evaluated below with the forward's own functions
+            const float ra = __builtin_amdgcn_rcpf(w.Au);
+            float lo = (fi - 1.5f + hb - xs0) * ra, hi = (fi + 0.5f + hb - xs0) * ra;
             if (lo > hi) {
                 const float t = lo;
                 lo = hi;
                 hi = t;
             }
-            const int b0 = max(0, int(floorf(lo)) - 1), b1 = min(P.Nu0 - 1, int(ceilf(hi)) + 1);
+            const int b0 = max(0, int(ceilf(lo - 0.01f))), b1 = min(P.Nu0 - 1, int(floorf(hi + 0.01f)));
             const float v0 = coord(iv0, P.Tv0, P.v0_0);
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<float*>(g + (int64_t(iv1) * P.Nv0 + iv0) * vplane), 0, int(4 * vplane), GEN_BUFFER_FLAGS);
@@ -519,26 +524,38 @@ __global__ void __launch_bounds__(64) gen_back_matched_col_kernel(const float* _
 #pragma unroll
             for (int r = 0; r < ZB; r++)
                 av[r] = 0.0f;
-            for (int iu0 = b0; iu0 <= b1; iu0++) {
+            // candidates in lockstep (a lane past its range carries zero weights), so that a candidate whose
+            // row window lies inside the unmasked rows for every lane reads with the row step as the loads'
+            // scalar offset
+            for (int c = 0; __any(b0 + c <= b1); c++) {
+                const int iu0 = min(b0 + c, P.Nu0 - 1);
                 const float xi = cross_a(w, xs0, iu0);
                 const int ta = i - sep_cell(xi, hb);
-                if (!valid || ta < 0 || ta > 1)
-                    continue;
                 const float wa = sep_weight(xi, hb, i);
-                if (!(wa > 0.0f) || mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0))
+                const bool ok = valid && b0 + c <= b1 && ta >= 0 && ta <= 1 && wa > 0.0f &&
+                                !mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0);
+                if (!__any(ok))
                     continue;
                 const float zb = cross_z_base(w, iu0, s);
                 const int k0 = sep_cell(zb, hz);
-                const float W0 = wa * sep_weight(zb, hz, k0), W1 = wa * sep_weight(zb, hz, k0 + 1);
+                const float W0 = ok ? wa * sep_weight(zb, hz, k0) : 0.0f;
+                const float W1 = ok ? wa * sep_weight(zb, hz, k0 + 1) : 0.0f;
                 // window rows qb ... qb + ZB: height zp + r takes G[r + 1] (weight W0) and G[r] (W1)
-                const int qb = zp - k0 - 1;
+                const int qb = ok ? zp - k0 - 1 : mr.x;
                 float G[ZB + 1];
+                if (__all(qb >= mr.x && qb + ZB <= mr.y)) {
+                    const int voff = qb * rowb + 4 * iu0;
 #pragma unroll
-                for (int t = 0; t <= ZB; t++) {
-                    const int q = qb + t;
-                    const float v = __builtin_bit_cast(
-                        float, __builtin_amdgcn_raw_buffer_load_b32(rs, min(max(q, 0), P.Nu1 - 1) * rowb + 4 * iu0, 0, 0));
-                    G[t] = (q >= mr.x && q <= mr.y) ? v : 0.0f;
+                    for (int t = 0; t <= ZB; t++)
+                        G[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, t * rowb, 0));
+                } else {
+#pragma unroll
+                    for (int t = 0; t <= ZB; t++) {
+                        const int q = qb + t;
+                        const float v = __builtin_bit_cast(
+                            float, __builtin_amdgcn_raw_buffer_load_b32(rs, min(max(q, 0), P.Nu1 - 1) * rowb + 4 * iu0, 0, 0));
+                        G[t] = (q >= mr.x && q <= mr.y) ? v : 0.0f;
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < ZB; r++) {
