@@ -16,7 +16,8 @@ Prints one JSON line.  Besides the headline rate it carries:
   lds_roofline  the same kernel's LDS read bandwidth (instruction count from the committed SQ counters)
   storage_forms the same pair with bfloat16 / binary16 volume and planograms (fp32 arithmetic)
   geometries    the same pair at the PET/X preset (fast kernels, and the general kernels forced) and at the
-                six-planogram rotating-panel demo geometry (four oblique planograms on the general kernels)
+                six-planogram rotating-panel demo geometry (four oblique planograms on the general kernels),
+                with the matched back projector's call time beside each
   binning       list-mode binning of 16.8M device-resident coincidences into both planograms
   batched_autograd  8 volumes of 64^3 through the autograd projector, forward + matched backward
   response      scintillator stopping power of every bin through 256 box modules (ray tracing)
@@ -465,10 +466,21 @@ def main():
                 eg[2].record()
                 torch.cuda.synchronize()
                 fms, bms = eg[0].elapsed_time(eg[1]) / reps, eg[1].elapsed_time(eg[2]) / reps
+                # the matched back projector (the forward's exact transpose: gradients, RWLS) on the same data
+                sg.set_backprojector_mode("matched")
+                sg.backproject_gpu(gg, fo, stream=stream, sync=False)
+                torch.cuda.synchronize()
+                eg[0].record()
+                for _ in range(reps):
+                    sg.backproject_gpu(gg, fo, stream=stream, sync=False)
+                eg[1].record()
+                torch.cuda.synchronize()
+                mms = eg[0].elapsed_time(eg[1]) / reps
                 res[f"{gname}_{path}"] = {
                     "planograms": [list(x) for x in gsh], "volume": list(vs),
                     "fast_kernels": [sg.uses_fast_path(i) for i in range(len(gsh))],
-                    "forward_call_ms": fms, "back_call_ms": bms, "pairs_per_s": 1e3 / (fms + bms)}
+                    "forward_call_ms": fms, "back_call_ms": bms, "pairs_per_s": 1e3 / (fms + bms),
+                    "matched_back_call_ms": mms}
                 del fg, gg, fo
         out["geometries"] = res
 
