@@ -83,9 +83,9 @@ std::unique_ptr<GenPlan> make_gen_plan(const PlanGeom& P, const VolGeom& V);
 void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_t s);
 
 // The separable forward's zero-padded copies of f: fpy[z + GEN_PZ][y][x + 1] (y planes) and
-// fpx[z + GEN_PZ][x][y + 1] (x planes), each gen_padded_elems(V) floats with GEN_PZ planes of zeros below
-// and above the volume and one zero column on each side of the footprint axis (every element written, no
-// separate clear).
+// fpx[z + GEN_PZ][x][y + 1] (x planes), whose cells hold two floats, f at heights z and z + 1; each copy is
+// gen_padded_elems(V) floats with GEN_PZ planes of zeros below and above the volume and one zero column on
+// each side of the footprint axis (every element written, no separate clear).
 size_t gen_padded_elems(const VolGeom& V);
 void launch_pad_volumes(const float* f, float* fpy, float* fpx, const VolGeom& V, hipStream_t s);
 

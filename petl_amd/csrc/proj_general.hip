@@ -170,11 +170,11 @@ __global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, 
     const int iv0 = blockIdx.y, iv1 = A.v1_first + (A.v1_chunk + int(blockIdx.z)) * A.v1_stride;
     const GenView w = A.views[int64_t(iv1) * P.Nv0 + iv0];
     const int Na = w.xs ? V.Ny : V.Nx, Ns = w.xs ? V.Nx : V.Ny, NaP = Na + 2;
-    const int plane = Ns * NaP;  // one z plane of the padded copy
+    const int plane = Ns * NaP;  // cells of one z plane of the padded copy (two floats each)
     // buffer resource over the copy this view reads (its size in bytes fits 32 bits: the plan checks the
     // padded element count); loads past it would return zero, and every load here stays inside it
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(w.xs ? fpx : fpy), 0, 4 * (V.Nz + 2 * PZ) * plane, GEN_BUFFER_FLAGS);
+        const_cast<float*>(w.xs ? fpx : fpy), 0, 8 * (V.Nz + 2 * PZ) * plane, GEN_BUFFER_FLAGS);
     const float v0 = coord(iv0, P.Tv0, P.v0_0), v1 = coord(iv1, P.Tv1, P.v1_0);
     const bool lane_live = iu0 < P.Nu0 && !mask_u0(P, coord(iu0, P.Tu0, P.u0_0), v0);
     const int iu0c = min(iu0, P.Nu0 - 1);
@@ -219,14 +219,24 @@ __global__ void __launch_bounds__(64) gen_fwd_sep_kernel(float* __restrict__ g, 
                     wz1 = sep_weight(zb, hz, k0 + 1);
                 }
                 const float W00 = wa0 * wz0, W01 = wa0 * wz1, W10 = wa1 * wz0, W11 = wa1 * wz1;
-                // byte offset of the lane's cell (kw, s, i0) in the padded copy; plane t of the window adds
-                // the uniform t * plane floats as the load's scalar offset
-                const int voff = 4 * (((live ? kw : 0) + PZ) * plane + s * NaP + (live ? i0 : 0) + 1);
+                // byte offset of the lane's cell (kw, s, i0) in the padded copy, whose cells hold the values of
+                // two heights: one 16-byte load gives cells i0 and i0 + 1 at heights kw + t and kw + t + 1,
+                // and plane t of the window adds the uniform t * plane cells as the load's scalar offset (9
+                // loads per slice for the 17 heights of a pass instead of 17 of 8 bytes)
+                const int voff = 8 * (((live ? kw : 0) + PZ) * plane + s * NaP + (live ? i0 : 0) + 1);
                 float va[RB + 1], vb[RB + 1];
 #pragma unroll
-                for (int t = 0; t <= RB; t++) {
-                    va[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 4 * t * plane, 0));
-                    vb[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4, 4 * t * plane, 0));
+                for (int t = 0; t <= RB; t += 2) {
+                    // elements taken out as integers first: __builtin_bit_cast of a subscript of this vector
+                    // type compiles to a one-dword load here
+                    const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 8 * t * plane, 0);
+                    const unsigned q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+                    va[t] = __uint_as_float(q0);
+                    vb[t] = __uint_as_float(q2);
+                    if (t + 1 <= RB) {
+                        va[t + 1] = __uint_as_float(q1);
+                        vb[t + 1] = __uint_as_float(q3);
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < RB; r++) {
@@ -590,30 +600,37 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(float* __restrict__ f
     f[i] += scale * t;
 }
 
-// fpy[z + PZ][y][x + 1] and fpx[z + PZ][x][y + 1] from f[z][y][x], zeros elsewhere: one thread per element
-// of the larger padded plane set, 32 x 32 tiles through LDS for the transposed copy
-__global__ void __launch_bounds__(256) pad_volumes_kernel(const float* __restrict__ f, float* __restrict__ fpy,
-                                                          float* __restrict__ fpx, int Nx, int Ny, int Nz)
+// fpy[z + PZ][y][x + 1] and fpx[z + PZ][x][y + 1] from f[z][y][x], each cell the pair (f at z, f at z + 1),
+// zeros elsewhere: one thread per cell of the larger padded plane set, 32 x 32 tiles through LDS for the
+// transposed copy
+__global__ void __launch_bounds__(256) pad_volumes_kernel(const float* __restrict__ f, float2* __restrict__ fpy,
+                                                          float2* __restrict__ fpx, int Nx, int Ny, int Nz)
 {
-    __shared__ float tile[32][33];
+    __shared__ float2 tile[32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     const int zp = blockIdx.z, z = zp - GEN_PZ;  // padded plane
-    const bool zin = z >= 0 && z < Nz;
+    const bool zin0 = z >= 0 && z < Nz, zin1 = z + 1 >= 0 && z + 1 < Nz;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     const int64_t zo = int64_t(z) * Nx * Ny;
+    const float2 zero = make_float2(0.0f, 0.0f);
     // fpy: row y, column x + 1 (columns 0 and Nx + 1 are zero)
     for (int r = ty; r < 32; r += 8) {
         const int x = bx + tx, yy = by + r;
-        float v = 0.0f;
-        if (zin && x < Nx && yy < Ny)
-            v = f[zo + int64_t(yy) * Nx + x];
+        float2 v = zero;
+        if (x < Nx && yy < Ny) {
+            const int64_t o = zo + int64_t(yy) * Nx + x;
+            if (zin0)
+                v.x = f[o];
+            if (zin1)
+                v.y = f[o + int64_t(Nx) * Ny];
+        }
         tile[r][tx] = v;
         if (yy < Ny && x < Nx)
             fpy[(int64_t(zp) * Ny + yy) * (Nx + 2) + x + 1] = v;
         if (yy < Ny && x == 0)
-            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2)] = 0.0f;
+            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2)] = zero;
         if (yy < Ny && x == Nx - 1)
-            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2) + Nx + 1] = 0.0f;
+            fpy[(int64_t(zp) * Ny + yy) * (Nx + 2) + Nx + 1] = zero;
     }
     __syncthreads();
     // fpx: row x, column y + 1 (columns 0 and Ny + 1 are zero)
@@ -622,9 +639,9 @@ __global__ void __launch_bounds__(256) pad_volumes_kernel(const float* __restric
         if (x < Nx && yy < Ny)
             fpx[(int64_t(zp) * Nx + x) * (Ny + 2) + yy + 1] = tile[tx][r];
         if (x < Nx && yy == 0)
-            fpx[(int64_t(zp) * Nx + x) * (Ny + 2)] = 0.0f;
+            fpx[(int64_t(zp) * Nx + x) * (Ny + 2)] = zero;
         if (x < Nx && yy == Ny - 1)
-            fpx[(int64_t(zp) * Nx + x) * (Ny + 2) + Ny + 1] = 0.0f;
+            fpx[(int64_t(zp) * Nx + x) * (Ny + 2) + Ny + 1] = zero;
     }
 }
 
@@ -948,15 +965,14 @@ void launch_transpose_xy(const float* f, float* fT, const VolGeom& V, hipStream_
 
 size_t gen_padded_elems(const VolGeom& V)
 {
-    return size_t(V.Nz + 2 * GEN_PZ) * V.Ny * (V.Nx + 2) > size_t(V.Nz + 2 * GEN_PZ) * V.Nx * (V.Ny + 2)
-               ? size_t(V.Nz + 2 * GEN_PZ) * V.Ny * (V.Nx + 2)
-               : size_t(V.Nz + 2 * GEN_PZ) * V.Nx * (V.Ny + 2);
+    // floats of one copy: two per cell
+    return 2 * std::max(size_t(V.Nz + 2 * GEN_PZ) * V.Ny * (V.Nx + 2), size_t(V.Nz + 2 * GEN_PZ) * V.Nx * (V.Ny + 2));
 }
 
 void launch_pad_volumes(const float* f, float* fpy, float* fpx, const VolGeom& V, hipStream_t s)
 {
     hipLaunchKernelGGL(pad_volumes_kernel, dim3((V.Nx + 31) / 32, (V.Ny + 31) / 32, V.Nz + 2 * GEN_PZ), dim3(256), 0,
-                       s, f, fpy, fpx, V.Nx, V.Ny, V.Nz);
+                       s, f, reinterpret_cast<float2*>(fpy), reinterpret_cast<float2*>(fpx), V.Nx, V.Ny, V.Nz);
     PETL_LAUNCH_CHECK();
 }
 
