@@ -432,6 +432,42 @@ __global__ void __launch_bounds__(64) gen_back_petl_sep_kernel(const float* __re
             const int pa = int(floorf(A0 + 0.5f));
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<float*>(g + (int64_t(iv1) * P.Nv0 + iv0) * vplane), 0, int(4 * vplane), GEN_BUFFER_FLAGS);
+            if (NP == 2 && inside && P.Nu0 >= 2) {
+                // one 8-byte load per row gives two neighbouring cells pl, pl + 1 of the row, pl = pa clamped
+                // to [0, Nu0 - 2]: the footprint (at most one cell wide) meets only cells pa and pa + 1, so a
+                // cell of a clamped pair outside them has no overlap (up to rounding at an exact cell edge),
+                // and where pl = pa the sums are those of the per-cell form below, in the same order
+                const int pl = min(max(pa, 0), P.Nu0 - 2);
+                float Wa[2], Wb[2];
+#pragma unroll
+                for (int pt = 0; pt < 2; pt++) {
+                    const float p = float(pl + pt);
+                    const float wl = fmaxf(0.0f, fminf(p + 0.5f, B0) - fmaxf(p - 0.5f, A0)) * l;
+                    Wa[pt] = wl * w1a;
+                    Wb[pt] = wl * w1b;
+                }
+                const int voff = qa * rowb + 4 * pl;
+                float G0[ZB + 1], G1[ZB + 1];
+#pragma unroll
+                for (int t = 0; t <= ZB; t++) {
+                    // elements taken out as integers (bit casts of this vector type's subscripts miscompile)
+                    const auto q = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, t * rowb, 0);
+                    const unsigned e0 = q[0], e1 = q[1];
+                    G0[t] = __uint_as_float(e0);
+                    G1[t] = __uint_as_float(e1);
+                }
+#pragma unroll
+                for (int r = 0; r < ZB; r++) {
+                    a1[r] = fmaf(G0[r], Wa[0], a1[r]);
+                    a1[r] = fmaf(G0[r + 1], Wb[0], a1[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < ZB; r++) {
+                    a1[r] = fmaf(G1[r], Wa[1], a1[r]);
+                    a1[r] = fmaf(G1[r + 1], Wb[1], a1[r]);
+                }
+                continue;
+            }
 #pragma unroll
             for (int pt = 0; pt < NP; pt++) {
                 const int p = pa + pt;
